@@ -1,0 +1,151 @@
+"""Writes the golden fixtures of this directory from the reference's own code.
+
+Runs only where the reference sources exist (they are compiled into
+oracle/_ref/ by oracle/Makefile); the fixtures it writes are committed so the
+tests can run anywhere.  Every value in a fixture is an output of the
+reference's buildTree / getList / kernels / passes (3d/ and 3dp/ headers).
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+import oracle  # noqa: E402
+from minimal_amd import inputs  # noqa: E402
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, **arrays)
+    print(f"wrote {name}: {os.path.getsize(path)} bytes")
+
+
+def tree_and_lists(ref, bodies, P, ncrit, theta, with_fmm=True, with_lists=True):
+    ref.set(P, ncrit, theta)
+    b = bodies.copy()
+    cells = ref.build_tree(b)
+    out = dict(input=bodies, sorted_xq=np.stack([b["X"][:, 0], b["X"][:, 1], b["X"][:, 2], b["q"]], 1),
+               cells_nchild=cells["nchild"], cells_nbody=cells["nbody"], cells_child=cells["child"],
+               cells_body=cells["body"], cells_X=cells["X"], cells_R=cells["R"])
+    if with_lists:
+        (mo, ms), (po, ps) = ref.lists()
+        out.update(m2l_off=mo.astype(np.int32), m2l_src=ms.astype(np.int32),
+                   p2p_off=po.astype(np.int32), p2p_src=ps.astype(np.int32))
+    if with_fmm:
+        b2 = bodies.copy()
+        ref.fmm(b2)
+        out.update(p=b2["p"], F=b2["F"])
+    return out
+
+
+def main():
+    oracle.build()
+    ref = oracle.Reference()
+    rng = np.random.default_rng(20261202)
+
+    # Config-1 body set (N=1e4, [0,1)^3, q=U[0,1)/N), P=10, theta=0.4, ncrit=64.
+    b = inputs.uniform_cube(10000, 0, "unit_over_n")
+    save("config1_n10000.npz", P=10, ncrit=64, theta=0.4, **tree_and_lists(ref, b, 10, 64, 0.4))
+
+    # Config-2 distribution at a small N (q=U[-1,1)), P=4 and an adaptive
+    # Plummer cluster with background (deep, unbalanced tree).
+    b = inputs.uniform_cube(1000, 7, "sym")
+    save("uniform_n1000_p4.npz", P=4, ncrit=16, theta=0.4, **tree_and_lists(ref, b, 4, 16, 0.4))
+    n = 4096
+    pos = np.concatenate([inputs.plummer(n - 410, 0.05, 0.5, rng), rng.random((410, 3))])
+    pb = np.zeros(n, inputs.BODY_DTYPE)
+    pb["X"] = pos
+    pb["q"] = rng.uniform(-1, 1, n)
+    save("plummer_n4096_p6.npz", P=6, ncrit=32, theta=0.4, **tree_and_lists(ref, pb, 6, 32, 0.4))
+
+    # Driver body set of 3d/fmm.cxx and test/test_fmm.h (X in [-pi,pi)^3,
+    # neutral q), N=1e4, ncrit=64: p, F of the FMM at P=10 and P=20 and the
+    # FP64 direct sum at the 10 stride-sampled targets (test/test_fmm.h:73-104).
+    drv = inputs.reference_driver_bodies(10000, 0)
+    res = {}
+    for P in (10, 20):
+        ref.set(P, 64, 0.4)
+        b2 = drv.copy()
+        ref.fmm(b2)
+        stride = len(b2) // 10
+        tg = b2[np.arange(10) * stride].copy()
+        fmm_p, fmm_F = tg["p"].copy(), tg["F"].copy()
+        tg["p"] = 0
+        tg["F"] = 0
+        ref.direct(tg, b2)
+        perr = np.sqrt(np.sum((tg["p"] - fmm_p) ** 2) / np.sum(fmm_p ** 2))
+        res[f"P{P}_fmm_p"], res[f"P{P}_fmm_F"] = fmm_p, fmm_F
+        res[f"P{P}_dir_p"], res[f"P{P}_dir_F"] = tg["p"].copy(), tg["F"].copy()
+        res[f"P{P}_perr"] = perr
+        print(f"test_fmm({P}) = {perr:.6e}")
+    save("driver_n10000.npz", input=drv, **res)
+
+    # Operator vectors (one reference kernel call each) at P = 4, 10, 14.
+    for P in (4, 10, 14):
+        ref.set(P, 64, 0.4)
+        nt = P * (P + 1) // 2
+        o = {}
+        cb = np.zeros(24, inputs.BODY_DTYPE)
+        c = np.array([0.3, -0.2, 0.1])
+        cb["X"] = c + rng.uniform(-0.25, 0.25, (24, 3))
+        cb["q"] = rng.uniform(-1, 1, 24)
+        o["p2m_center"], o["p2m_bodies"] = c, cb
+        o["p2m_M"] = ref.p2m(c, cb)
+        cp = c + np.array([0.25, -0.25, 0.25])
+        Mc = o["p2m_M"]
+        M0 = rng.normal(size=nt) + 1j * rng.normal(size=nt)
+        o["m2m_parent"], o["m2m_child"], o["m2m_Mchild"], o["m2m_M0"] = cp, c, Mc, M0
+        o["m2m_M"] = ref.m2m(cp, c, Mc, M0)
+        xi = np.array([2.1, 0.7, -1.3])
+        o["m2l_Xi"], o["m2l_Xj"] = xi, cp
+        L0 = rng.normal(size=nt) + 1j * rng.normal(size=nt)
+        o["m2l_L0"] = L0
+        o["m2l_L"] = ref.m2l(xi, cp, o["m2m_M"], L0)
+        xc = xi + np.array([-0.125, 0.125, 0.125])
+        Lc0 = rng.normal(size=nt) + 1j * rng.normal(size=nt)
+        o["l2l_child"], o["l2l_Lc0"] = xc, Lc0
+        o["l2l_L"] = ref.l2l(xi, o["m2l_L"], xc, Lc0)
+        tb = np.zeros(20, inputs.BODY_DTYPE)
+        tb["X"] = xc + rng.uniform(-0.12, 0.12, (20, 3))
+        tb["q"] = 1.0
+        tb["p"] = rng.normal(size=20)
+        tb["F"] = rng.normal(size=(20, 3))
+        o["l2p_bodies_in"] = tb.copy()
+        ref.l2p(xc, o["l2l_L"], tb)
+        o["l2p_bodies_out"] = tb
+        src = np.zeros(40, inputs.BODY_DTYPE)
+        src["X"] = rng.random((40, 3))
+        src["q"] = rng.uniform(-1, 1, 40)
+        tgt = src[:15].copy()
+        tgt["X"][5] += 0.5
+        o["p2p_src"], o["p2p_tgt_in"] = src, tgt.copy()
+        ref.p2p(tgt, src)
+        o["p2p_tgt_out"] = tgt
+        rab = np.array([1.7, 0.9, -2.2])
+        o["harm_rab"] = rab
+        o["eval_local"] = ref.eval_local(*rab)
+        o["eval_multipole"], o["eval_multipole_theta"] = ref.eval_multipole(0.6, 1.1, 0.4)
+        save(f"ops_P{P}.npz", P=P, **o)
+
+    # Periodic (3dp): the shipped driver's set (N=1000, X in [-pi,pi)^3,
+    # neutral q, P=10, ncrit=64, theta=0.4, images=4, cycle=2pi) and the
+    # Ewald sum with its parameters (3dp/fmm.cxx:13-23).
+    refp = oracle.ReferencePeriodic()
+    drv = inputs.reference_driver_bodies(1000, 0)
+    b2 = drv.copy()
+    refp.fmm(b2, 10, 64, 0.4, 4, 2 * np.pi)
+    ew = b2.copy()
+    refp.ewald(ew, 64, 11, 11 / (2 * np.pi), 0.25 / np.pi, np.pi, 2 * np.pi)
+    save("periodic_n1000.npz", input=drv, fmm_sorted=b2, ewald=ew)
+
+
+if __name__ == "__main__":
+    main()
