@@ -1,0 +1,178 @@
+/*
+ * exafmm_mi355x.h -- C ABI of the MI355X 3-D Laplace FMM evaluation path.
+ *
+ * Drop-in boundary for minimal-ExaFMM's 3d/ path: every entry point below
+ * replaces one reference function (cited per declaration) and keeps its
+ * argument meaning; the C++ surface with the reference's own names
+ * (buildTree, upwardPass, horizontalPass, downwardPass, direct, P2P ... L2P)
+ * is include/exafmm_mi355x.hpp, built on these calls.
+ *
+ * Conventions
+ *  - All functions return 0 on success or a negative EXAFMM_ERR_* code; the
+ *    message of the last failure is exafmm_last_error(ctx) (or
+ *    exafmm_last_error(NULL) for failures without a context).  Nothing
+ *    aborts the process.
+ *  - Bodies use the reference's 64-byte layout (3d/exafmm.h:14-19).  A body
+ *    pointer is a host pointer (on_device = 0) or a HIP device pointer
+ *    (on_device = 1).
+ *  - Expansion coefficients are NTERM = P(P+1)/2 complex doubles per cell,
+ *    interleaved (re, im), index n(n+1)/2 + m for 0 <= m <= n < P
+ *    (3d/kernel.h:196-197), cells in the reference's numbering.
+ *  - One context per stream; calls on one context are serialised by the
+ *    caller, different contexts may be used from different threads.
+ */
+#ifndef EXAFMM_MI355X_H
+#define EXAFMM_MI355X_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EXAFMM_ABI_VERSION 1
+
+enum exafmm_status {
+  EXAFMM_OK = 0,
+  EXAFMM_ERR_ARG = -1,      /* invalid argument (N<1, NaN coordinate, bad config) */
+  EXAFMM_ERR_STATE = -2,    /* call out of order (e.g. upward before build_tree) */
+  EXAFMM_ERR_DEPTH = -3,    /* tree deeper than 21 levels (> ncrit coincident bodies) */
+  EXAFMM_ERR_ALLOC = -4,    /* device allocation failed */
+  EXAFMM_ERR_HIP = -5,      /* HIP runtime error */
+  EXAFMM_ERR_CAPACITY = -6, /* a fixed-capacity structure overflowed */
+  EXAFMM_ERR_NODEVICE = -7, /* no HIP device */
+  EXAFMM_ERR_UNSUPPORTED = -8
+};
+
+enum exafmm_m2l_truncation {
+  EXAFMM_M2L_REFERENCE = 0, /* terms j+n <= P-1, as 3d/kernel.h:83-116,233-260 evaluate them */
+  EXAFMM_M2L_FULL = 1       /* every term j+n <= 2P-2 */
+};
+
+enum exafmm_precision {
+  EXAFMM_PREC_FP64 = 0,   /* FP64 everywhere (parity mode) */
+  EXAFMM_PREC_FP32 = 1,   /* FP32 P2P / FP32 MFMA M2L */
+  EXAFMM_PREC_BF16X3 = 2  /* M2L only: 3-term BF16 split products, FP32 accumulate */
+};
+
+enum exafmm_order_policy {
+  EXAFMM_ORDER_FIXED = 0,  /* expansion order P for every M2L pair */
+  EXAFMM_ORDER_LEARNED = 1 /* per-pair order from the order model (exafmm_set_order_model) */
+};
+
+/* Reference Body (3d/exafmm.h:14-19); F is +grad(phi) in the reference's sign. */
+typedef struct exafmm_body {
+  double X[3];
+  double q;
+  double p;
+  double F[3];
+} exafmm_body;
+
+/* Flattened reference Cell (3d/exafmm.h:23-36): pointers become indices. */
+typedef struct exafmm_cell {
+  int32_t nchild;
+  int32_t nbody;
+  int64_t child; /* index of the first child, -1 for a leaf */
+  int64_t body;  /* index of the first body in the permuted body array */
+  double X[3];
+  double R;
+} exafmm_cell;
+
+/* Replaces the mutable globals P, ncrit, theta (3d/exafmm.h:40-45) and the
+ * periodic globals images, cycle (3dp/exafmm.h:45-47). */
+typedef struct exafmm_config {
+  int32_t P;              /* expansion order, 2..16 */
+  int32_t ncrit;          /* max bodies per leaf, 1..1024 */
+  double theta;           /* multipole acceptance criterion */
+  int32_t m2l_truncation; /* enum exafmm_m2l_truncation */
+  int32_t m2l_precision;  /* enum exafmm_precision */
+  int32_t p2p_precision;  /* EXAFMM_PREC_FP64 or EXAFMM_PREC_FP32 */
+  int32_t images;         /* 0: free space; >=1: periodic, 3dp semantics */
+  double cycle;           /* periodic cycle (images >= 1) */
+  int32_t order_policy;   /* enum exafmm_order_policy */
+  int32_t device;         /* HIP device ordinal */
+  int32_t reserved[6];
+} exafmm_config;
+
+typedef struct exafmm_ctx exafmm_ctx;
+
+/* Defaults: P=10, ncrit=64, theta=0.4, full truncation, BF16X3 M2L, FP32
+ * P2P, free space, fixed order, device 0. */
+int exafmm_config_default(exafmm_config *cfg);
+int exafmm_device_count(int *count);
+const char *exafmm_last_error(const exafmm_ctx *ctx);
+
+int exafmm_create(const exafmm_config *cfg, exafmm_ctx **out);
+int exafmm_destroy(exafmm_ctx *ctx);
+int exafmm_synchronize(exafmm_ctx *ctx);
+
+/* buildTree (3d/build_tree.h:88-96): octree of `bodies`; X and q are
+ * permuted in place exactly as the reference permutes them, p and F are
+ * left in place.  The array must stay valid until exafmm_downward. */
+int exafmm_build_tree(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_device);
+int exafmm_num_cells(exafmm_ctx *ctx, int64_t *ncells, int32_t *nlevels);
+/* The cell table in the reference's numbering (children blocks in DFS
+ * preorder of their parents). */
+int exafmm_get_cells(exafmm_ctx *ctx, exafmm_cell *out, int64_t capacity);
+
+/* upwardPass (3d/traverse_eager.h:127-144): P2M at leaves, M2M upward. */
+int exafmm_upward(exafmm_ctx *ctx);
+/* horizontalPass (3d/traverse_eager.h:147-172, lazy form
+ * 3d/traverse_lazy.h:27-56): dual-tree traversal of (target, source)
+ * emitting M2L and P2P lists, then M2L into the target's locals and P2P
+ * into the target's bodies.  source may equal target. */
+int exafmm_horizontal(exafmm_ctx *target, exafmm_ctx *source);
+/* downwardPass (3d/traverse_eager.h:175-190): L2L downward, L2P at
+ * leaves; p and F are accumulated (+=) into the bodies array. */
+int exafmm_downward(exafmm_ctx *ctx);
+/* buildTree + upwardPass + horizontalPass(self) + downwardPass in one call
+ * (3d/fmm.cxx:40-53). */
+int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_device);
+
+/* direct (3d/traverse_eager.h:193-202): FP64 direct sum of every source on
+ * every target, accumulated into the targets' p and F. */
+int exafmm_direct(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources,
+                  int64_t ns, int on_device);
+
+/* Interaction lists of the last horizontal pass (Cell::listM2L /
+ * Cell::listP2P, 3d/exafmm.h:30-33), reference cell numbering, CSR by
+ * target cell (offsets has ncells+1 entries). */
+int exafmm_list_sizes(exafmm_ctx *ctx, int64_t *m2l_pairs, int64_t *p2p_pairs);
+int exafmm_get_lists(exafmm_ctx *ctx, int which, int64_t *offsets, int64_t *sources);
+
+/* Cell::M / Cell::L (3d/exafmm.h:34-35) of every cell, reference numbering. */
+int exafmm_get_multipoles(exafmm_ctx *ctx, double *out);
+int exafmm_get_locals(exafmm_ctx *ctx, double *out);
+
+/* Stage times (ms, device clock) of the last build/upward/horizontal/
+ * downward calls: [tree, lists, p2m+m2m, m2l, p2p, l2l+l2p, total]. */
+int exafmm_stage_times(exafmm_ctx *ctx, double *ms, int capacity);
+
+/* Work of the last horizontal pass, for throughput and roofline figures:
+ * [0] M2L pairs, [1] P2P leaf pairs, [2] P2P body-body interactions
+ * (sum of ni*nj), [3] P2P source bodies read (sum of nj), [4] cells,
+ * [5] leaves, [6] distinct M2L translation keys, [7] deepest level. */
+int exafmm_work_counts(exafmm_ctx *ctx, int64_t *out, int capacity);
+
+/* ---- the kernel.h operator surface, one reference call per entry ------
+ * Host arrays; the work runs on the context's device in FP64 unless the
+ * context's precision fields say otherwise.  Coefficient arrays are
+ * NTERM complex doubles (see conventions).  */
+/* P2P(Ci,Cj) (3d/kernel.h:122-150): targets += interaction with sources. */
+int exafmm_p2p(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources, int64_t ns);
+/* P2M(C) (3d/kernel.h:187-202): M += multipole of bodies about center. */
+int exafmm_p2m(exafmm_ctx *ctx, const double center[3], const exafmm_body *bodies, int64_t nb, double *M);
+/* M2M(Ci) (3d/kernel.h:204-231), one child: Mparent += shift(Mchild). */
+int exafmm_m2m(exafmm_ctx *ctx, const double cparent[3], const double cchild[3], const double *Mchild,
+               double *Mparent);
+/* M2L(Ci,Cj) (3d/kernel.h:233-260): Li += translate(Mj, Xi - Xj). */
+int exafmm_m2l(exafmm_ctx *ctx, const double Xi[3], const double Xj[3], const double *Mj, double *Li);
+/* L2L(Cj) (3d/kernel.h:262-291), one child: Lchild += shift(Lparent). */
+int exafmm_l2l(exafmm_ctx *ctx, const double cparent[3], const double *Lparent, const double cchild[3],
+               double *Lchild);
+/* L2P(Ci) (3d/kernel.h:293-322): bodies p, F += local expansion. */
+int exafmm_l2p(exafmm_ctx *ctx, const double center[3], const double *L, exafmm_body *bodies, int64_t nb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,456 @@
+// api.hip -- the C ABI (include/exafmm_mi355x.h): context lifetime, the
+// pass-level drop-in for buildTree / upwardPass / horizontalPass /
+// downwardPass / direct, exports in the reference's numbering, and stage
+// timing with HIP events.
+#include <algorithm>
+#include <exception>
+
+#include "fmm_internal.h"
+
+struct exafmm_ctx {
+  xf::Ctx c;
+};
+
+namespace xf {
+
+void fail(int code, const std::string &msg) { throw Failure{code, msg}; }
+
+namespace {
+
+thread_local std::string g_error;
+
+template <class F>
+int guard(exafmm_ctx *ctx, F &&f) {
+  try {
+    if (ctx) XF_HIP(hipSetDevice(ctx->c.cfg.device));
+    f();
+    return EXAFMM_OK;
+  } catch (const Failure &e) {
+    (ctx ? ctx->c.last_error : g_error) = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    (ctx ? ctx->c.last_error : g_error) = "host allocation failed";
+    return EXAFMM_ERR_ALLOC;
+  } catch (const std::exception &e) {
+    (ctx ? ctx->c.last_error : g_error) = e.what();
+    return EXAFMM_ERR_ARG;
+  }
+}
+
+void check_config(const exafmm_config &cfg) {
+  if (cfg.P < 2 || cfg.P > kMaxP) fail(EXAFMM_ERR_ARG, "P must be in [2, 16]");
+  if (cfg.ncrit < 1 || cfg.ncrit > 1024) fail(EXAFMM_ERR_ARG, "ncrit must be in [1, 1024]");
+  if (!(cfg.theta > 0) || !(cfg.theta < 1.5)) fail(EXAFMM_ERR_ARG, "theta must be in (0, 1.5)");
+  if (cfg.m2l_truncation != EXAFMM_M2L_REFERENCE && cfg.m2l_truncation != EXAFMM_M2L_FULL)
+    fail(EXAFMM_ERR_ARG, "bad m2l_truncation");
+  if (cfg.m2l_precision < EXAFMM_PREC_FP64 || cfg.m2l_precision > EXAFMM_PREC_BF16X3)
+    fail(EXAFMM_ERR_ARG, "bad m2l_precision");
+  if (cfg.p2p_precision != EXAFMM_PREC_FP64 && cfg.p2p_precision != EXAFMM_PREC_FP32)
+    fail(EXAFMM_ERR_ARG, "bad p2p_precision");
+  if (cfg.images < 0 || cfg.images > 8) fail(EXAFMM_ERR_ARG, "images must be in [0, 8]");
+  if (cfg.images > 0 && !(cfg.cycle > 0)) fail(EXAFMM_ERR_ARG, "periodic cycle must be positive");
+  if (cfg.order_policy != EXAFMM_ORDER_FIXED && cfg.order_policy != EXAFMM_ORDER_LEARNED)
+    fail(EXAFMM_ERR_ARG, "bad order_policy");
+}
+
+void rec(Ctx &c, int i, hipStream_t s) { XF_HIP(hipEventRecord(c.ev[i], s)); }
+
+float elapsed(Ctx &c, int a, int b) {
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, c.ev[a], c.ev[b]) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return ms;
+}
+
+void set_bodies(Ctx &c, exafmm_body *b, int64_t n, int on_device) {
+  if (!b || n < 1) fail(EXAFMM_ERR_ARG, "need at least one body");
+  c.t.n = n;
+  if (on_device) {
+    c.bodies = b;
+    c.host_mode = false;
+    c.host_bodies = nullptr;
+  } else {
+    c.mirror.ensure(n);
+    XF_HIP(hipMemcpyAsync(c.mirror.p, b, sizeof(exafmm_body) * n, hipMemcpyHostToDevice, c.s0));
+    c.bodies = c.mirror.p;
+    c.host_mode = true;
+    c.host_bodies = b;
+  }
+}
+
+void bodies_to_host(Ctx &c) {
+  if (!c.host_mode) return;
+  XF_HIP(hipMemcpyAsync(c.host_bodies, c.bodies, sizeof(exafmm_body) * c.t.n, hipMemcpyDeviceToHost, c.s0));
+  XF_HIP(hipStreamSynchronize(c.s0));
+}
+
+// event slots: 0/1 build, 2/3 upward, 11/12 traversal, 9/4 M2L (s0),
+// 5/6 P2P (s1), 10/8 downward
+void do_build(Ctx &c) {
+  rec(c, 0, c.s0);
+  tree_build(c);
+  rec(c, 1, c.s0);
+  c.have_M = c.have_L = false;
+  c.lists.valid = false;
+}
+
+void do_upward(Ctx &c) {
+  if (!c.t.built) fail(EXAFMM_ERR_STATE, "upward pass before build_tree");
+  rec(c, 2, c.s0);
+  upward(c);
+  rec(c, 3, c.s0);
+}
+
+void do_horizontal(Ctx &tc, const Ctx &sc) {
+  if (!tc.t.built || !sc.t.built) fail(EXAFMM_ERR_STATE, "horizontal pass before build_tree");
+  if (!sc.have_M) fail(EXAFMM_ERR_STATE, "horizontal pass before the source's upward pass");
+  if (tc.P != sc.P) fail(EXAFMM_ERR_ARG, "target and source contexts differ in P");
+  if (tc.cfg.images != 0 || sc.cfg.images != 0) fail(EXAFMM_ERR_UNSUPPORTED, "periodic horizontal pass");
+  rec(tc, 11, tc.s0);
+  traverse(tc, sc);
+  rec(tc, 12, tc.s0);
+  tc.L.ensure(size_t(tc.t.ncells) * tc.NT);
+  XF_HIP(hipMemsetAsync(tc.L.p, 0, sizeof(double2) * tc.t.ncells * tc.NT, tc.s0));
+  // P2P on the second stream, concurrent with M2L (VALU vs MFMA work)
+  XF_HIP(hipStreamWaitEvent(tc.s1, tc.ev[12], 0));
+  rec(tc, 5, tc.s1);
+  p2p_apply(tc, sc, tc.s1);
+  rec(tc, 6, tc.s1);
+  rec(tc, 9, tc.s0);
+  m2l_apply(tc, sc);
+  rec(tc, 4, tc.s0);
+  XF_HIP(hipStreamWaitEvent(tc.s0, tc.ev[6], 0));
+  tc.src = &sc;
+  tc.have_L = true;
+}
+
+void do_downward(Ctx &c) {
+  if (!c.have_L) fail(EXAFMM_ERR_STATE, "downward pass before horizontal pass");
+  rec(c, 10, c.s0);
+  l2l_pass(c);
+  l2p_pass(c);
+  rec(c, 8, c.s0);
+}
+
+}  // namespace
+}  // namespace xf
+
+using namespace xf;
+
+extern "C" {
+
+int exafmm_config_default(exafmm_config *cfg) {
+  if (!cfg) return EXAFMM_ERR_ARG;
+  std::memset(cfg, 0, sizeof(*cfg));
+  cfg->P = 10;
+  cfg->ncrit = 64;
+  cfg->theta = 0.4;
+  cfg->m2l_truncation = EXAFMM_M2L_FULL;
+  cfg->m2l_precision = EXAFMM_PREC_BF16X3;
+  cfg->p2p_precision = EXAFMM_PREC_FP32;
+  cfg->images = 0;
+  cfg->cycle = 0;
+  cfg->order_policy = EXAFMM_ORDER_FIXED;
+  cfg->device = 0;
+  return EXAFMM_OK;
+}
+
+int exafmm_device_count(int *count) {
+  return guard(nullptr, [&] {
+    if (!count) fail(EXAFMM_ERR_ARG, "null count");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+      (void)hipGetLastError();
+      n = 0;
+    }
+    *count = n;
+  });
+}
+
+const char *exafmm_last_error(const exafmm_ctx *ctx) { return ctx ? ctx->c.last_error.c_str() : g_error.c_str(); }
+
+int exafmm_create(const exafmm_config *cfg, exafmm_ctx **out) {
+  return guard(nullptr, [&] {
+    if (!cfg || !out) fail(EXAFMM_ERR_ARG, "null argument");
+    *out = nullptr;
+    check_config(*cfg);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+      (void)hipGetLastError();
+      fail(EXAFMM_ERR_NODEVICE, "no HIP device");
+    }
+    if (cfg->device < 0 || cfg->device >= ndev) fail(EXAFMM_ERR_ARG, "device ordinal out of range");
+    XF_HIP(hipSetDevice(cfg->device));
+    auto *h = new exafmm_ctx();
+    Ctx &c = h->c;
+    c.cfg = *cfg;
+    c.P = cfg->P;
+    c.NT = cfg->P * (cfg->P + 1) / 2;
+    try {
+      XF_HIP(hipStreamCreateWithFlags(&c.s0, hipStreamNonBlocking));
+      XF_HIP(hipStreamCreateWithFlags(&c.s1, hipStreamNonBlocking));
+      for (auto &e : c.ev) XF_HIP(hipEventCreate(&e));
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+int exafmm_destroy(exafmm_ctx *ctx) {
+  if (!ctx) return EXAFMM_ERR_ARG;
+  hipSetDevice(ctx->c.cfg.device);
+  if (ctx->c.s0) hipStreamSynchronize(ctx->c.s0);
+  if (ctx->c.s1) hipStreamSynchronize(ctx->c.s1);
+  for (auto &e : ctx->c.ev)
+    if (e) hipEventDestroy(e);
+  if (ctx->c.s0) hipStreamDestroy(ctx->c.s0);
+  if (ctx->c.s1) hipStreamDestroy(ctx->c.s1);
+  delete ctx;
+  return EXAFMM_OK;
+}
+
+int exafmm_synchronize(exafmm_ctx *ctx) {
+  return guard(ctx, [&] {
+    if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
+    XF_HIP(hipStreamSynchronize(ctx->c.s1));
+    XF_HIP(hipStreamSynchronize(ctx->c.s0));
+  });
+}
+
+int exafmm_build_tree(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_device) {
+  return guard(ctx, [&] {
+    if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    set_bodies(c, bodies, n, on_device);
+    do_build(c);
+    bodies_to_host(c);
+  });
+}
+
+int exafmm_num_cells(exafmm_ctx *ctx, int64_t *ncells, int32_t *nlevels) {
+  return guard(ctx, [&] {
+    if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
+    if (!ctx->c.t.built) fail(EXAFMM_ERR_STATE, "no tree");
+    if (ncells) *ncells = ctx->c.t.ncells;
+    if (nlevels) *nlevels = ctx->c.t.depth + 1;
+  });
+}
+
+int exafmm_get_cells(exafmm_ctx *ctx, exafmm_cell *out, int64_t capacity) {
+  return guard(ctx, [&] {
+    if (!ctx || !out) fail(EXAFMM_ERR_ARG, "null argument");
+    if (!ctx->c.t.built) fail(EXAFMM_ERR_STATE, "no tree");
+    if (capacity < ctx->c.t.ncells) fail(EXAFMM_ERR_ARG, "capacity below the number of cells");
+    XF_HIP(hipStreamSynchronize(ctx->c.s0));
+    tree_export_cells(ctx->c, out);
+  });
+}
+
+int exafmm_upward(exafmm_ctx *ctx) {
+  return guard(ctx, [&] {
+    if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
+    do_upward(ctx->c);
+  });
+}
+
+int exafmm_horizontal(exafmm_ctx *target, exafmm_ctx *source) {
+  return guard(target, [&] {
+    if (!target || !source) fail(EXAFMM_ERR_ARG, "null context");
+    if (source->c.cfg.device != target->c.cfg.device) fail(EXAFMM_ERR_ARG, "contexts on different devices");
+    if (source != target) XF_HIP(hipStreamSynchronize(source->c.s0));
+    do_horizontal(target->c, source->c);
+  });
+}
+
+int exafmm_downward(exafmm_ctx *ctx) {
+  return guard(ctx, [&] {
+    if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
+    do_downward(ctx->c);
+    bodies_to_host(ctx->c);
+  });
+}
+
+int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_device) {
+  return guard(ctx, [&] {
+    if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    set_bodies(c, bodies, n, on_device);
+    do_build(c);
+    do_upward(c);
+    do_horizontal(c, c);
+    do_downward(c);
+    bodies_to_host(c);
+  });
+}
+
+int exafmm_stage_times(exafmm_ctx *ctx, double *ms, int capacity) {
+  return guard(ctx, [&] {
+    if (!ctx || !ms) fail(EXAFMM_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    XF_HIP(hipStreamSynchronize(c.s1));
+    XF_HIP(hipStreamSynchronize(c.s0));
+    const double v[8] = {elapsed(c, 0, 1), elapsed(c, 11, 12), elapsed(c, 2, 3), elapsed(c, 9, 4),
+                         elapsed(c, 5, 6), elapsed(c, 10, 8), elapsed(c, 0, 8), 0};
+    for (int i = 0; i < std::min(capacity, 8); i++) ms[i] = v[i];
+  });
+}
+
+int exafmm_direct(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources, int64_t ns,
+                  int on_device) {
+  return guard(ctx, [&] {
+    if (!ctx || !targets || !sources || nt < 0 || ns < 0) fail(EXAFMM_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    if (on_device) {
+      direct_sum(c.cfg.device, c.s0, targets, nt, sources, ns);
+      XF_HIP(hipStreamSynchronize(c.s0));
+    } else {
+      op_p2p(c, targets, nt, sources, ns);
+    }
+  });
+}
+
+int exafmm_list_sizes(exafmm_ctx *ctx, int64_t *m2l_pairs, int64_t *p2p_pairs) {
+  return guard(ctx, [&] {
+    if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
+    if (!ctx->c.lists.valid) fail(EXAFMM_ERR_STATE, "no lists");
+    if (m2l_pairs) *m2l_pairs = ctx->c.lists.nm2l;
+    if (p2p_pairs) *p2p_pairs = ctx->c.lists.np2p;
+  });
+}
+
+int exafmm_get_lists(exafmm_ctx *ctx, int which, int64_t *offsets, int64_t *sources) {
+  return guard(ctx, [&] {
+    if (!ctx || !offsets || !sources) fail(EXAFMM_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    if (!c.lists.valid) fail(EXAFMM_ERR_STATE, "no lists");
+    XF_HIP(hipStreamSynchronize(c.s0));
+    const int64_t nc = c.t.ncells;
+    std::vector<int64_t> b2r, r2b, sb2r, sr2b;
+    tree_ref_order(c, b2r, r2b);
+    const Ctx &sc = c.src ? *c.src : c;
+    if (&sc == &c) {
+      sb2r = b2r;
+    } else {
+      tree_ref_order(const_cast<Ctx &>(sc), sb2r, sr2b);
+    }
+    std::vector<int> beg(nc), cnt(nc);
+    const Lists &L = c.lists;
+    XF_HIP(hipMemcpy(beg.data(), which == 0 ? L.m2l_beg.p : L.p2p_beg.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    XF_HIP(hipMemcpy(cnt.data(), which == 0 ? L.m2l_cnt.p : L.p2p_cnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    const int64_t tot = which == 0 ? L.nm2l : L.np2p;
+    std::vector<int> src(std::max<int64_t>(tot, 1));
+    if (which == 0) {
+      std::vector<int2> e(std::max<int64_t>(tot, 1));
+      if (tot) XF_HIP(hipMemcpy(e.data(), L.m2l.p, sizeof(int2) * tot, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < tot; i++) src[i] = e[i].x;
+    } else if (tot) {
+      XF_HIP(hipMemcpy(src.data(), L.p2p.p, sizeof(int) * tot, hipMemcpyDeviceToHost));
+    }
+    int64_t o = 0;
+    for (int64_t r = 0; r < nc; r++) {
+      offsets[r] = o;
+      const int64_t b = r2b[r];
+      for (int k = 0; k < cnt[b]; k++) sources[o++] = sb2r[src[beg[b] + k]];
+    }
+    offsets[nc] = o;
+  });
+}
+
+int exafmm_work_counts(exafmm_ctx *ctx, int64_t *out, int capacity) {
+  return guard(ctx, [&] {
+    if (!ctx || !out) fail(EXAFMM_ERR_ARG, "null argument");
+    Ctx &c = ctx->c;
+    if (!c.lists.valid) fail(EXAFMM_ERR_STATE, "no lists");
+    XF_HIP(hipStreamSynchronize(c.s1));
+    XF_HIP(hipStreamSynchronize(c.s0));
+    const Ctx &sc = c.src ? *c.src : c;
+    const int64_t nc = c.t.ncells, snc = sc.t.ncells;
+    std::vector<int> beg(nc), cnt(nc), ccnt(nc), sccnt(snc), src(std::max<int64_t>(c.lists.np2p, 1));
+    XF_HIP(hipMemcpy(beg.data(), c.lists.p2p_beg.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    XF_HIP(hipMemcpy(cnt.data(), c.lists.p2p_cnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    XF_HIP(hipMemcpy(ccnt.data(), c.t.ccnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+    XF_HIP(hipMemcpy(sccnt.data(), sc.t.ccnt.p, sizeof(int) * snc, hipMemcpyDeviceToHost));
+    if (c.lists.np2p) XF_HIP(hipMemcpy(src.data(), c.lists.p2p.p, sizeof(int) * c.lists.np2p, hipMemcpyDeviceToHost));
+    int64_t inter = 0, nj = 0;
+    for (int64_t i = 0; i < nc; i++)
+      for (int k = 0; k < cnt[i]; k++) {
+        const int64_t s = sccnt[src[beg[i] + k]];
+        inter += int64_t(ccnt[i]) * s;
+        nj += s;
+      }
+    const int64_t v[8] = {c.lists.nm2l, c.lists.np2p, inter, nj, nc, c.t.nleaves, c.lists.nkeys, c.t.depth};
+    for (int i = 0; i < std::min(capacity, 8); i++) out[i] = v[i];
+  });
+}
+
+static void export_coef(exafmm_ctx *ctx, const Buf<double2> &B, double *out) {
+  Ctx &c = ctx->c;
+  XF_HIP(hipStreamSynchronize(c.s1));
+  XF_HIP(hipStreamSynchronize(c.s0));
+  const int64_t nc = c.t.ncells;
+  std::vector<int64_t> b2r, r2b;
+  tree_ref_order(c, b2r, r2b);
+  std::vector<double2> h(size_t(nc) * c.NT);
+  XF_HIP(hipMemcpy(h.data(), B.p, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
+  for (int64_t b = 0; b < nc; b++)
+    std::memcpy(out + 2 * c.NT * b2r[b], h.data() + size_t(b) * c.NT, sizeof(double2) * c.NT);
+}
+
+int exafmm_get_multipoles(exafmm_ctx *ctx, double *out) {
+  return guard(ctx, [&] {
+    if (!ctx || !out) fail(EXAFMM_ERR_ARG, "null argument");
+    if (!ctx->c.have_M) fail(EXAFMM_ERR_STATE, "no multipoles");
+    export_coef(ctx, ctx->c.M, out);
+  });
+}
+
+int exafmm_get_locals(exafmm_ctx *ctx, double *out) {
+  return guard(ctx, [&] {
+    if (!ctx || !out) fail(EXAFMM_ERR_ARG, "null argument");
+    if (!ctx->c.have_L) fail(EXAFMM_ERR_STATE, "no locals");
+    export_coef(ctx, ctx->c.L, out);
+  });
+}
+
+int exafmm_p2p(exafmm_ctx *ctx, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns) {
+  return guard(ctx, [&] {
+    if (!ctx || !t || !s || nt < 0 || ns < 0) fail(EXAFMM_ERR_ARG, "bad argument");
+    op_p2p(ctx->c, t, nt, s, ns);
+  });
+}
+int exafmm_p2m(exafmm_ctx *ctx, const double center[3], const exafmm_body *b, int64_t nb, double *M) {
+  return guard(ctx, [&] {
+    if (!ctx || !center || !b || !M || nb < 0) fail(EXAFMM_ERR_ARG, "bad argument");
+    op_p2m(ctx->c, center, b, nb, M);
+  });
+}
+int exafmm_m2m(exafmm_ctx *ctx, const double cp[3], const double cc[3], const double *Mc, double *Mp) {
+  return guard(ctx, [&] {
+    if (!ctx || !cp || !cc || !Mc || !Mp) fail(EXAFMM_ERR_ARG, "bad argument");
+    op_m2m(ctx->c, cp, cc, Mc, Mp);
+  });
+}
+int exafmm_m2l(exafmm_ctx *ctx, const double Xi[3], const double Xj[3], const double *Mj, double *Li) {
+  return guard(ctx, [&] {
+    if (!ctx || !Xi || !Xj || !Mj || !Li) fail(EXAFMM_ERR_ARG, "bad argument");
+    op_m2l(ctx->c, Xi, Xj, Mj, Li);
+  });
+}
+int exafmm_l2l(exafmm_ctx *ctx, const double cp[3], const double *Lp, const double cc[3], double *Lc) {
+  return guard(ctx, [&] {
+    if (!ctx || !cp || !cc || !Lp || !Lc) fail(EXAFMM_ERR_ARG, "bad argument");
+    op_l2l(ctx->c, cp, Lp, cc, Lc);
+  });
+}
+int exafmm_l2p(exafmm_ctx *ctx, const double center[3], const double *L, exafmm_body *b, int64_t nb) {
+  return guard(ctx, [&] {
+    if (!ctx || !center || !L || !b || nb < 0) fail(EXAFMM_ERR_ARG, "bad argument");
+    op_l2p(ctx->c, center, L, b, nb);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,252 @@
+// fmm_internal.h -- shared host/device declarations of the MI355X FMM library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "exafmm_mi355x.h"
+
+namespace xf {
+
+constexpr int kMaxLevel = 21;     // 63-bit octant keys
+constexpr int kMaxP = 16;         // expansion order cap (NTERM <= 136)
+constexpr int kWave = 64;
+
+// ------------------------------------------------------------------ errors
+struct Failure {
+  int code;
+  std::string msg;
+};
+[[noreturn]] void fail(int code, const std::string &msg);
+
+#define XF_HIP(expr)                                                                       \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      ::xf::fail(EXAFMM_ERR_HIP, std::string(#expr) + " -> " + hipGetErrorString(e_));     \
+  } while (0)
+
+#define XF_LAUNCH()                                                                        \
+  do {                                                                                     \
+    hipError_t e_ = hipGetLastError();                                                     \
+    if (e_ != hipSuccess)                                                                  \
+      ::xf::fail(EXAFMM_ERR_HIP, std::string("kernel launch at ") + __FILE__ + ":" +       \
+                                     std::to_string(__LINE__) + " -> " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ device buffer
+// Grow-only device allocation: kept across evaluations so the timed path
+// performs no allocation once the sizes have been seen.
+template <class T>
+struct Buf {
+  T *p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t count) {
+    if (count <= cap && p) return;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      fail(EXAFMM_ERR_ALLOC, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    cap = count ? count : 1;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  ~Buf() { release(); }
+  Buf() = default;
+  Buf(const Buf &) = delete;
+  Buf &operator=(const Buf &) = delete;
+};
+
+inline int ceil_div(int64_t a, int64_t b) { return int((a + b - 1) / b); }
+
+// ------------------------------------------------------------------ context
+// Device-side tree in breadth-first (level-major) order; children of a
+// cell are contiguous and in octant order, so each level is a Morton-
+// ordered slice.  The reference (DFS-block) numbering is produced on
+// export only.
+struct Tree {
+  int64_t n = 0;
+  int depth = 0;                 // deepest level index
+  int64_t ncells = 0, nleaves = 0;
+  std::vector<int> lstart;       // host copy: level start, size depth+2
+  double X0[3] = {0, 0, 0}, R0 = 0;
+
+  Buf<double4> cx;     // centre xyz, radius R
+  Buf<int> cbeg, ccnt; // body range in tree order
+  Buf<int> cchild, cnch, cparent, clevel;
+  Buf<int> leaves;     // leaf cells, BFS order
+  Buf<int> perm;       // tree-order position -> caller's body index
+  Buf<double4> xs;     // tree-order X, q (FP64)
+  Buf<float4> xq;      // tree-order X - leaf centre (FP32), q
+  Buf<int> body_leaf;  // tree-order position -> leaf cell
+
+  // build scratch
+  Buf<unsigned long long> key, key2;
+  Buf<int> val, val2;
+  Buf<int> hist;       // dense prefix histograms, levels 1..7
+  Buf<int> lcount, lstart_d, tmp_nch, tmp_off, tmp_bnd, misc;
+  Buf<double> bounds_part;
+  Buf<int> rs_hist;
+  bool built = false;
+};
+
+struct Lists {
+  bool valid = false;
+  Buf<int> m2l_beg, m2l_cnt;  // per target cell
+  Buf<int2> m2l;              // (source cell, key slot)
+  Buf<int> p2p_beg, p2p_cnt;  // per target cell
+  Buf<int> p2p;               // source cell
+  Buf<int> out_beg, out_cnt;  // traversal hand-down lists
+  Buf<int> out;
+  Buf<int> tot;               // device totals [m2l, p2p, out], overflow flags
+  int64_t cap_m2l = 0, cap_p2p = 0, cap_out = 0;
+  int64_t nm2l = 0, np2p = 0;
+  // translation keys
+  Buf<unsigned long long> hslots;  // open-addressing key set
+  int hcap = 0;
+  Buf<int> slot2id;
+  Buf<unsigned long long> keys;    // distinct keys, ascending
+  Buf<int> kscratch;
+  int nkeys = 0;
+};
+
+struct Ctx {
+  exafmm_config cfg{};
+  int P = 10, NT = 55;
+  hipStream_t s0 = nullptr, s1 = nullptr;
+  hipEvent_t ev[16] = {};
+  std::string last_error;
+
+  exafmm_body *bodies = nullptr;   // caller's array (device pointer or our mirror)
+  exafmm_body *host_bodies = nullptr;
+  bool host_mode = false;
+  Buf<exafmm_body> mirror;         // device copy when the caller passed host memory
+
+  Tree t;
+  Lists lists;
+  const Ctx *src = nullptr;        // source tree of the last horizontal pass
+  Buf<double2> M, L;               // FP64 complex [ncells][NT]
+  bool have_M = false, have_L = false;
+
+  // M2L fast path
+  Buf<float> ops32;                // [nkeys][P^2][P^2] scaled real-form operators
+  Buf<float> mt32;                 // scaled multipoles, real form [ncells][P^2]
+  Buf<float> lt32;                 // scaled local accumulators [ncells][P^2]
+  Buf<int> tiles;                  // M2L tile descriptors
+  Buf<int> tile_mat;               // per tile [U][kTileN] source indices
+  Buf<int> tile_keys;
+  Buf<int> misc;
+
+  float stage_ms[8] = {};
+};
+
+// ------------------------------------------------------------------ stages
+void tree_build(Ctx &c);
+void tree_export_cells(Ctx &c, exafmm_cell *out);     // reference numbering
+void tree_ref_order(Ctx &c, std::vector<int64_t> &bfs2ref, std::vector<int64_t> &ref2bfs);
+void traverse(Ctx &tgt, const Ctx &src);
+void upward(Ctx &c);
+void m2l_apply(Ctx &tgt, const Ctx &src);
+void p2p_apply(Ctx &tgt, const Ctx &src, hipStream_t s);
+void l2l_pass(Ctx &c);
+void l2p_pass(Ctx &c);
+void direct_sum(int device, hipStream_t s, exafmm_body *t, int64_t nt, const exafmm_body *src, int64_t ns);
+
+// single-operator entry points (FP64, one call == one reference call)
+void op_p2p(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns);
+void op_p2m(Ctx &c, const double *C, const exafmm_body *b, int64_t nb, double *M);
+void op_m2m(Ctx &c, const double *Cp, const double *Cc, const double *Mc, double *Mp);
+void op_m2l(Ctx &c, const double *Xi, const double *Xj, const double *Mj, double *Li);
+void op_l2l(Ctx &c, const double *Cp, const double *Lp, const double *Cc, double *Lc);
+void op_l2p(Ctx &c, const double *C, const double *L, exafmm_body *b, int64_t nb);
+
+// generic helpers (scan.hip)
+void scan_exclusive(const int *in, int *out, int count, int *total, hipStream_t s);
+void scan_exclusive_dev(const int *in, int *out, const int *count_dev, int *total, hipStream_t s);
+void radix_sort_pairs(unsigned long long *&k, int *&v, unsigned long long *&k2, int *&v2, int n, int lo_bit,
+                      int hi_bit, Buf<int> &hist, hipStream_t s);
+
+}  // namespace xf
+
+// ==================================================================== device helpers
+namespace xfd {
+
+__host__ __device__ inline int nms(int n, int m) { return n * (n + 1) / 2 + m; }
+// real-form index of (n, m>=0, part) with part 0 = Re, 1 = Im (m > 0 only)
+__host__ __device__ inline int rf(int n, int m, int part) { return n * n + (m == 0 ? 0 : 2 * m - 1 + part); }
+__host__ __device__ inline int sgn(int n) { return (n & 1) ? -1 : 1; }
+
+template <class T>
+struct cpx {
+  T re, im;
+};
+template <class T>
+__device__ inline cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
+// Regular solid harmonics Y_n^m(x,y,z) for 0<=m<=n<P in the reference's
+// normalisation (3d/kernel.h:43-80 evaluates the same functions in
+// spherical form): Y_n^m = (-1)^n rho^n P_n^m(cos a) e^{imb} / (n+m)!.
+// Cartesian recurrences (no trigonometry, finite at r = 0):
+//   Y_m^m = Y_{m-1}^{m-1} (x+iy)/(2m),  Y_{m+1}^m = -z Y_m^m,
+//   Y_n^m = -((2n-1) z Y_{n-1}^m + r^2 Y_{n-2}^m) / ((n+m)(n-m)).
+// Y is stored by column: Y[nms(n,m)].
+template <class T>
+__device__ inline void regular_all(int P, T x, T y, T z, cpx<T> *Y) {
+  const T r2 = x * x + y * y + z * z;
+  cpx<T> ymm = {T(1), T(0)};
+  for (int m = 0; m < P; m++) {
+    if (m > 0) {
+      const T inv = T(1) / T(2 * m);
+      ymm = {(ymm.re * x - ymm.im * y) * inv, (ymm.re * y + ymm.im * x) * inv};
+    }
+    Y[nms(m, m)] = ymm;
+    if (m + 1 < P) Y[nms(m + 1, m)] = {-z * ymm.re, -z * ymm.im};
+    for (int n = m + 2; n < P; n++) {
+      const cpx<T> a = Y[nms(n - 1, m)], b = Y[nms(n - 2, m)];
+      const T s = T(-1) / T((n + m) * (n - m));
+      const T c = T(2 * n - 1) * z;
+      Y[nms(n, m)] = {(c * a.re + r2 * b.re) * s, (c * a.im + r2 * b.im) * s};
+    }
+  }
+}
+
+// Singular solid harmonics I_n^m = (-1)^n (n-m)! P_n^m(cos a) e^{imb} / rho^{n+1}
+// for 0<=m<=n<Pf, the functions 3d/kernel.h:83-116 evaluates:
+//   I_m^m = I_{m-1}^{m-1} (2m-1)(x+iy)/r^2,   I_0^0 = 1/r,
+//   I_{n+1}^m = -((2n+1) z I_n^m + (n+m)(n-m) I_{n-1}^m) / r^2.
+template <class T>
+__device__ inline void singular_all(int Pf, T x, T y, T z, cpx<T> *I) {
+  const T r2 = x * x + y * y + z * z;
+  const T inv = T(1) / r2;
+  cpx<T> imm = {T(1) / sqrt(r2), T(0)};
+  for (int m = 0; m < Pf; m++) {
+    if (m > 0) {
+      const T f = T(2 * m - 1) * inv;
+      imm = {(imm.re * x - imm.im * y) * f, (imm.re * y + imm.im * x) * f};
+    }
+    I[nms(m, m)] = imm;
+    cpx<T> a = imm, b = {T(0), T(0)};
+    for (int n = m; n + 1 < Pf; n++) {
+      const T c = T(2 * n + 1) * z;
+      const T d = T((n + m) * (n - m));
+      const cpx<T> nx = {-(c * a.re + d * b.re) * inv, -(c * a.im + d * b.im) * inv};
+      I[nms(n + 1, m)] = nx;
+      b = a;
+      a = nx;
+    }
+  }
+}
+
+}  // namespace xfd

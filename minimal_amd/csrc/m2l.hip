@@ -1,0 +1,85 @@
+// m2l.hip -- multipole-to-local translation.
+//
+// Reference: M2L 3d/kernel.h:233-260 with singular harmonics 3d/kernel.h:83-116,
+// evaluated over each target's M2L list (3d/traverse_lazy.h:46-56).
+//
+// Two paths:
+//  * exact  (m2l_precision = FP64): one wavefront per target cell, lanes =
+//    output coefficients (j,k), the reference's double sum per pair on the
+//    FP64 singular harmonics of the true offset.  Parity path.
+//  * fast   (FP32 / BF16X3): pairs grouped by translation key; every key's
+//    radius-scaled P^2 x P^2 real operator is applied to tiles of targets
+//    with MFMA (m2l_mfma.hip).
+// Truncation: "reference" keeps the terms the reference evaluates (its
+// harmonic table is zero above degree P-1, so j+n <= P-1); "full" keeps
+// every j+n <= 2P-2.
+#include "fmm_internal.h"
+#include "fmm_ops.h"
+
+namespace xf {
+using namespace xfd;
+
+__global__ void __launch_bounds__(64) k_m2l_fp64(int P, int NT, int fillP, int ncells, const double4 *__restrict__ cx,
+                                                 const double4 *__restrict__ scx, const int *__restrict__ beg,
+                                                 const int *__restrict__ cnt, const int2 *__restrict__ list,
+                                                 const double2 *__restrict__ M, double2 *__restrict__ L,
+                                                 const double *__restrict__ shifts) {
+  extern __shared__ double2 sm[];
+  const int c = blockIdx.x;
+  if (c >= ncells) return;
+  const int n_pairs = cnt[c];
+  if (n_pairs == 0) return;
+  const int NF = fillP * (fillP + 1) / 2;
+  cpx<double> *I = reinterpret_cast<cpx<double> *>(sm);  // [NF]
+  double2 *Ms = sm + NF;                                 // [NT]
+  const int lane = threadIdx.x;
+  const double4 Xc = cx[c];
+  const int b0 = beg[c];
+  double accr[3] = {0, 0, 0}, acci[3] = {0, 0, 0};
+  for (int p = 0; p < n_pairs; p++) {
+    const int2 e = list[b0 + p];
+    const double4 Xs = scx[e.x];
+    double dx = Xc.x - Xs.x, dy = Xc.y - Xs.y, dz = Xc.z - Xs.z;
+    if (shifts) {
+      dx -= shifts[3 * e.y];
+      dy -= shifts[3 * e.y + 1];
+      dz -= shifts[3 * e.y + 2];
+    }
+    __syncthreads();
+    if (lane < fillP) singular_column(lane, fillP, dx, dy, dz, I);
+    for (int i = lane; i < NT; i += 64) Ms[i] = M[int64_t(e.x) * NT + i];
+    __syncthreads();
+    for (int r = 0; r * 64 + lane < NT; r++) {
+      int j, k;
+      jk_of(r * 64 + lane, j, k);
+      const double2 v = m2l_coef(P, fillP, j, k, Ms, I);
+      accr[r] += v.x;
+      acci[r] += v.y;
+    }
+  }
+  for (int r = 0; r * 64 + lane < NT; r++) {
+    const int jk = r * 64 + lane;
+    double2 *o = L + int64_t(c) * NT + jk;
+    *o = make_double2(o->x + accr[r], o->y + acci[r]);
+  }
+}
+
+void m2l_fast(Ctx &tc, const Ctx &sc);  // m2l_mfma.hip
+
+void m2l_apply(Ctx &tc, const Ctx &sc) {
+  const int P = tc.P, NT = tc.NT;
+  const int fillP = tc.cfg.m2l_truncation == EXAFMM_M2L_FULL ? 2 * P - 1 : P;
+  Tree &t = tc.t;
+  tc.L.ensure(size_t(t.ncells) * NT);
+  if (tc.cfg.m2l_precision != EXAFMM_PREC_FP64 && tc.lists.nkeys > 0) {
+    m2l_fast(tc, sc);
+    return;
+  }
+  const size_t lds = sizeof(double2) * (fillP * (fillP + 1) / 2 + NT);
+  hipLaunchKernelGGL(k_m2l_fp64, dim3(t.ncells), dim3(64), lds, tc.s0, P, NT, fillP, int(t.ncells), t.cx.p,
+                     sc.t.cx.p, tc.lists.m2l_beg.p, tc.lists.m2l_cnt.p, tc.lists.m2l.p, sc.M.p, tc.L.p,
+                     (const double *)nullptr);
+  XF_LAUNCH();
+}
+
+}  // namespace xf

@@ -1,0 +1,403 @@
+// traverse.hip -- dual-tree traversal emitting flat M2L / P2P lists.
+//
+// Reference: the dual-tree rule of 3d/traverse_eager.h:147-165 and its lazy
+// list form 3d/traverse_lazy.h:27-43:
+//   MAC |Xi-Xj|^2 theta^2 > (Ri+Rj)^2          -> M2L(Ci, Cj)
+//   both leaves                                -> P2P(Ci, Cj)
+//   Cj leaf or (Ri >= Rj and Ci not a leaf)    -> split Ci
+//   otherwise                                  -> split Cj
+// Target-centric, level-synchronous form: a pair (c, b) is visited iff b
+// is handed down from c's parent (the parent's pair split on the target
+// side) or b is a child of a pair (c, parent(b)) split on the source side.
+// One wavefront per target cell works through its pairs breadth-first in
+// LDS; every target's lists are therefore built by one wave, in a fixed
+// order, with no atomics.  Each level runs twice: a count pass, an
+// exclusive scan, and a fill pass writing at the scanned offsets.
+// The MAC is evaluated in FP64 with contraction disabled, so each pair is
+// classified exactly as the reference classifies it.
+#include "fmm_internal.h"
+
+namespace xf {
+
+constexpr int kTravWaves = 4;
+constexpr int kQCap = 2048;
+constexpr unsigned long long kEmpty = ~0ull;
+
+struct TravArgs {
+  // target tree
+  const double4 *cx;
+  const int *cnch, *clevel, *cparent;
+  // source tree
+  const double4 *scx;
+  const int *scnch, *scchild, *sclevel;
+  double theta;
+  int lstart, lcount;
+  // hand-down lists
+  const int *in_beg, *in_cnt, *in;
+  int *out_cnt, *out_beg, *out;
+  // results
+  int *m2l_cnt, *m2l_beg;
+  int2 *m2l;
+  int *p2p_cnt, *p2p_beg, *p2p;
+  int *flags;
+  int64_t cap_m2l, cap_p2p, cap_out;
+  // translation keys
+  unsigned long long *hslots;
+  int hmask;
+  int want_keys;
+};
+
+__device__ inline unsigned long long mix64(unsigned long long x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return x;
+}
+
+// Key of the M2L translation (c <- b): offset in units of the finer of the
+// two radii (an integer vector for cells of one tree) and the level
+// difference.  Under radius scaling of M and L the operator depends on
+// nothing else.
+__device__ inline bool m2l_key(double4 Xc, int lc, double4 Xb, int lb, unsigned long long &key) {
+  const double rf = fmin(Xc.w, Xb.w);
+  const double dx = rint((Xc.x - Xb.x) / rf), dy = rint((Xc.y - Xb.y) / rf), dz = rint((Xc.z - Xb.z) / rf);
+  const double lim = 131071.0;
+  if (fabs(dx) > lim || fabs(dy) > lim || fabs(dz) > lim) return false;
+  const long long ix = (long long)dx + 131072, iy = (long long)dy + 131072, iz = (long long)dz + 131072;
+  key = ((unsigned long long)ix << 46) | ((unsigned long long)iy << 28) | ((unsigned long long)iz << 10) |
+        (unsigned long long)(lc - lb + 64);
+  return true;
+}
+
+__device__ inline int key_insert(unsigned long long *slots, int mask, unsigned long long key, int *flags) {
+  int h = int(mix64(key) & (unsigned long long)mask);
+  for (int probe = 0; probe <= mask; probe++) {
+    unsigned long long v = __atomic_load_n(&slots[h], __ATOMIC_RELAXED);
+    if (v == key) return h;
+    if (v == kEmpty) {
+      const unsigned long long prev = atomicCAS(&slots[h], kEmpty, key);
+      if (prev == kEmpty || prev == key) return h;
+    }
+    h = (h + 1) & mask;
+  }
+  atomicOr(flags, 8);
+  return 0;
+}
+
+__device__ inline int wave_excl_scan(int v, int lane, int &total) {
+  int x = v;
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int y = __shfl_up(x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  total = __shfl(x, kWave - 1, kWave);
+  return x - v;
+}
+
+template <bool FILL>
+__global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
+  __shared__ int queue[kTravWaves][kQCap];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int w = threadIdx.x / kWave;
+  const int t = blockIdx.x * kTravWaves + w;
+  if (t >= a.lcount) return;
+  const int c = a.lstart + t;
+  int *q = queue[w];
+  const unsigned long long lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+
+  // initial pairs: handed down from the parent, or (root, root)
+  int tail;
+  if (a.in) {
+    const int par = a.cparent[c];
+    const int ib = a.in_beg[par];
+    tail = a.in_cnt[par];
+    if (tail > kQCap) {
+      if (lane == 0) atomicOr(a.flags, 4);
+      return;
+    }
+    for (int i = lane; i < tail; i += kWave) q[i] = a.in[ib + i];
+  } else {
+    tail = 1;
+    if (lane == 0) q[0] = 0;
+  }
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+
+  const double4 Xc = a.cx[c];
+  const int leafc = a.cnch[c] == 0;
+  const int lc = a.clevel[c];
+  const double th2 = a.theta;
+  int head = 0;
+  int nm = 0, np = 0, no = 0;
+  int bm = 0, bp = 0, bo = 0;
+  if (FILL) {
+    bm = a.m2l_beg[c];
+    bp = a.p2p_beg[c];
+    bo = a.out_beg[c];
+  }
+  while (head < tail) {
+    const int k = min(kWave, tail - head);
+    int b = -1, kind = -1, nchb = 0;
+    double4 Xb;
+    if (lane < k) {
+      b = q[(head + lane) % kQCap];
+      Xb = a.scx[b];
+      nchb = a.scnch[b];
+      bool mac;
+      {
+#pragma clang fp contract(off)
+        const double dx = Xc.x - Xb.x, dy = Xc.y - Xb.y, dz = Xc.z - Xb.z;
+        const double R2 = (dx * dx + dy * dy + dz * dz) * th2 * th2;
+        const double Rs = Xc.w + Xb.w;
+        mac = R2 > Rs * Rs;
+      }
+      if (mac)
+        kind = 0;
+      else if (leafc && nchb == 0)
+        kind = 1;
+      else if (nchb == 0 || (Xc.w >= Xb.w && !leafc))
+        kind = 2;
+      else
+        kind = 3;
+    }
+    const unsigned long long mm = __ballot(kind == 0), mp = __ballot(kind == 1), mo = __ballot(kind == 2);
+    if (FILL) {
+      if (kind == 0) {
+        const int pos = bm + nm + __popcll(mm & lt);
+        int slot = 0;
+        if (a.want_keys) {
+          unsigned long long key;
+          if (m2l_key(Xc, lc, Xb, a.sclevel[b], key))
+            slot = key_insert(a.hslots, a.hmask, key, a.flags);
+          else
+            atomicOr(a.flags, 16);
+        }
+        if (pos < a.cap_m2l) a.m2l[pos] = make_int2(b, slot);
+      } else if (kind == 1) {
+        const int pos = bp + np + __popcll(mp & lt);
+        if (pos < a.cap_p2p) a.p2p[pos] = b;
+      } else if (kind == 2) {
+        const int pos = bo + no + __popcll(mo & lt);
+        if (pos < a.cap_out) a.out[pos] = b;
+      }
+    }
+    nm += __popcll(mm);
+    np += __popcll(mp);
+    no += __popcll(mo);
+    // split the source: enqueue its children
+    const int nadd = kind == 3 ? nchb : 0;
+    int total;
+    const int off = wave_excl_scan(nadd, lane, total);
+    if (tail - head - k + total > kQCap) {
+      if (lane == 0) atomicOr(a.flags, 4);
+      break;
+    }
+    if (nadd) {
+      const int first = a.scchild[b];
+      for (int i = 0; i < nadd; i++) q[(tail + off + i) % kQCap] = first + i;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    head += k;
+    tail += total;
+  }
+  if (!FILL && lane == 0) {
+    a.m2l_cnt[c] = nm;
+    a.p2p_cnt[c] = np;
+    a.out_cnt[c] = no;
+  }
+}
+
+// per-level offsets: exclusive scans of the three counts + running totals
+__global__ void __launch_bounds__(1024) k_trav_scan(int lstart, int lcount, const int *__restrict__ m2l_cnt,
+                                                    const int *__restrict__ p2p_cnt,
+                                                    const int *__restrict__ out_cnt, int *__restrict__ m2l_beg,
+                                                    int *__restrict__ p2p_beg, int *__restrict__ out_beg,
+                                                    int *__restrict__ tot, int64_t cap_m2l, int64_t cap_p2p,
+                                                    int64_t cap_out) {
+  __shared__ int part[3][1024];
+  const int t = threadIdx.x;
+  const int chunk = (lcount + 1023) / 1024;
+  const int b = t * chunk, e = min(b + chunk, lcount);
+  int s0 = 0, s1 = 0, s2 = 0;
+  for (int i = b; i < e; i++) {
+    s0 += m2l_cnt[lstart + i];
+    s1 += p2p_cnt[lstart + i];
+    s2 += out_cnt[lstart + i];
+  }
+  part[0][t] = s0;
+  part[1][t] = s1;
+  part[2][t] = s2;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    int v0 = t >= off ? part[0][t - off] : 0;
+    int v1 = t >= off ? part[1][t - off] : 0;
+    int v2 = t >= off ? part[2][t - off] : 0;
+    __syncthreads();
+    part[0][t] += v0;
+    part[1][t] += v1;
+    part[2][t] += v2;
+    __syncthreads();
+  }
+  int r0 = tot[0] + (t ? part[0][t - 1] : 0);
+  int r1 = tot[1] + (t ? part[1][t - 1] : 0);
+  int r2 = tot[2] + (t ? part[2][t - 1] : 0);
+  for (int i = b; i < e; i++) {
+    m2l_beg[lstart + i] = r0;
+    p2p_beg[lstart + i] = r1;
+    out_beg[lstart + i] = r2;
+    r0 += m2l_cnt[lstart + i];
+    r1 += p2p_cnt[lstart + i];
+    r2 += out_cnt[lstart + i];
+  }
+  __syncthreads();
+  if (t == 0) {
+    tot[0] += part[0][1023];
+    tot[1] += part[1][1023];
+    tot[2] += part[2][1023];
+    if (tot[0] > cap_m2l || tot[1] > cap_p2p || tot[2] > cap_out) tot[3] |= 1;
+  }
+}
+
+__global__ void k_hash_clear(unsigned long long *slots, int n) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) slots[i] = kEmpty;
+}
+
+__global__ void k_hash_collect(const unsigned long long *__restrict__ slots, int n,
+                               unsigned long long *__restrict__ keys, int *__restrict__ sl, int *__restrict__ count) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n && slots[i] != kEmpty) {
+    const int p = atomicAdd(count, 1);
+    keys[p] = slots[i];
+    sl[p] = i;
+  }
+}
+
+__global__ void k_slot_ids(const int *__restrict__ sl, int n, int *__restrict__ slot2id) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) slot2id[sl[i]] = i;
+}
+
+void traverse(Ctx &tc, const Ctx &sc) {
+  Tree &T = tc.t;
+  const Tree &S = sc.t;
+  Lists &Lr = tc.lists;
+  hipStream_t s = tc.s0;
+  const int64_t nc = T.ncells;
+  const bool keys = (&tc == &sc) && tc.cfg.images == 0;
+  Lr.m2l_beg.ensure(nc);
+  Lr.m2l_cnt.ensure(nc);
+  Lr.p2p_beg.ensure(nc);
+  Lr.p2p_cnt.ensure(nc);
+  Lr.out_beg.ensure(nc);
+  Lr.out_cnt.ensure(nc);
+  Lr.tot.ensure(8);
+  if (Lr.cap_m2l < 200 * nc) Lr.cap_m2l = 200 * nc;
+  if (Lr.cap_p2p < 96 * nc) Lr.cap_p2p = 96 * nc;
+  if (Lr.cap_out < 96 * nc) Lr.cap_out = 96 * nc;
+  if (keys && Lr.hcap == 0) Lr.hcap = 1 << 16;
+
+  for (int attempt = 0; attempt < 4; attempt++) {
+    Lr.m2l.ensure(Lr.cap_m2l);
+    Lr.p2p.ensure(Lr.cap_p2p);
+    Lr.out.ensure(Lr.cap_out);
+    XF_HIP(hipMemsetAsync(Lr.tot.p, 0, 8 * sizeof(int), s));
+    if (keys) {
+      Lr.hslots.ensure(Lr.hcap);
+      hipLaunchKernelGGL(k_hash_clear, dim3(ceil_div(Lr.hcap, 256)), dim3(256), 0, s, Lr.hslots.p, Lr.hcap);
+      XF_LAUNCH();
+    }
+    TravArgs a{};
+    a.cx = T.cx.p;
+    a.cnch = T.cnch.p;
+    a.clevel = T.clevel.p;
+    a.cparent = T.cparent.p;
+    a.scx = S.cx.p;
+    a.scnch = S.cnch.p;
+    a.scchild = S.cchild.p;
+    a.sclevel = S.clevel.p;
+    a.theta = tc.cfg.theta;
+    a.out_cnt = Lr.out_cnt.p;
+    a.out_beg = Lr.out_beg.p;
+    a.out = Lr.out.p;
+    a.m2l_cnt = Lr.m2l_cnt.p;
+    a.m2l_beg = Lr.m2l_beg.p;
+    a.m2l = Lr.m2l.p;
+    a.p2p_cnt = Lr.p2p_cnt.p;
+    a.p2p_beg = Lr.p2p_beg.p;
+    a.p2p = Lr.p2p.p;
+    a.flags = Lr.tot.p + 3;
+    a.cap_m2l = Lr.cap_m2l;
+    a.cap_p2p = Lr.cap_p2p;
+    a.cap_out = Lr.cap_out;
+    a.hslots = Lr.hslots.p;
+    a.hmask = Lr.hcap - 1;
+    a.want_keys = keys;
+    for (int l = 0; l <= T.depth; l++) {
+      a.lstart = T.lstart[l];
+      a.lcount = T.lstart[l + 1] - T.lstart[l];
+      a.in = l ? Lr.out.p : nullptr;
+      a.in_beg = Lr.out_beg.p;
+      a.in_cnt = Lr.out_cnt.p;
+      const int g = ceil_div(a.lcount, kTravWaves);
+      hipLaunchKernelGGL(k_traverse<false>, dim3(g), dim3(kTravWaves * kWave), 0, s, a);
+      XF_LAUNCH();
+      hipLaunchKernelGGL(k_trav_scan, dim3(1), dim3(1024), 0, s, a.lstart, a.lcount, Lr.m2l_cnt.p, Lr.p2p_cnt.p,
+                         Lr.out_cnt.p, Lr.m2l_beg.p, Lr.p2p_beg.p, Lr.out_beg.p, Lr.tot.p, Lr.cap_m2l, Lr.cap_p2p,
+                         Lr.cap_out);
+      XF_LAUNCH();
+      hipLaunchKernelGGL(k_traverse<true>, dim3(g), dim3(kTravWaves * kWave), 0, s, a);
+      XF_LAUNCH();
+    }
+    int tot[8];
+    XF_HIP(hipMemcpyAsync(tot, Lr.tot.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+    XF_HIP(hipStreamSynchronize(s));
+    const int flags = tot[3];
+    if (flags & 4) fail(EXAFMM_ERR_CAPACITY, "traversal work queue overflow (pairs per target cell > 2048)");
+    if (flags & 16) fail(EXAFMM_ERR_CAPACITY, "M2L offset out of key range");
+    if (flags & 1) {  // list capacity: retry with the exact totals
+      Lr.cap_m2l = std::max<int64_t>(Lr.cap_m2l * 2, tot[0]);
+      Lr.cap_p2p = std::max<int64_t>(Lr.cap_p2p * 2, tot[1]);
+      Lr.cap_out = std::max<int64_t>(Lr.cap_out * 2, tot[2]);
+      continue;
+    }
+    if (flags & 8) {  // key table full
+      Lr.hcap *= 4;
+      continue;
+    }
+    Lr.nm2l = tot[0];
+    Lr.np2p = tot[1];
+    break;
+  }
+  Lr.valid = true;
+  Lr.nkeys = 0;
+  if (keys) {
+    Lr.keys.ensure(size_t(Lr.hcap) * 2);
+    Lr.kscratch.ensure(size_t(Lr.hcap) * 2 + 8);
+    Lr.slot2id.ensure(Lr.hcap);
+    int *cnt = Lr.tot.p + 6;
+    XF_HIP(hipMemsetAsync(cnt, 0, sizeof(int), s));
+    unsigned long long *k1 = Lr.keys.p, *k2 = Lr.keys.p + Lr.hcap;
+    int *v1 = Lr.kscratch.p, *v2 = Lr.kscratch.p + Lr.hcap;
+    hipLaunchKernelGGL(k_hash_collect, dim3(ceil_div(Lr.hcap, 256)), dim3(256), 0, s, Lr.hslots.p, Lr.hcap, k1, v1,
+                       cnt);
+    XF_LAUNCH();
+    int nk = 0;
+    XF_HIP(hipMemcpyAsync(&nk, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
+    XF_HIP(hipStreamSynchronize(s));
+    Buf<int> rh;
+    radix_sort_pairs(k1, v1, k2, v2, nk, 0, 64, rh, s);
+    hipLaunchKernelGGL(k_slot_ids, dim3(ceil_div(std::max(nk, 1), 256)), dim3(256), 0, s, v1, nk, Lr.slot2id.p);
+    XF_LAUNCH();
+    if (k1 != Lr.keys.p)
+      XF_HIP(hipMemcpyAsync(Lr.keys.p, k1, sizeof(unsigned long long) * nk, hipMemcpyDeviceToDevice, s));
+    XF_HIP(hipStreamSynchronize(s));
+    Lr.nkeys = nk;
+  }
+}
+
+}  // namespace xf

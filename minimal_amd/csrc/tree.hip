@@ -1,0 +1,495 @@
+// tree.hip -- octree construction on the GPU, bit-exact with the reference.
+//
+// Reference: 3d/build_tree.h:7-96.  The reference builds the tree top-down
+// by recursive stable octant partitions against FP64 cell centres
+// (child centre = parent centre +/- R0/2^(l+1), strict '>'), stopping at
+// NBODY <= ncrit.  Here each body's full octant path is computed once from
+// the same FP64 centre sequence (63-bit key, 21 levels), the keys are
+// radix-sorted (stable, so bodies with equal prefixes keep input order),
+// levels are split by binary search on the sorted keys, and each leaf's
+// bodies are put back into input order -- which is the order the
+// reference's stable partitions leave inside a leaf.  Cells are stored
+// level by level (BFS); the reference's numbering (children blocks in DFS
+// preorder of their parents) is computed on export.
+#include <algorithm>
+
+#include "fmm_internal.h"
+
+namespace xf {
+
+constexpr int kHistLevels = 7;
+__host__ __device__ constexpr int hist_off(int l) {  // offset of level l (1..7) in the histogram array
+  return l <= 1 ? 0 : hist_off(l - 1) + (1 << (3 * (l - 1)));
+}
+constexpr int kHistTotal = hist_off(kHistLevels + 1);
+constexpr int kHistShared = hist_off(5);  // levels 1..4 aggregated in LDS
+
+// ---------------------------------------------------------------- bounds
+__global__ void __launch_bounds__(256) k_bounds_part(const exafmm_body *__restrict__ b, int64_t n,
+                                                     double *__restrict__ part, int *__restrict__ flags) {
+  __shared__ double s[6][256];
+  double lo[3] = {b[0].X[0], b[0].X[1], b[0].X[2]};
+  double hi[3] = {lo[0], lo[1], lo[2]};
+  int bad = 0;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256) {
+    for (int d = 0; d < 3; d++) {
+      const double x = b[i].X[d];
+      if (!isfinite(x)) bad = 1;
+      lo[d] = fmin(x, lo[d]);
+      hi[d] = fmax(x, hi[d]);
+    }
+    if (!isfinite(b[i].q)) bad = 1;
+  }
+  if (bad) atomicOr(flags, 1);
+  for (int d = 0; d < 3; d++) {
+    s[d][threadIdx.x] = lo[d];
+    s[3 + d][threadIdx.x] = hi[d];
+  }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int d = 0; d < 3; d++) {
+        s[d][threadIdx.x] = fmin(s[d][threadIdx.x], s[d][threadIdx.x + o]);
+        s[3 + d][threadIdx.x] = fmax(s[3 + d][threadIdx.x], s[3 + d][threadIdx.x + o]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+// X0 = (max+min)/2, R0 = max half-extent * 1.00001 (3d/build_tree.h:7-21).
+__global__ void k_bounds_final(const double *__restrict__ part, int nparts, double *__restrict__ root) {
+  if (threadIdx.x != 0) return;
+  double lo[3], hi[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = part[d];
+    hi[d] = part[3 + d];
+  }
+  for (int p = 1; p < nparts; p++)
+    for (int d = 0; d < 3; d++) {
+      lo[d] = fmin(part[p * 6 + d], lo[d]);
+      hi[d] = fmax(part[p * 6 + 3 + d], hi[d]);
+    }
+  double X0[3];
+  for (int d = 0; d < 3; d++) X0[d] = (hi[d] + lo[d]) / 2;
+  double R0 = 0;
+  for (int d = 0; d < 3; d++) {
+    R0 = fmax(X0[d] - lo[d], R0);
+    R0 = fmax(hi[d] - X0[d], R0);
+  }
+  R0 *= 1.00001;
+  root[0] = X0[0];
+  root[1] = X0[1];
+  root[2] = X0[2];
+  root[3] = R0;
+}
+
+// ---------------------------------------------------------------- keys
+// Octant path of each body through the reference's centre sequence.
+__global__ void __launch_bounds__(256) k_keys(const exafmm_body *__restrict__ b, int64_t n,
+                                              const double *__restrict__ root, unsigned long long *__restrict__ key,
+                                              int *__restrict__ val, int *__restrict__ hist) {
+  __shared__ int sh[kHistShared];
+  for (int i = threadIdx.x; i < kHistShared; i += 256) sh[i] = 0;
+  __syncthreads();
+  const double R0 = root[3];
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) {
+    const double x = b[i].X[0], y = b[i].X[1], z = b[i].X[2];
+    double cx = root[0], cy = root[1], cz = root[2];
+    unsigned long long k = 0;
+    for (int l = 0; l < kMaxLevel; l++) {
+      const int bx = x > cx, by = y > cy, bz = z > cz;
+      const int oct = bx + (by << 1) + (bz << 2);
+      k = (k << 3) | unsigned(oct);
+      const double r = R0 / (1 << (l + 1));
+      cx += bx ? r : -r;
+      cy += by ? r : -r;
+      cz += bz ? r : -r;
+    }
+    key[i] = k;
+    val[i] = int(i);
+    for (int l = 1; l <= kHistLevels; l++) {
+      const int pre = int(k >> (3 * (kMaxLevel - l)));
+      if (l <= 4)
+        atomicAdd(&sh[hist_off(l) + pre], 1);
+      else
+        atomicAdd(&hist[hist_off(l) + pre], 1);
+    }
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kHistShared; j += 256)
+    if (sh[j]) atomicAdd(&hist[j], sh[j]);
+}
+
+// Largest prefix population of each histogram level.
+__global__ void __launch_bounds__(256) k_hist_max(const int *__restrict__ hist, int *__restrict__ lmax) {
+  for (int j = blockIdx.x * 256 + threadIdx.x; j < kHistTotal; j += gridDim.x * 256) {
+    const int v = hist[j];
+    if (!v) continue;
+    int l = 1;
+    while (l < kHistLevels && j >= hist_off(l + 1)) l++;
+    atomicMax(&lmax[l], v);
+  }
+}
+
+// ---------------------------------------------------------------- levels
+// Node t of level l: if it holds more than ncrit bodies, find the body
+// sub-ranges of its 8 octants (keys are sorted by the level-(l+1) digit
+// inside the node's range) by binary search.
+__global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__restrict__ lcount,
+                                               const int *__restrict__ lstart, const int *__restrict__ cbeg,
+                                               const int *__restrict__ ccnt,
+                                               const unsigned long long *__restrict__ key, int *__restrict__ nch,
+                                               int *__restrict__ bnd) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= lcount[l]) return;
+  const int c = lstart[l] + t;
+  const int beg = cbeg[c], cnt = ccnt[c];
+  if (cnt <= ncrit || l >= kMaxLevel) {
+    nch[t] = 0;
+    return;
+  }
+  const int sh = 3 * (kMaxLevel - 1 - l);
+  int b[9];
+  b[0] = beg;
+  b[8] = beg + cnt;
+  for (int o = 1; o < 8; o++) {
+    int lo = b[o - 1], hi = beg + cnt;  // first index with digit >= o
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (int((key[mid] >> sh) & 7ull) < o)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    b[o] = lo;
+  }
+  int k = 0;
+  for (int o = 0; o < 8; o++) {
+    bnd[t * 9 + o] = b[o];
+    k += b[o + 1] > b[o];
+  }
+  bnd[t * 9 + 8] = b[8];
+  nch[t] = k;
+}
+
+__global__ void k_level_next(int l, int *__restrict__ lcount, int *__restrict__ lstart, const int *__restrict__ total) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    lcount[l + 1] = *total;
+    lstart[l + 1] = lstart[l] + lcount[l];
+  }
+}
+
+// Children of level-l node t, in octant order (3d/build_tree.h:115-132):
+// centre = parent centre + r*(+/-1), r = R0/2^(l+1); R = R0/2^(l+1).
+__global__ void __launch_bounds__(256) k_emit(int l, const int *__restrict__ lcount, const int *__restrict__ lstart,
+                                              const int *__restrict__ nch, const int *__restrict__ off,
+                                              const int *__restrict__ bnd, const double *__restrict__ root,
+                                              double4 *__restrict__ cx, int *__restrict__ cbeg,
+                                              int *__restrict__ ccnt, int *__restrict__ cchild,
+                                              int *__restrict__ cnch, int *__restrict__ cparent,
+                                              int *__restrict__ clevel) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= lcount[l]) return;
+  const int c = lstart[l] + t;
+  const int k = nch[t];
+  cnch[c] = k;
+  if (k == 0) {
+    cchild[c] = -1;
+    return;
+  }
+  const int first = lstart[l + 1] + off[t];
+  cchild[c] = first;
+  const double4 P = cx[c];
+  const double r = root[3] / (1 << (l + 1));
+  int j = 0;
+  for (int o = 0; o < 8; o++) {
+    const int b0 = bnd[t * 9 + o], b1 = bnd[t * 9 + o + 1];
+    if (b1 <= b0) continue;
+    const int ch = first + j++;
+    double4 X;
+    X.x = P.x + ((o & 1) ? r : -r);
+    X.y = P.y + ((o & 2) ? r : -r);
+    X.z = P.z + ((o & 4) ? r : -r);
+    X.w = r;
+    cx[ch] = X;
+    cbeg[ch] = b0;
+    ccnt[ch] = b1 - b0;
+    cparent[ch] = c;
+    clevel[ch] = l + 1;
+  }
+}
+
+__global__ void k_root(int64_t n, const double *__restrict__ root, double4 *__restrict__ cx, int *__restrict__ cbeg,
+                       int *__restrict__ ccnt, int *__restrict__ cparent, int *__restrict__ clevel,
+                       int *__restrict__ lcount, int *__restrict__ lstart) {
+  cx[0] = make_double4(root[0], root[1], root[2], root[3]);
+  cbeg[0] = 0;
+  ccnt[0] = int(n);
+  cparent[0] = -1;
+  clevel[0] = 0;
+  lcount[0] = 1;
+  lstart[0] = 0;
+}
+
+// Cells of the deepest sorted level: leaves; more than ncrit bodies there
+// means the tree needs more than 21 levels.
+__global__ void __launch_bounds__(256) k_last_level(int l, int ncrit, const int *__restrict__ lcount,
+                                                    const int *__restrict__ lstart, const int *__restrict__ ccnt,
+                                                    int *__restrict__ cchild, int *__restrict__ cnch,
+                                                    int *__restrict__ flags) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= lcount[l]) return;
+  const int c = lstart[l] + t;
+  cnch[c] = 0;
+  cchild[c] = -1;
+  if (ccnt[c] > ncrit) atomicOr(flags, 2);
+}
+
+__global__ void __launch_bounds__(256) k_leaf_flag(int ncells, const int *__restrict__ cnch, int *__restrict__ f) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < ncells) f[c] = cnch[c] == 0;
+}
+__global__ void __launch_bounds__(256) k_leaf_compact(int ncells, const int *__restrict__ cnch,
+                                                      const int *__restrict__ off, int *__restrict__ leaves) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c < ncells && cnch[c] == 0) leaves[off[c]] = c;
+}
+
+// One workgroup per leaf: restore input order inside the leaf (rank of
+// each original index among the leaf's), then gather X, q (FP64) and the
+// leaf-relative FP32 copy used by P2P.
+__global__ void __launch_bounds__(64) k_leaf_gather(const int *__restrict__ leaves, const int *__restrict__ nleaves,
+                                                    const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                                    const double4 *__restrict__ cx, const int *__restrict__ val,
+                                                    const exafmm_body *__restrict__ b, int *__restrict__ perm,
+                                                    double4 *__restrict__ xs, float4 *__restrict__ xq,
+                                                    int *__restrict__ body_leaf) {
+  extern __shared__ int sidx[];
+  const int li = blockIdx.x;
+  if (li >= *nleaves) return;
+  const int c = leaves[li];
+  const int beg = cbeg[c], cnt = ccnt[c];
+  for (int i = threadIdx.x; i < cnt; i += 64) sidx[i] = val[beg + i];
+  __syncthreads();
+  const double4 C = cx[c];
+  for (int i = threadIdx.x; i < cnt; i += 64) {
+    const int me = sidx[i];
+    int r = 0;
+    for (int j = 0; j < cnt; j++) r += sidx[j] < me;
+    const int pos = beg + r;
+    perm[pos] = me;
+    const double x = b[me].X[0], y = b[me].X[1], z = b[me].X[2], q = b[me].q;
+    xs[pos] = make_double4(x, y, z, q);
+    xq[pos] = make_float4(float(x - C.x), float(y - C.y), float(z - C.z), float(q));
+    body_leaf[pos] = c;
+  }
+}
+
+// Reference permutes X and q of the caller's array in place (p, F stay).
+__global__ void __launch_bounds__(256) k_writeback(int64_t n, const double4 *__restrict__ xs,
+                                                   exafmm_body *__restrict__ b) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double4 v = xs[i];
+  b[i].X[0] = v.x;
+  b[i].X[1] = v.y;
+  b[i].X[2] = v.z;
+  b[i].q = v.w;
+}
+
+void tree_build(Ctx &c) {
+  Tree &t = c.t;
+  const int64_t n = t.n;
+  hipStream_t s = c.s0;
+  const int ncrit = c.cfg.ncrit;
+  if (n > (int64_t(1) << 30)) fail(EXAFMM_ERR_ARG, "N above 2^30 is not supported");
+
+  // misc layout (ints): [0] flags, [1..8] level max, [16..] totals
+  t.misc.ensure(64);
+  t.bounds_part.ensure(256 * 6 + 8);
+  double *root = t.bounds_part.p + 256 * 6;
+  XF_HIP(hipMemsetAsync(t.misc.p, 0, 64 * sizeof(int), s));
+
+  hipLaunchKernelGGL(k_bounds_part, dim3(256), dim3(256), 0, s, c.bodies, n, t.bounds_part.p, t.misc.p);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, t.bounds_part.p, 256, root);
+  XF_LAUNCH();
+
+  t.key.ensure(n);
+  t.key2.ensure(n);
+  t.val.ensure(n);
+  t.val2.ensure(n);
+  t.hist.ensure(kHistTotal);
+  XF_HIP(hipMemsetAsync(t.hist.p, 0, sizeof(int) * kHistTotal, s));
+  hipLaunchKernelGGL(k_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, c.bodies, n, root, t.key.p, t.val.p, t.hist.p);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_hist_max, dim3(512), dim3(256), 0, s, t.hist.p, t.misc.p);
+  XF_LAUNCH();
+
+  // sync 1: bounds, input check, depth needed
+  int hm[9];
+  double rootv[4];
+  XF_HIP(hipMemcpyAsync(hm, t.misc.p, sizeof(hm), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipMemcpyAsync(rootv, root, sizeof(rootv), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipStreamSynchronize(s));
+  if (hm[0] & 1) fail(EXAFMM_ERR_ARG, "non-finite body coordinate or charge");
+  for (int d = 0; d < 3; d++) t.X0[d] = rootv[d];
+  t.R0 = rootv[3];
+  int Ds = kMaxLevel;
+  if (n <= ncrit) {
+    Ds = 0;
+  } else {
+    for (int l = 1; l <= kHistLevels; l++)
+      if (hm[l] <= ncrit) {
+        Ds = l;
+        break;
+      }
+  }
+
+  // stable sort by the top Ds octant digits
+  unsigned long long *k1 = t.key.p, *k2 = t.key2.p;
+  int *v1 = t.val.p, *v2 = t.val2.p;
+  radix_sort_pairs(k1, v1, k2, v2, int(n), 3 * (kMaxLevel - Ds), 3 * kMaxLevel, t.rs_hist, s);
+
+  // level-by-level split
+  std::vector<int64_t> ub(Ds + 2);
+  int64_t ubtot = 0;
+  // nodes of level l+1 are children of level-l nodes holding > ncrit bodies
+  const int64_t ub_split = 8 * (n / (ncrit + 1)) + 8;
+  for (int l = 0; l <= Ds; l++) {
+    ub[l] = std::min<int64_t>(l >= 10 ? n : (int64_t(1) << (3 * l)), n);
+    if (l > 0) ub[l] = std::min(ub[l], ub_split);
+    ubtot += ub[l];
+  }
+  t.cx.ensure(ubtot);
+  t.cbeg.ensure(ubtot);
+  t.ccnt.ensure(ubtot);
+  t.cchild.ensure(ubtot);
+  t.cnch.ensure(ubtot);
+  t.cparent.ensure(ubtot);
+  t.clevel.ensure(ubtot);
+  t.lcount.ensure(kMaxLevel + 2);
+  t.lstart_d.ensure(kMaxLevel + 2);
+  XF_HIP(hipMemsetAsync(t.lcount.p, 0, sizeof(int) * (kMaxLevel + 2), s));
+  int64_t ubmax = 1;
+  for (int l = 0; l <= Ds; l++) ubmax = std::max(ubmax, ub[l]);
+  t.tmp_nch.ensure(ubmax);
+  t.tmp_off.ensure(ubmax);
+  t.tmp_bnd.ensure(ubmax * 9);
+  hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, n, root, t.cx.p, t.cbeg.p, t.ccnt.p, t.cparent.p, t.clevel.p,
+                     t.lcount.p, t.lstart_d.p);
+  XF_LAUNCH();
+  for (int l = 0; l < Ds; l++) {
+    const int g = ceil_div(ub[l], 256);
+    hipLaunchKernelGGL(k_split, dim3(g), dim3(256), 0, s, l, ncrit, t.lcount.p, t.lstart_d.p, t.cbeg.p, t.ccnt.p, k1,
+                       t.tmp_nch.p, t.tmp_bnd.p);
+    XF_LAUNCH();
+    scan_exclusive_dev(t.tmp_nch.p, t.tmp_off.p, t.lcount.p + l, t.misc.p + 16, s);
+    hipLaunchKernelGGL(k_level_next, dim3(1), dim3(1), 0, s, l, t.lcount.p, t.lstart_d.p, t.misc.p + 16);
+    XF_LAUNCH();
+    hipLaunchKernelGGL(k_emit, dim3(g), dim3(256), 0, s, l, t.lcount.p, t.lstart_d.p, t.tmp_nch.p, t.tmp_off.p,
+                       t.tmp_bnd.p, root, t.cx.p, t.cbeg.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.cparent.p,
+                       t.clevel.p);
+    XF_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_last_level, dim3(ceil_div(ub[Ds], 256)), dim3(256), 0, s, Ds, ncrit, t.lcount.p,
+                     t.lstart_d.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.misc.p);
+  XF_LAUNCH();
+
+  // sync 2: level sizes
+  std::vector<int> lc(kMaxLevel + 2), fl(1);
+  XF_HIP(hipMemcpyAsync(lc.data(), t.lcount.p, sizeof(int) * (kMaxLevel + 2), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipMemcpyAsync(fl.data(), t.misc.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipStreamSynchronize(s));
+  if (fl[0] & 2) fail(EXAFMM_ERR_DEPTH, "more than ncrit bodies share a 21-level octant path");
+  t.lstart.assign(1, 0);
+  int depth = 0;
+  for (int l = 0; l <= Ds; l++) {
+    if (lc[l] == 0) break;
+    depth = l;
+    t.lstart.push_back(t.lstart.back() + lc[l]);
+  }
+  t.depth = depth;
+  t.ncells = t.lstart.back();
+
+  // leaves
+  const int nc = int(t.ncells);
+  t.tmp_nch.ensure(nc);
+  t.tmp_off.ensure(nc);
+  t.leaves.ensure(nc);
+  hipLaunchKernelGGL(k_leaf_flag, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, t.cnch.p, t.tmp_nch.p);
+  XF_LAUNCH();
+  scan_exclusive(t.tmp_nch.p, t.tmp_off.p, nc, t.misc.p + 17, s);
+  hipLaunchKernelGGL(k_leaf_compact, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, t.cnch.p, t.tmp_off.p,
+                     t.leaves.p);
+  XF_LAUNCH();
+
+  t.perm.ensure(n);
+  t.xs.ensure(n);
+  t.xq.ensure(n);
+  t.body_leaf.ensure(n);
+  hipLaunchKernelGGL(k_leaf_gather, dim3(nc), dim3(64), sizeof(int) * ncrit, s, t.leaves.p, t.misc.p + 17, t.cbeg.p,
+                     t.ccnt.p, t.cx.p, v1, c.bodies, t.perm.p, t.xs.p, t.xq.p, t.body_leaf.p);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_writeback, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, t.xs.p, c.bodies);
+  XF_LAUNCH();
+  int nl = 0;
+  XF_HIP(hipMemcpyAsync(&nl, t.misc.p + 17, sizeof(int), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipStreamSynchronize(s));
+  t.nleaves = nl;
+  t.built = true;
+}
+
+// Reference numbering: the reference appends a parent's children block
+// when it visits the parent, in DFS preorder (3d/build_tree.h:115-132).
+void tree_ref_order(Ctx &c, std::vector<int64_t> &bfs2ref, std::vector<int64_t> &ref2bfs) {
+  Tree &t = c.t;
+  const int64_t nc = t.ncells;
+  std::vector<int> cchild(nc), cnch(nc);
+  XF_HIP(hipMemcpy(cchild.data(), t.cchild.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+  XF_HIP(hipMemcpy(cnch.data(), t.cnch.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+  bfs2ref.assign(nc, -1);
+  ref2bfs.assign(nc, -1);
+  bfs2ref[0] = 0;
+  int64_t next = 1;
+  std::vector<int> stack{0};
+  while (!stack.empty()) {
+    const int u = stack.back();
+    stack.pop_back();
+    if (cnch[u] == 0) continue;
+    const int64_t first = next;
+    next += cnch[u];
+    for (int k = 0; k < cnch[u]; k++) bfs2ref[cchild[u] + k] = first + k;
+    for (int k = cnch[u] - 1; k >= 0; k--) stack.push_back(cchild[u] + k);
+  }
+  for (int64_t i = 0; i < nc; i++) ref2bfs[bfs2ref[i]] = i;
+}
+
+void tree_export_cells(Ctx &c, exafmm_cell *out) {
+  Tree &t = c.t;
+  const int64_t nc = t.ncells;
+  std::vector<int64_t> b2r, r2b;
+  tree_ref_order(c, b2r, r2b);
+  std::vector<double4> cx(nc);
+  std::vector<int> cbeg(nc), ccnt(nc), cchild(nc), cnch(nc);
+  XF_HIP(hipMemcpy(cx.data(), t.cx.p, sizeof(double4) * nc, hipMemcpyDeviceToHost));
+  XF_HIP(hipMemcpy(cbeg.data(), t.cbeg.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+  XF_HIP(hipMemcpy(ccnt.data(), t.ccnt.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+  XF_HIP(hipMemcpy(cchild.data(), t.cchild.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+  XF_HIP(hipMemcpy(cnch.data(), t.cnch.p, sizeof(int) * nc, hipMemcpyDeviceToHost));
+  for (int64_t b = 0; b < nc; b++) {
+    exafmm_cell &o = out[b2r[b]];
+    o.nchild = cnch[b];
+    o.nbody = ccnt[b];
+    o.child = cnch[b] ? b2r[cchild[b]] : -1;
+    o.body = cbeg[b];
+    o.X[0] = cx[b].x;
+    o.X[1] = cx[b].y;
+    o.X[2] = cx[b].z;
+    o.R = cx[b].w;
+  }
+}
+
+}  // namespace xf

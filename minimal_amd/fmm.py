@@ -1,0 +1,267 @@
+"""Host-side mirror of the reference's evaluation interface over the HIP C ABI.
+
+The reference drives the path with free functions on global state
+(3d/fmm.cxx:40-53): ``buildTree(bodies)``, ``upwardPass(cells)``,
+``horizontalPass(cells, cells)``, ``downwardPass(cells)``,
+``direct(targets, sources)``.  :class:`FMM` keeps that call sequence and
+meaning (bodies are permuted in place by ``build_tree``, p and F are
+accumulated by the passes), with the globals P / ncrit / theta moved into a
+configuration, over ``libexafmm_mi355x.so``.  There is no CPU fallback: if
+the library or a device is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .inputs import BODY_DTYPE
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libexafmm_mi355x.so")
+
+CELL_DTYPE = np.dtype([("nchild", "<i4"), ("nbody", "<i4"), ("child", "<i8"), ("body", "<i8"),
+                       ("X", "<f8", (3,)), ("R", "<f8")])
+
+M2L_REFERENCE, M2L_FULL = 0, 1
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+ORDER_FIXED, ORDER_LEARNED = 0, 1
+
+ERRORS = {-1: "EXAFMM_ERR_ARG", -2: "EXAFMM_ERR_STATE", -3: "EXAFMM_ERR_DEPTH", -4: "EXAFMM_ERR_ALLOC",
+          -5: "EXAFMM_ERR_HIP", -6: "EXAFMM_ERR_CAPACITY", -7: "EXAFMM_ERR_NODEVICE",
+          -8: "EXAFMM_ERR_UNSUPPORTED"}
+
+EXPORTS = [
+    "exafmm_config_default", "exafmm_device_count", "exafmm_last_error", "exafmm_create", "exafmm_destroy",
+    "exafmm_synchronize", "exafmm_build_tree", "exafmm_num_cells", "exafmm_get_cells", "exafmm_upward",
+    "exafmm_horizontal", "exafmm_downward", "exafmm_evaluate", "exafmm_direct", "exafmm_list_sizes",
+    "exafmm_get_lists", "exafmm_get_multipoles", "exafmm_get_locals", "exafmm_stage_times", "exafmm_p2p",
+    "exafmm_p2m", "exafmm_m2m", "exafmm_m2l", "exafmm_l2l", "exafmm_l2p", "exafmm_work_counts",
+]
+
+
+class Config(C.Structure):
+    _fields_ = [("P", C.c_int32), ("ncrit", C.c_int32), ("theta", C.c_double), ("m2l_truncation", C.c_int32),
+                ("m2l_precision", C.c_int32), ("p2p_precision", C.c_int32), ("images", C.c_int32),
+                ("cycle", C.c_double), ("order_policy", C.c_int32), ("device", C.c_int32),
+                ("reserved", C.c_int32 * 6)]
+
+
+class ExaFMMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_P, _I, _L, _D = C.c_void_p, C.c_int, C.c_int64, C.c_double
+
+
+class Library:
+    def __init__(self, path: str = LIBPATH):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built (run __graft_entry__.build())")
+        self.path = path
+        self.lib = C.CDLL(path)
+        L = self.lib
+        L.exafmm_last_error.restype = C.c_char_p
+        L.exafmm_last_error.argtypes = [_P]
+        L.exafmm_config_default.argtypes = [C.POINTER(Config)]
+        L.exafmm_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.exafmm_create.argtypes = [C.POINTER(Config), C.POINTER(_P)]
+        L.exafmm_destroy.argtypes = [_P]
+        L.exafmm_synchronize.argtypes = [_P]
+        L.exafmm_build_tree.argtypes = [_P, _P, _L, _I]
+        L.exafmm_num_cells.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int32)]
+        L.exafmm_get_cells.argtypes = [_P, _P, _L]
+        L.exafmm_upward.argtypes = [_P]
+        L.exafmm_horizontal.argtypes = [_P, _P]
+        L.exafmm_downward.argtypes = [_P]
+        L.exafmm_evaluate.argtypes = [_P, _P, _L, _I]
+        L.exafmm_direct.argtypes = [_P, _P, _L, _P, _L, _I]
+        L.exafmm_list_sizes.argtypes = [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+        L.exafmm_get_lists.argtypes = [_P, _I, _P, _P]
+        L.exafmm_get_multipoles.argtypes = [_P, _P]
+        L.exafmm_get_locals.argtypes = [_P, _P]
+        L.exafmm_stage_times.argtypes = [_P, _P, _I]
+        L.exafmm_work_counts.argtypes = [_P, _P, _I]
+        L.exafmm_p2p.argtypes = [_P, _P, _L, _P, _L]
+        L.exafmm_p2m.argtypes = [_P, _P, _P, _L, _P]
+        L.exafmm_m2m.argtypes = [_P, _P, _P, _P, _P]
+        L.exafmm_m2l.argtypes = [_P, _P, _P, _P, _P]
+        L.exafmm_l2l.argtypes = [_P, _P, _P, _P, _P]
+        L.exafmm_l2p.argtypes = [_P, _P, _P, _P, _L]
+
+    def device_count(self) -> int:
+        n = C.c_int(0)
+        self.lib.exafmm_device_count(C.byref(n))
+        return n.value
+
+    def check(self, code: int, ctx=None):
+        if code != 0:
+            msg = self.lib.exafmm_last_error(ctx)
+            raise ExaFMMError(code, msg.decode() if msg else "")
+
+
+_LIB: Library | None = None
+
+
+def load() -> Library:
+    global _LIB
+    if _LIB is None:
+        _LIB = Library()
+    return _LIB
+
+
+def _ptr(a):
+    if isinstance(a, int):
+        return C.c_void_p(a)
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def default_config(**kw) -> Config:
+    cfg = Config()
+    load().lib.exafmm_config_default(C.byref(cfg))
+    for k, v in kw.items():
+        if not hasattr(cfg, k):
+            raise KeyError(k)
+        setattr(cfg, k, v)
+    return cfg
+
+
+class FMM:
+    """One evaluation context (the reference's globals + Cells, on one device)."""
+
+    def __init__(self, **config):
+        self.L = load()
+        self.cfg = default_config(**config)
+        h = C.c_void_p()
+        self.L.check(self.L.lib.exafmm_create(C.byref(self.cfg), C.byref(h)))
+        self.h = h
+        self.P = self.cfg.P
+        self.nterm = self.P * (self.P + 1) // 2
+        self._keep = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lib.exafmm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, code):
+        self.L.check(code, self.h)
+
+    # ---- passes, host numpy arrays or device pointers (int)
+    def build_tree(self, bodies, n=None, on_device=False):
+        n = len(bodies) if n is None else n
+        self._keep = bodies
+        self._chk(self.L.lib.exafmm_build_tree(self.h, _ptr(bodies), n, int(on_device)))
+
+    def upward(self):
+        self._chk(self.L.lib.exafmm_upward(self.h))
+
+    def horizontal(self, source: "FMM | None" = None):
+        self._chk(self.L.lib.exafmm_horizontal(self.h, (source or self).h))
+
+    def downward(self):
+        self._chk(self.L.lib.exafmm_downward(self.h))
+
+    def evaluate(self, bodies, n=None, on_device=False):
+        n = len(bodies) if n is None else n
+        self._keep = bodies
+        self._chk(self.L.lib.exafmm_evaluate(self.h, _ptr(bodies), n, int(on_device)))
+
+    def direct(self, targets, sources, nt=None, ns=None, on_device=False):
+        nt = len(targets) if nt is None else nt
+        ns = len(sources) if ns is None else ns
+        self._chk(self.L.lib.exafmm_direct(self.h, _ptr(targets), nt, _ptr(sources), ns, int(on_device)))
+
+    def synchronize(self):
+        self._chk(self.L.lib.exafmm_synchronize(self.h))
+
+    # ---- inspection
+    def num_cells(self):
+        n = C.c_int64()
+        lv = C.c_int32()
+        self._chk(self.L.lib.exafmm_num_cells(self.h, C.byref(n), C.byref(lv)))
+        return n.value, lv.value
+
+    def cells(self) -> np.ndarray:
+        n, _ = self.num_cells()
+        out = np.zeros(n, CELL_DTYPE)
+        self._chk(self.L.lib.exafmm_get_cells(self.h, _ptr(out), n))
+        return out
+
+    def lists(self, which: int):
+        nm = C.c_int64()
+        npp = C.c_int64()
+        self._chk(self.L.lib.exafmm_list_sizes(self.h, C.byref(nm), C.byref(npp)))
+        n, _ = self.num_cells()
+        tot = nm.value if which == 0 else npp.value
+        off = np.zeros(n + 1, np.int64)
+        src = np.zeros(max(tot, 1), np.int64)
+        self._chk(self.L.lib.exafmm_get_lists(self.h, which, _ptr(off), _ptr(src)))
+        return off, src[:tot]
+
+    def multipoles(self):
+        n, _ = self.num_cells()
+        out = np.zeros(n * self.nterm, np.complex128)
+        self._chk(self.L.lib.exafmm_get_multipoles(self.h, _ptr(out)))
+        return out.reshape(n, self.nterm)
+
+    def locals(self):
+        n, _ = self.num_cells()
+        out = np.zeros(n * self.nterm, np.complex128)
+        self._chk(self.L.lib.exafmm_get_locals(self.h, _ptr(out)))
+        return out.reshape(n, self.nterm)
+
+    def stage_times(self) -> dict:
+        v = np.zeros(8)
+        self._chk(self.L.lib.exafmm_stage_times(self.h, _ptr(v), 8))
+        keys = ["tree", "lists", "upward", "m2l", "p2p", "downward", "total"]
+        return {k: float(x) for k, x in zip(keys, v)}
+
+    def work_counts(self) -> dict:
+        v = np.zeros(8, np.int64)
+        self._chk(self.L.lib.exafmm_work_counts(self.h, _ptr(v), 8))
+        keys = ["m2l_pairs", "p2p_pairs", "p2p_interactions", "p2p_source_bodies", "cells", "leaves", "m2l_keys",
+                "depth"]
+        return {k: int(x) for k, x in zip(keys, v)}
+
+    # ---- kernel.h operator surface (host arrays)
+    def p2p(self, targets, sources):
+        self._chk(self.L.lib.exafmm_p2p(self.h, _ptr(targets), len(targets), _ptr(sources), len(sources)))
+
+    def p2m(self, center, bodies, M=None):
+        M = np.zeros(self.nterm, np.complex128) if M is None else np.array(M, np.complex128)
+        c = np.ascontiguousarray(center, np.float64)
+        self._chk(self.L.lib.exafmm_p2m(self.h, _ptr(c), _ptr(bodies), len(bodies), _ptr(M)))
+        return M
+
+    def m2m(self, cparent, cchild, Mchild, Mparent=None):
+        Mp = np.zeros(self.nterm, np.complex128) if Mparent is None else np.array(Mparent, np.complex128)
+        a, b = np.ascontiguousarray(cparent, np.float64), np.ascontiguousarray(cchild, np.float64)
+        Mc = np.ascontiguousarray(Mchild, np.complex128)
+        self._chk(self.L.lib.exafmm_m2m(self.h, _ptr(a), _ptr(b), _ptr(Mc), _ptr(Mp)))
+        return Mp
+
+    def m2l(self, Xi, Xj, Mj, Li=None):
+        Lo = np.zeros(self.nterm, np.complex128) if Li is None else np.array(Li, np.complex128)
+        a, b = np.ascontiguousarray(Xi, np.float64), np.ascontiguousarray(Xj, np.float64)
+        Mc = np.ascontiguousarray(Mj, np.complex128)
+        self._chk(self.L.lib.exafmm_m2l(self.h, _ptr(a), _ptr(b), _ptr(Mc), _ptr(Lo)))
+        return Lo
+
+    def l2l(self, cparent, Lparent, cchild, Lchild=None):
+        Lc = np.zeros(self.nterm, np.complex128) if Lchild is None else np.array(Lchild, np.complex128)
+        a, b = np.ascontiguousarray(cparent, np.float64), np.ascontiguousarray(cchild, np.float64)
+        Lp = np.ascontiguousarray(Lparent, np.complex128)
+        self._chk(self.L.lib.exafmm_l2l(self.h, _ptr(a), _ptr(Lp), _ptr(b), _ptr(Lc)))
+        return Lc
+
+    def l2p(self, center, L, bodies):
+        c = np.ascontiguousarray(center, np.float64)
+        Lc = np.ascontiguousarray(L, np.complex128)
+        self._chk(self.L.lib.exafmm_l2p(self.h, _ptr(c), _ptr(Lc), _ptr(bodies), len(bodies)))

@@ -1,0 +1,155 @@
+"""HIP path against the reference fixtures and the FP64 oracle.
+
+Bars (written in each test):
+  tree, body permutation           bit-exact
+  M2L / P2P lists                  equal as sets per target cell
+  FP64 mode (M2L, P2P in FP64)     |diff| <= 1e-11 * max |ref|   (reference truncation, vs fixtures)
+  upward multipoles (FP64)         <= 1e-12 relative
+  FP32 P2P                         relative L2 of p, F vs oracle <= 2e-6
+  direct sum (FP64)                <= 1e-13 relative
+  operator surface (FP64)          <= 1e-12 relative
+"""
+import numpy as np
+import pytest
+
+from conftest import golden
+from minimal_amd import fmm
+from minimal_amd.inputs import BODY_DTYPE, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TREES = ["config1_n10000.npz", "uniform_n1000_p4.npz", "plummer_n4096_p6.npz"]
+
+
+def _ctx(g, **kw):
+    return fmm.FMM(P=int(g["P"]), ncrit=int(g["ncrit"]), theta=float(g["theta"]), **kw)
+
+
+@pytest.mark.parametrize("name", TREES)
+def test_tree_bitexact(gpu, name):
+    g = golden(name)
+    f = _ctx(g)
+    b = g["input"].copy()
+    f.build_tree(b)
+    assert np.array_equal(b["X"], g["sorted_xq"][:, :3])
+    assert np.array_equal(b["q"], g["sorted_xq"][:, 3])
+    assert np.array_equal(b["p"], g["input"]["p"])  # p, F stay in place
+    c = f.cells()
+    assert len(c) == len(g["cells_nbody"])
+    for k in ("nchild", "nbody", "child", "body"):
+        assert np.array_equal(c[k], g["cells_" + k]), k
+    assert np.array_equal(c["X"], g["cells_X"])
+    assert np.array_equal(c["R"], g["cells_R"])
+
+
+@pytest.mark.parametrize("name", TREES)
+def test_lists_set_equal(gpu, name):
+    g = golden(name)
+    f = _ctx(g, m2l_precision=fmm.PREC_FP64, p2p_precision=fmm.PREC_FP64, m2l_truncation=fmm.M2L_REFERENCE)
+    b = g["input"].copy()
+    f.build_tree(b)
+    f.upward()
+    f.horizontal()
+    for which, (ok, sk) in ((0, ("m2l_off", "m2l_src")), (1, ("p2p_off", "p2p_src"))):
+        off, src = f.lists(which)
+        go, gs = g[ok], g[sk]
+        assert len(src) == len(gs)
+        for c in range(len(off) - 1):
+            a = np.sort(src[off[c]:off[c + 1]])
+            r = np.sort(gs[go[c]:go[c + 1]])
+            assert np.array_equal(a, r), (which, c)
+
+
+@pytest.mark.parametrize("name", TREES)
+def test_fp64_mode_matches_reference(gpu, name):
+    """Reference truncation, FP64 M2L and P2P: the whole evaluation agrees
+    with the reference's p and F to FP64 rounding."""
+    g = golden(name)
+    f = _ctx(g, m2l_precision=fmm.PREC_FP64, p2p_precision=fmm.PREC_FP64, m2l_truncation=fmm.M2L_REFERENCE)
+    b = g["input"].copy()
+    f.evaluate(b)
+    assert np.max(np.abs(b["p"] - g["p"])) <= 1e-11 * np.max(np.abs(g["p"]))
+    assert np.max(np.abs(b["F"] - g["F"])) <= 1e-11 * np.max(np.abs(g["F"]))
+
+
+def test_multipoles_match_oracle(gpu, orc):
+    g = golden("config1_n10000.npz")
+    f = _ctx(g)
+    b = g["input"].copy()
+    f.build_tree(b)
+    f.upward()
+    M = f.multipoles()
+    s = orc.session(int(g["P"]), int(g["ncrit"]), float(g["theta"]))
+    b2 = g["input"].copy()
+    s.build_tree(b2)
+    s.upward(b2)
+    Mo = s.M()
+    scale = np.max(np.abs(Mo), axis=1, keepdims=True)
+    assert np.max(np.abs(M - Mo) / scale) <= 1e-12
+
+
+@pytest.mark.parametrize("name", TREES)
+def test_fp32_p2p_vs_oracle(gpu, orc, name):
+    g = golden(name)
+    P, nc, th = int(g["P"]), int(g["ncrit"]), float(g["theta"])
+    f = _ctx(g, m2l_precision=fmm.PREC_FP64, p2p_precision=fmm.PREC_FP32, m2l_truncation=fmm.M2L_REFERENCE)
+    b = g["input"].copy()
+    f.evaluate(b)
+    assert rel_l2(b["p"], g["p"]) <= 2e-6
+    assert rel_l2(b["F"], g["F"]) <= 2e-6
+
+
+def test_direct_fp64(gpu, orc):
+    g = golden("driver_n10000.npz")
+    src = g["input"].copy()
+    tg = src[::97].copy()
+    tg["p"] = 0
+    tg["F"] = 0
+    ref = tg.copy()
+    orc.p2p(ref, src)
+    f = fmm.FMM()
+    f.direct(tg, src)
+    assert np.max(np.abs(tg["p"] - ref["p"])) <= 1e-13 * np.max(np.abs(ref["p"]))
+    assert np.max(np.abs(tg["F"] - ref["F"])) <= 1e-13 * np.max(np.abs(ref["F"]))
+
+
+@pytest.mark.parametrize("P", [4, 10, 14])
+def test_operator_surface(gpu, P):
+    g = golden(f"ops_P{P}.npz")
+    f = fmm.FMM(P=P, m2l_truncation=fmm.M2L_REFERENCE)
+
+    def close(a, b, rt=1e-12):
+        assert np.max(np.abs(np.asarray(a) - np.asarray(b))) <= rt * np.max(np.abs(b))
+
+    close(f.p2m(g["p2m_center"], g["p2m_bodies"]), g["p2m_M"])
+    close(f.m2m(g["m2m_parent"], g["m2m_child"], g["m2m_Mchild"], g["m2m_M0"]), g["m2m_M"])
+    close(f.m2l(g["m2l_Xi"], g["m2l_Xj"], g["m2m_M"], g["m2l_L0"]), g["m2l_L"])
+    close(f.l2l(g["m2l_Xi"], g["m2l_L"], g["l2l_child"], g["l2l_Lc0"]), g["l2l_L"])
+    tb = g["l2p_bodies_in"].copy()
+    f.l2p(g["l2l_child"], g["l2l_L"], tb)
+    close(tb["p"], g["l2p_bodies_out"]["p"])
+    close(tb["F"], g["l2p_bodies_out"]["F"])
+    tg = g["p2p_tgt_in"].copy()
+    f.p2p(tg, g["p2p_src"])
+    close(tg["p"], g["p2p_tgt_out"]["p"], 1e-14)
+    close(tg["F"], g["p2p_tgt_out"]["F"], 1e-14)
+
+
+def test_errors_are_status_codes(gpu):
+    f = fmm.FMM()
+    with pytest.raises(fmm.ExaFMMError) as e:
+        f.upward()
+    assert e.value.code == -2
+    b = np.zeros(10, BODY_DTYPE)
+    b["X"][3, 1] = np.nan
+    with pytest.raises(fmm.ExaFMMError) as e:
+        f.build_tree(b)
+    assert e.value.code == -1
+    # more than ncrit coincident bodies: no finite tree depth
+    c = np.zeros(200, BODY_DTYPE)
+    c["X"] = 0.5
+    c["X"][0] = 0.1
+    g = fmm.FMM(ncrit=8)
+    with pytest.raises(fmm.ExaFMMError) as e:
+        g.build_tree(c)
+    assert e.value.code == -3
