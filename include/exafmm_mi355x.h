@@ -133,6 +133,19 @@ int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_devi
 int exafmm_direct(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources,
                   int64_t ns, int on_device);
 
+/* Periodic contexts (config.images >= 1, 3dp semantics): build_tree,
+ * upward and downward are unchanged; horizontal visits the 27 near images
+ * (3dp/traverse_lazy.h:124-135) and then the images-1 far levels
+ * (3dp/traverse_lazy.h:75-121).  The dipole correction of 3dp/fmm.cxx:63-75
+ * is a separate call made after downward. */
+int exafmm_dipole_correction(exafmm_ctx *ctx);
+/* Ewald summation oracle (3dp/ewald.h:22-150) in FP64 on the device, with
+ * the context's periodic cycle: p and F of the targets += Ewald sums over
+ * the sources (wave part over |k| <= ksize, real part within cutoff using
+ * each pair's nearest image, self term). */
+int exafmm_ewald(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources, int64_t ns,
+                 int ksize, double alpha, double sigma, double cutoff, int on_device);
+
 /* Interaction lists of the last horizontal pass (Cell::listM2L /
  * Cell::listP2P, 3d/exafmm.h:30-33), reference cell numbering, CSR by
  * target cell (offsets has ncells+1 entries). */
@@ -144,7 +157,8 @@ int exafmm_get_multipoles(exafmm_ctx *ctx, double *out);
 int exafmm_get_locals(exafmm_ctx *ctx, double *out);
 
 /* Stage times (ms, device clock) of the last build/upward/horizontal/
- * downward calls: [tree, lists, p2m+m2m, m2l, p2p, l2l+l2p, total]. */
+ * downward calls: [tree, lists, p2m+m2m, m2l, p2p, l2l+l2p, total,
+ * m2l kernel alone]. */
 int exafmm_stage_times(exafmm_ctx *ctx, double *ms, int capacity);
 
 /* Work of the last horizontal pass, for throughput and roofline figures:

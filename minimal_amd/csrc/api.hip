@@ -3,6 +3,8 @@
 // downwardPass / direct, exports in the reference's numbering, and stage
 // timing with HIP events.
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <exception>
 
 #include "fmm_internal.h"
@@ -103,27 +105,47 @@ void do_upward(Ctx &c) {
   rec(c, 3, c.s0);
 }
 
-void do_horizontal(Ctx &tc, const Ctx &sc) {
+// Interaction lists (geometry only: no multipoles needed).
+void do_lists(Ctx &tc, const Ctx &sc) {
   if (!tc.t.built || !sc.t.built) fail(EXAFMM_ERR_STATE, "horizontal pass before build_tree");
-  if (!sc.have_M) fail(EXAFMM_ERR_STATE, "horizontal pass before the source's upward pass");
   if (tc.P != sc.P) fail(EXAFMM_ERR_ARG, "target and source contexts differ in P");
-  if (tc.cfg.images != 0 || sc.cfg.images != 0) fail(EXAFMM_ERR_UNSUPPORTED, "periodic horizontal pass");
+  if (tc.cfg.images != sc.cfg.images || (tc.cfg.images > 0 && &tc != &sc))
+    fail(EXAFMM_ERR_UNSUPPORTED, "a periodic horizontal pass needs source == target");
   rec(tc, 11, tc.s0);
   traverse(tc, sc);
   rec(tc, 12, tc.s0);
-  tc.L.ensure(size_t(tc.t.ncells) * tc.NT);
-  XF_HIP(hipMemsetAsync(tc.L.p, 0, sizeof(double2) * tc.t.ncells * tc.NT, tc.s0));
-  // P2P on the second stream, concurrent with M2L (VALU vs MFMA work)
-  XF_HIP(hipStreamWaitEvent(tc.s1, tc.ev[12], 0));
+  tc.L.ensure(size_t(tc.t.ncells) * tc.P * tc.P);
+  XF_HIP(hipMemsetAsync(tc.L.p, 0, sizeof(double) * tc.t.ncells * tc.P * tc.P, tc.s0));
+}
+
+// P2P on the second stream from the current point of the main stream, so it
+// overlaps M2L (VALU against MFMA work).
+void do_p2p(Ctx &tc, const Ctx &sc) {
+  rec(tc, 14, tc.s0);
+  XF_HIP(hipStreamWaitEvent(tc.s1, tc.ev[14], 0));
   rec(tc, 5, tc.s1);
   p2p_apply(tc, sc, tc.s1);
   rec(tc, 6, tc.s1);
+}
+
+// M2L (and the periodic far field) on the main stream; joins the P2P stream.
+void do_far(Ctx &tc, const Ctx &sc) {
+  if (!sc.have_M) fail(EXAFMM_ERR_STATE, "horizontal pass before the source's upward pass");
   rec(tc, 9, tc.s0);
+  rec(tc, 13, tc.s0);
   m2l_apply(tc, sc);
   rec(tc, 4, tc.s0);
+  if (tc.cfg.images > 0) periodic_far(tc);
   XF_HIP(hipStreamWaitEvent(tc.s0, tc.ev[6], 0));
   tc.src = &sc;
   tc.have_L = true;
+}
+
+void do_horizontal(Ctx &tc, const Ctx &sc) {
+  if (!sc.have_M) fail(EXAFMM_ERR_STATE, "horizontal pass before the source's upward pass");
+  do_lists(tc, sc);
+  do_p2p(tc, sc);
+  do_far(tc, sc);
 }
 
 void do_downward(Ctx &c) {
@@ -190,7 +212,13 @@ int exafmm_create(const exafmm_config *cfg, exafmm_ctx **out) {
     c.NT = cfg->P * (cfg->P + 1) / 2;
     try {
       XF_HIP(hipStreamCreateWithFlags(&c.s0, hipStreamNonBlocking));
-      XF_HIP(hipStreamCreateWithFlags(&c.s1, hipStreamNonBlocking));
+      // EXAFMM_SERIAL_STREAMS=1 runs P2P on the main stream (per-kernel
+      // profiles without the P2P / M2L overlap)
+      const char *serial = std::getenv("EXAFMM_SERIAL_STREAMS");
+      if (serial && serial[0] == '1')
+        c.s1 = c.s0;
+      else
+        XF_HIP(hipStreamCreateWithFlags(&c.s1, hipStreamNonBlocking));
       for (auto &e : c.ev) XF_HIP(hipEventCreate(&e));
     } catch (...) {
       delete h;
@@ -208,7 +236,7 @@ int exafmm_destroy(exafmm_ctx *ctx) {
   for (auto &e : ctx->c.ev)
     if (e) hipEventDestroy(e);
   if (ctx->c.s0) hipStreamDestroy(ctx->c.s0);
-  if (ctx->c.s1) hipStreamDestroy(ctx->c.s1);
+  if (ctx->c.s1 && ctx->c.s1 != ctx->c.s0) hipStreamDestroy(ctx->c.s1);
   delete ctx;
   return EXAFMM_OK;
 }
@@ -280,8 +308,10 @@ int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_devi
     Ctx &c = ctx->c;
     set_bodies(c, bodies, n, on_device);
     do_build(c);
+    do_lists(c, c);
     do_upward(c);
-    do_horizontal(c, c);
+    do_p2p(c, c);
+    do_far(c, c);
     do_downward(c);
     bodies_to_host(c);
   });
@@ -294,7 +324,7 @@ int exafmm_stage_times(exafmm_ctx *ctx, double *ms, int capacity) {
     XF_HIP(hipStreamSynchronize(c.s1));
     XF_HIP(hipStreamSynchronize(c.s0));
     const double v[8] = {elapsed(c, 0, 1), elapsed(c, 11, 12), elapsed(c, 2, 3), elapsed(c, 9, 4),
-                         elapsed(c, 5, 6), elapsed(c, 10, 8), elapsed(c, 0, 8), 0};
+                         elapsed(c, 5, 6), elapsed(c, 10, 8), elapsed(c, 0, 8), elapsed(c, 13, 4)};
     for (int i = 0; i < std::min(capacity, 8); i++) ms[i] = v[i];
   });
 }
@@ -309,6 +339,39 @@ int exafmm_direct(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafm
       XF_HIP(hipStreamSynchronize(c.s0));
     } else {
       op_p2p(c, targets, nt, sources, ns);
+    }
+  });
+}
+
+int exafmm_dipole_correction(exafmm_ctx *ctx) {
+  return guard(ctx, [&] {
+    if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
+    Ctx &c = ctx->c;
+    if (c.cfg.images < 1) fail(EXAFMM_ERR_STATE, "dipole correction needs a periodic context");
+    if (!c.t.built) fail(EXAFMM_ERR_STATE, "no bodies");
+    dipole_correction(c);
+    bodies_to_host(c);
+    XF_HIP(hipStreamSynchronize(c.s0));
+  });
+}
+
+int exafmm_ewald(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources, int64_t ns,
+                 int ksize, double alpha, double sigma, double cutoff, int on_device) {
+  return guard(ctx, [&] {
+    if (!ctx || !targets || !sources || nt < 0 || ns < 0 || ksize < 0 || !(alpha > 0) || !(sigma > 0))
+      fail(EXAFMM_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    if (!(c.cfg.cycle > 0)) fail(EXAFMM_ERR_ARG, "Ewald sum needs the periodic cycle in the configuration");
+    if (on_device) {
+      ewald_sum(c, targets, nt, sources, ns, ksize, alpha, sigma, cutoff, c.cfg.cycle);
+    } else {
+      Buf<exafmm_body> dt, ds;
+      dt.ensure(nt);
+      ds.ensure(ns);
+      XF_HIP(hipMemcpyAsync(dt.p, targets, sizeof(exafmm_body) * nt, hipMemcpyHostToDevice, c.s0));
+      XF_HIP(hipMemcpyAsync(ds.p, sources, sizeof(exafmm_body) * ns, hipMemcpyHostToDevice, c.s0));
+      ewald_sum(c, dt.p, nt, ds.p, ns, ksize, alpha, sigma, cutoff, c.cfg.cycle);
+      XF_HIP(hipMemcpy(targets, dt.p, sizeof(exafmm_body) * nt, hipMemcpyDeviceToHost));
     }
   });
 }
@@ -346,9 +409,10 @@ int exafmm_get_lists(exafmm_ctx *ctx, int which, int64_t *offsets, int64_t *sour
     if (which == 0) {
       std::vector<int2> e(std::max<int64_t>(tot, 1));
       if (tot) XF_HIP(hipMemcpy(e.data(), L.m2l.p, sizeof(int2) * tot, hipMemcpyDeviceToHost));
-      for (int64_t i = 0; i < tot; i++) src[i] = e[i].x;
+      for (int64_t i = 0; i < tot; i++) src[i] = e[i].x >> 5;
     } else if (tot) {
       XF_HIP(hipMemcpy(src.data(), L.p2p.p, sizeof(int) * tot, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < tot; i++) src[i] >>= 5;
     }
     int64_t o = 0;
     for (int64_t r = 0; r < nc; r++) {
@@ -378,7 +442,7 @@ int exafmm_work_counts(exafmm_ctx *ctx, int64_t *out, int capacity) {
     int64_t inter = 0, nj = 0;
     for (int64_t i = 0; i < nc; i++)
       for (int k = 0; k < cnt[i]; k++) {
-        const int64_t s = sccnt[src[beg[i] + k]];
+        const int64_t s = sccnt[src[beg[i] + k] >> 5];
         inter += int64_t(ccnt[i]) * s;
         nj += s;
       }
@@ -387,24 +451,38 @@ int exafmm_work_counts(exafmm_ctx *ctx, int64_t *out, int capacity) {
   });
 }
 
-static void export_coef(exafmm_ctx *ctx, const Buf<double2> &B, double *out) {
+// Scaled real-form coefficients -> complex M_n^m = M~ R^n or L_j^k = L~ / R^(j+1),
+// reference cell numbering.
+static void export_coef(exafmm_ctx *ctx, const Buf<double> &B, bool locals, double *out) {
   Ctx &c = ctx->c;
   XF_HIP(hipStreamSynchronize(c.s1));
   XF_HIP(hipStreamSynchronize(c.s0));
   const int64_t nc = c.t.ncells;
+  const int P = c.P, PP = P * P, NT = c.NT;
   std::vector<int64_t> b2r, r2b;
   tree_ref_order(c, b2r, r2b);
-  std::vector<double2> h(size_t(nc) * c.NT);
-  XF_HIP(hipMemcpy(h.data(), B.p, sizeof(double2) * h.size(), hipMemcpyDeviceToHost));
-  for (int64_t b = 0; b < nc; b++)
-    std::memcpy(out + 2 * c.NT * b2r[b], h.data() + size_t(b) * c.NT, sizeof(double2) * c.NT);
+  std::vector<double> h(size_t(nc) * PP);
+  std::vector<double4> cx(nc);
+  XF_HIP(hipMemcpy(h.data(), B.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+  XF_HIP(hipMemcpy(cx.data(), c.t.cx.p, sizeof(double4) * nc, hipMemcpyDeviceToHost));
+  for (int64_t b = 0; b < nc; b++) {
+    double *o = out + 2 * NT * b2r[b];
+    const double *v = h.data() + size_t(b) * PP;
+    for (int n = 0; n < P; n++)
+      for (int m = 0; m <= n; m++) {
+        const double s = locals ? std::pow(cx[b].w, -double(n + 1)) : std::pow(cx[b].w, double(n));
+        const int i = xfd::nms(n, m);
+        o[2 * i] = v[xfd::rf(n, m, 0)] * s;
+        o[2 * i + 1] = m ? v[xfd::rf(n, m, 1)] * s : 0.0;
+      }
+  }
 }
 
 int exafmm_get_multipoles(exafmm_ctx *ctx, double *out) {
   return guard(ctx, [&] {
     if (!ctx || !out) fail(EXAFMM_ERR_ARG, "null argument");
     if (!ctx->c.have_M) fail(EXAFMM_ERR_STATE, "no multipoles");
-    export_coef(ctx, ctx->c.M, out);
+    export_coef(ctx, ctx->c.M, false, out);
   });
 }
 
@@ -412,7 +490,7 @@ int exafmm_get_locals(exafmm_ctx *ctx, double *out) {
   return guard(ctx, [&] {
     if (!ctx || !out) fail(EXAFMM_ERR_ARG, "null argument");
     if (!ctx->c.have_L) fail(EXAFMM_ERR_STATE, "no locals");
-    export_coef(ctx, ctx->c.L, out);
+    export_coef(ctx, ctx->c.L, true, out);
   });
 }
 

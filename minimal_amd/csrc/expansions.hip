@@ -2,12 +2,21 @@
 //
 // Reference operators: P2M 3d/kernel.h:187-202, M2M :204-231, L2L :262-291,
 // L2P :293-322; passes 3d/traverse_lazy.h:7-24 (upward) and :65-80
-// (downward).  Harmonics are evaluated with the trig-free Cartesian
-// recurrences of fmm_internal.h (same functions, same normalisation).
-// P2M, M2M and L2L run in FP64; L2P evaluates the local expansion in FP32
-// on radius-scaled coordinates and adds into the FP64 body accumulators,
-// with the force taken from Cartesian ladder identities of the solid
-// harmonics (finite at r = 0 and on the z axis, where the reference's
+// (downward).
+//
+// Storage: every cell's expansions are kept in radius-scaled real form,
+//   M~[rf(n,m,.)] = (Re, Im) M_n^m / R^n,   L~[rf(j,k,.)] = (Re, Im) L_j^k R^(j+1),
+// P^2 doubles per cell (the imaginary parts of the m = 0 coefficients are
+// zero for real charges and are not stored).  In this form the child <-> parent
+// shifts do not depend on the level: M2M and L2L are one fixed P^2 x P^2 FP64
+// matrix per octant, generated once from the reference's shift formulas
+// (m2m_coef / l2l_coef applied to unit vectors) and applied as batched
+// matrix-vector products.  Harmonics come from the trig-free Cartesian
+// recurrences of fmm_internal.h (same functions and normalisation as the
+// reference's spherical evaluation).  P2M, M2M and L2L run in FP64; L2P
+// evaluates in FP32 (FP64 in parity mode) and adds into FP64 body
+// accumulators, with the force from Cartesian ladder identities of the
+// solid harmonics (finite at r = 0 and on the z axis, where the reference's
 // spherical form divides by zero, 3d/kernel.h:31-40).
 #include "fmm_internal.h"
 #include "fmm_ops.h"
@@ -16,181 +25,422 @@ namespace xf {
 using namespace xfd;
 
 // ---------------------------------------------------------------- P2M
-// One thread per (leaf, order m): the column n = m..P-1 of
-// conj(Y_n^m(x_b - X_c)) summed over the leaf's bodies with weight q
-// (3d/kernel.h:193 evaluates Y at -beta, i.e. the conjugate).
-__global__ void __launch_bounds__(256) k_p2m(int P, const int *__restrict__ leaves, const int *__restrict__ nleaves,
-                                             const int *__restrict__ cbeg, const int *__restrict__ ccnt,
-                                             const double4 *__restrict__ cx, const double4 *__restrict__ xs,
-                                             double2 *__restrict__ M, int NT) {
-  const int g = blockIdx.x * 256 + threadIdx.x;
-  const int li = g / P, m = g % P;
+// One wave per leaf: the leaf's bodies are staged in LDS 64 at a time; lane
+// (g, m) = (lane / 16, lane % 16) sums column n = m..P-1 of conj(Y_n^m(u)),
+// u = (x_b - X_c)/R, weighted by q over the bodies b = g mod 4
+// (3d/kernel.h:193 evaluates Y at -beta, i.e. the conjugate); the four body
+// groups are then added with cross-lane shuffles in a fixed order.
+constexpr int kP2MWaves = 4;
+
+template <int PMAX>
+__global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m(int P, const int *__restrict__ leaves,
+                                                         const int *__restrict__ nleaves,
+                                                         const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                                         const double4 *__restrict__ cx,
+                                                         const double4 *__restrict__ xs, double *__restrict__ M) {
+  static_assert(PMAX <= 16, "16 orders per body group");
+  __shared__ double4 sb[kP2MWaves][kWave];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int li = blockIdx.x * kP2MWaves + w;
   if (li >= *nleaves) return;
   const int c = leaves[li];
   const int beg = cbeg[c], cnt = ccnt[c];
   const double4 C = cx[c];
-  double ar[kMaxP], ai[kMaxP];
-  for (int n = 0; n < P; n++) ar[n] = ai[n] = 0;
-  for (int b = beg; b < beg + cnt; b++) {
-    const double4 B = xs[b];
-    const double x = B.x - C.x, y = -(B.y - C.y), z = B.z - C.z;  // conjugate: y -> -y
-    const double r2 = x * x + y * y + z * z;
-    double yr = 1, yi = 0;
-    for (int k = 1; k <= m; k++) {
-      const double inv = 1.0 / (2 * k);
-      const double tr = (yr * x - yi * y) * inv, ti = (yr * y + yi * x) * inv;
-      yr = tr;
-      yi = ti;
-    }
-    double pr = 0, pi = 0;  // Y_{n-2}
-    double cr = yr, ci = yi;
-    ar[m] += B.w * cr;
-    ai[m] += B.w * ci;
-    for (int n = m + 1; n < P; n++) {
-      double nr, ni;
-      if (n == m + 1) {
-        nr = -z * cr;
-        ni = -z * ci;
-      } else {
-        const double s = -1.0 / double((n + m) * (n - m));
-        const double cz = double(2 * n - 1) * z;
-        nr = (cz * cr + r2 * pr) * s;
-        ni = (cz * ci + r2 * pi) * s;
+  const double invR = 1.0 / C.w;
+  const int m = lane & 15, g = lane >> 4;
+  double ar[PMAX], ai[PMAX];
+#pragma unroll
+  for (int n = 0; n < PMAX; n++) ar[n] = ai[n] = 0;
+  for (int b0 = 0; b0 < cnt; b0 += kWave) {
+    const int nb = min(kWave, cnt - b0);
+    if (lane < nb) sb[w][lane] = xs[beg + b0 + lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (m < P) {
+      for (int bi = g; bi < nb; bi += 4) {
+        const double4 B = sb[w][bi];
+        const double x = (B.x - C.x) * invR, y = -(B.y - C.y) * invR, z = (B.z - C.z) * invR;  // conjugate
+        const double r2 = x * x + y * y + z * z;
+        double yr = B.w, yi = 0;  // q * Y_m^m
+        for (int k = 1; k <= m; k++) {
+          const double inv = 1.0 / (2 * k);
+          const double tr = (yr * x - yi * y) * inv, ti = (yr * y + yi * x) * inv;
+          yr = tr;
+          yi = ti;
+        }
+        double pr = 0, pi = 0, cr = yr, ci = yi;
+#pragma unroll
+        for (int n = 0; n < PMAX; n++) {
+          if (n < m || n >= P) continue;
+          if (n > m) {
+            double nr, ni;
+            if (n == m + 1) {
+              nr = -z * cr;
+              ni = -z * ci;
+            } else {
+              const double s = -1.0 / double((n + m) * (n - m));
+              const double cz = double(2 * n - 1) * z;
+              nr = (cz * cr + r2 * pr) * s;
+              ni = (cz * ci + r2 * pi) * s;
+            }
+            pr = cr;
+            pi = ci;
+            cr = nr;
+            ci = ni;
+          }
+          ar[n] += cr;
+          ai[n] += ci;
+        }
       }
-      pr = cr;
-      pi = ci;
-      cr = nr;
-      ci = ni;
-      ar[n] += B.w * cr;
-      ai[n] += B.w * ci;
     }
+    __builtin_amdgcn_wave_barrier();
   }
-  double2 *Mc = M + int64_t(c) * NT;
-  for (int n = m; n < P; n++) Mc[nms(n, m)] = make_double2(ar[n], ai[n]);
+#pragma unroll
+  for (int n = 0; n < PMAX; n++) {
+    ar[n] += __shfl_xor(ar[n], 16);
+    ai[n] += __shfl_xor(ai[n], 16);
+    ar[n] += __shfl_xor(ar[n], 32);
+    ai[n] += __shfl_xor(ai[n], 32);
+  }
+  if (g != 0 || m >= P) return;
+  double *Mc = M + int64_t(c) * P * P;
+#pragma unroll
+  for (int n = 0; n < PMAX; n++) {
+    if (n < m || n >= P) continue;
+    Mc[rf(n, m, 0)] = ar[n];
+    if (m > 0) Mc[rf(n, m, 1)] = ai[n];
+  }
 }
 
-// Regular harmonics of the 8 parent<->child shift vectors of one level
-// into LDS: Y[o][nms], shift = sgn * (+/-r) per axis by octant o.
-__device__ inline void octant_tables(int P, int NT, double r, double sgn, cpx<double> *Y) {
-  const int t = threadIdx.x;
-  if (t < 8) {
-    const double x = sgn * ((t & 1) ? r : -r), y = sgn * ((t & 2) ? r : -r), z = sgn * ((t & 4) ? r : -r);
-    regular_all<double>(P, x, y, z, Y + t * NT);
+// ---------------------------------------------------------------- shift operators
+// Column `col` of the scaled M2M (kind 0) or L2L (kind 1) operator of octant
+// o, for a parent of radius 1 and a child of radius 1/2 centred at
+// parent + (+/-1/2) per axis.  T[kind][o][col][row] (row fastest).
+__global__ void k_shift_ops(int P, double *__restrict__ T) {
+  const int PP = P * P, NT = P * (P + 1) / 2;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * 8 * PP) return;
+  const int kind = t / (8 * PP), o = (t / PP) % 8, col = t % PP;
+  cpx<double> Y[kMaxP * (kMaxP + 1) / 2];
+  double2 in[kMaxP * (kMaxP + 1) / 2];
+  for (int i = 0; i < NT; i++) in[i] = make_double2(0, 0);
+  int n = 0;
+  while ((n + 1) * (n + 1) <= col) n++;
+  const int rem = col - n * n, m = (rem + 1) / 2;
+  const bool im = rem > 0 && rem % 2 == 0;
+  const double ox = (o & 1) ? 0.5 : -0.5, oy = (o & 2) ? 0.5 : -0.5, oz = (o & 4) ? 0.5 : -0.5;
+  double *out = T + ((int64_t(kind) * 8 + o) * PP + col) * PP;
+  if (kind == 0) {
+    // child scaled unit vector -> unscaled child M (radius 1/2), shift by Xp - Xc
+    const double v = ldexp(1.0, -n);
+    in[nms(n, m)] = im ? make_double2(0, v) : make_double2(v, 0);
+    regular_all<double>(P, -ox, -oy, -oz, Y);
+    for (int jk = 0; jk < NT; jk++) {
+      int j, k;
+      jk_of(jk, j, k);
+      const double2 r = m2m_coef(j, k, in, Y);  // parent radius 1: scaled == unscaled
+      out[rf(j, k, 0)] = r.x;
+      if (k > 0) out[rf(j, k, 1)] = r.y;
+    }
+  } else {
+    // parent scaled unit vector (radius 1: L = L~) -> child L, scaled by (1/2)^(j+1)
+    in[nms(n, m)] = im ? make_double2(0, 1) : make_double2(1, 0);
+    regular_all<double>(P, ox, oy, oz, Y);
+    for (int jk = 0; jk < NT; jk++) {
+      int j, k;
+      jk_of(jk, j, k);
+      const double2 r = l2l_coef(P, j, k, in, Y);
+      const double s = ldexp(1.0, -(j + 1));
+      out[rf(j, k, 0)] = r.x * s;
+      if (k > 0) out[rf(j, k, 1)] = r.y * s;
+    }
   }
-  __syncthreads();
+}
+
+// Rows per workgroup and staged columns of the M2M / L2L GEMMs.
+constexpr int kShiftRows = 16;
+constexpr int kShiftCols = 16;
+
+void shift_ops(Ctx &c) {
+  if (c.shift_P == c.P) return;
+  const int PP = c.P * c.P;
+  c.shift_T.ensure(size_t(2) * 8 * PP * PP);
+  hipLaunchKernelGGL(k_shift_ops, dim3(ceil_div(2 * 8 * PP, 64)), dim3(64), 0, c.s0, c.P, c.shift_T.p);
+  XF_LAUNCH();
+  c.shift_P = c.P;
 }
 
 // ---------------------------------------------------------------- M2M
-// One workgroup per parent at level l; thread = coefficient (j,k).  The
-// sum over children in child order and the (n,m) loops are the
-// reference's (3d/kernel.h:204-231) with dX = Xparent - Xchild.
-__global__ void __launch_bounds__(64) k_m2m(int P, int NT, int lstart, int lcount, const int *__restrict__ cchild,
-                                            const int *__restrict__ cnch, const double4 *__restrict__ cx,
-                                            double2 *__restrict__ M) {
-  extern __shared__ cpx<double> Ytab[];  // [8][NT]
-  const int pidx = lstart + blockIdx.x;
-  if (blockIdx.x >= lcount) return;
-  const int nch = cnch[pidx];
-  if (nch == 0) return;
-  const double4 Cp = cx[pidx];
-  const int first = cchild[pidx];
-  octant_tables(P, NT, cx[first].w, -1.0, Ytab);
-  for (int jk = threadIdx.x; jk < NT; jk += 64) {
-    int j, k;
-    jk_of(jk, j, k);
-    double accr = 0, acci = 0;
+// M2M and L2L as per-octant GEMMs over the cells of one level.  A workgroup
+// is 4 waves over the same 64 parents (lane = parent) and one block of
+// kShiftRows operator rows (blockIdx.y); wave w takes octants w and w+4.
+// Each wave walks its (octant, column chunk) steps on its own: the values
+// of the next step are loaded into registers while the current one, staged
+// in the wave's LDS slice (expansions X[col][lane], operator Tw[col][row]
+// read back as broadcasts), is multiplied, so every step exposes at most
+// one memory latency.  Degree structure: an M2M row of degree j reads
+// columns of degree <= j, an L2L row of degree j reads columns of degree
+// >= j, so a row block stops (starts) at the column bound of its last
+// (first) row.
+constexpr int kShiftWaves = 4;
+constexpr int kShiftPad = kWave + 1;
+constexpr int kXPer = kShiftCols * kWave / kWave;           // staged expansion values per lane
+constexpr int kTPer = kShiftCols * kShiftRows / kWave;      // staged operator values per lane
+
+__device__ inline int degree_of(int row) {
+  int j = 0;
+  while ((j + 1) * (j + 1) <= row) j++;
+  return j;
+}
+
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// chidx[o][lane] = child of the lane's parent in octant o, or -1; the
+// octant is recomputed from the centres (strict >, 3d/build_tree.h:50-56).
+__device__ inline void child_slots(int p, bool valid, const int *__restrict__ cchild, const int *__restrict__ cnch,
+                                   const double4 *__restrict__ cx, int (*chidx)[kWave], int lane, int w) {
+  if (w != 0) return;
+  int slot[8];
+#pragma unroll
+  for (int o = 0; o < 8; o++) slot[o] = -1;
+  if (valid) {
+    const int nch = cnch[p], first = cchild[p];
+    const double4 Cp = cx[p];
     for (int ch = first; ch < first + nch; ch++) {
       const double4 Cc = cx[ch];
       const int o = (Cc.x > Cp.x) + ((Cc.y > Cp.y) << 1) + ((Cc.z > Cp.z) << 2);
-      const double2 v = m2m_coef(j, k, M + int64_t(ch) * NT, Ytab + o * NT);
-      accr += v.x;
-      acci += v.y;
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (q == o) slot[q] = ch;
     }
-    M[int64_t(pidx) * NT + jk] = make_double2(accr, acci);
+  }
+#pragma unroll
+  for (int o = 0; o < 8; o++) chidx[o][lane] = slot[o];
+}
+
+// Register stage of one step: expansion values (cell src[e / kShiftCols],
+// column c0 + e % kShiftCols for e = i * 64 + lane) and operator values
+// To[c0 + e / kShiftRows][r0 + e % kShiftRows].
+struct ShiftRegs {
+  double x[kXPer];
+  double t[kTPer];
+};
+
+__device__ inline void load_step(ShiftRegs &g, const double *__restrict__ E, int PP, const int *src, int c0,
+                                 int ncol, const double *__restrict__ To, int r0, int lane) {
+#pragma unroll
+  for (int i = 0; i < kXPer; i++) {
+    const int e = i * kWave + lane;
+    const int cell = src[e / kShiftCols];
+    const int col = c0 + e % kShiftCols;
+    g.x[i] = (cell >= 0 && col < ncol) ? E[int64_t(cell) * PP + col] : 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < kTPer; i++) {
+    const int e = i * kWave + lane;
+    const int col = c0 + e / kShiftRows, r = r0 + e % kShiftRows;
+    g.t[i] = (col < ncol && r < PP) ? To[int64_t(col) * PP + r] : 0.0;
   }
 }
 
-// ---------------------------------------------------------------- L2L
-// One workgroup per parent at level l; every child gets
-// L_child[j,k] += sum over n>=j of the parent's terms (3d/kernel.h:262-291)
-// with dX = Xchild - Xparent.
-__global__ void __launch_bounds__(64) k_l2l(int P, int NT, int lstart, int lcount, const int *__restrict__ cchild,
-                                            const int *__restrict__ cnch, const double4 *__restrict__ cx,
-                                            double2 *__restrict__ L) {
-  extern __shared__ cpx<double> Ytab[];
-  const int pidx = lstart + blockIdx.x;
-  if (blockIdx.x >= lcount) return;
-  const int nch = cnch[pidx];
-  if (nch == 0) return;
-  const double4 Cp = cx[pidx];
-  const int first = cchild[pidx];
-  octant_tables(P, NT, cx[first].w, 1.0, Ytab);
-  const double2 *Lp = L + int64_t(pidx) * NT;
-  for (int e = threadIdx.x; e < nch * NT; e += 64) {
-    const int ch = first + e / NT;
-    const int jk = e % NT;
-    int j, k;
-    jk_of(jk, j, k);
-    const double4 Cc = cx[ch];
-    const int o = (Cc.x > Cp.x) + ((Cc.y > Cp.y) << 1) + ((Cc.z > Cp.z) << 2);
-    const double2 v = l2l_coef(P, j, k, Lp, Ytab + o * NT);
-    const double accr = v.x, acci = v.y;
-    double2 *Lc = L + int64_t(ch) * NT + jk;
-    *Lc = make_double2(Lc->x + accr, Lc->y + acci);
+__device__ inline void store_step(const ShiftRegs &g, double (*X)[kShiftPad], double (*Tw)[kShiftRows], int lane) {
+#pragma unroll
+  for (int i = 0; i < kXPer; i++) {
+    const int e = i * kWave + lane;
+    X[e % kShiftCols][e / kShiftCols] = g.x[i];
+  }
+#pragma unroll
+  for (int i = 0; i < kTPer; i++) {
+    const int e = i * kWave + lane;
+    Tw[e / kShiftRows][e % kShiftRows] = g.t[i];
+  }
+}
+
+__device__ inline void shift_fma(const double (*X)[kShiftPad], const double (*Tw)[kShiftRows], int lane,
+                                 double *acc) {
+#pragma unroll 4
+  for (int cc = 0; cc < kShiftCols; cc++) {
+    const double x = X[cc][lane];
+#pragma unroll
+    for (int r = 0; r < kShiftRows; r++) acc[r] = fma(Tw[cc][r], x, acc[r]);
+  }
+}
+
+// M~_parent = sum over children of T_M2M[octant] M~_child (3d/kernel.h:206).
+__global__ void __launch_bounds__(kShiftWaves *kWave) k_m2m(int P, int lstart, int lcount,
+                                                            const int *__restrict__ cchild,
+                                                            const int *__restrict__ cnch,
+                                                            const double4 *__restrict__ cx,
+                                                            const double *__restrict__ T, double *__restrict__ M) {
+  __shared__ int chidx[8][kWave];
+  __shared__ double X[kShiftWaves][kShiftRows > kShiftCols ? kShiftRows : kShiftCols][kShiftPad];
+  __shared__ double Tw[kShiftWaves][kShiftCols][kShiftRows];
+  const int PP = P * P;
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int pl = blockIdx.x * kWave + lane;
+  const bool valid = pl < lcount;
+  const int p = lstart + pl;
+  child_slots(p, valid, cchild, cnch, cx, chidx, lane, w);
+  const int r0 = blockIdx.y * kShiftRows;
+  const int jmax = degree_of(min(r0 + kShiftRows, PP) - 1);
+  const int ncol = (jmax + 1) * (jmax + 1);
+  const int nchunk = (ncol + kShiftCols - 1) / kShiftCols;
+  const int nstep = 2 * nchunk;
+  double acc[kShiftRows];
+#pragma unroll
+  for (int r = 0; r < kShiftRows; r++) acc[r] = 0.0;
+  __syncthreads();
+  ShiftRegs g;
+  load_step(g, M, PP, chidx[w], 0, ncol, T + int64_t(w) * PP * PP, r0, lane);
+  for (int st = 0; st < nstep; st++) {
+    store_step(g, X[w], Tw[w], lane);
+    wave_lds_sync();
+    if (st + 1 < nstep) {
+      const int o = w + 4 * ((st + 1) / nchunk), c0 = ((st + 1) % nchunk) * kShiftCols;
+      load_step(g, M, PP, chidx[o], c0, ncol, T + int64_t(o) * PP * PP, r0, lane);
+    }
+    shift_fma(X[w], Tw[w], lane, acc);
+    wave_lds_sync();
+  }
+  // sum the four waves' partial rows
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kShiftRows; r++) X[w][r][lane] = acc[r];
+  __syncthreads();
+  if (!valid || cnch[p] == 0) return;
+  for (int r = w; r < kShiftRows; r += kShiftWaves) {
+    if (r0 + r >= PP) break;
+    const double v = ((X[0][r][lane] + X[1][r][lane]) + X[2][r][lane]) + X[3][r][lane];
+    M[int64_t(p) * PP + r0 + r] = v;
+  }
+}
+
+// L~_child += T_L2L[octant] L~_parent for the children of one level of
+// parents (3d/kernel.h:265-280).  Wave w produces the children in octants
+// w and w+4 from the same staged parent columns.
+__global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l(int P, int lstart, int lcount,
+                                                            const int *__restrict__ cchild,
+                                                            const int *__restrict__ cnch,
+                                                            const double4 *__restrict__ cx,
+                                                            const double *__restrict__ T, double *__restrict__ L) {
+  __shared__ int chidx[8][kWave];
+  __shared__ int par[kWave];
+  __shared__ double X[kShiftWaves][kShiftCols][kShiftPad];
+  __shared__ double Tw[kShiftWaves][2][kShiftCols][kShiftRows];
+  const int PP = P * P;
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int pl = blockIdx.x * kWave + lane;
+  const bool valid = pl < lcount;
+  const int p = lstart + pl;
+  child_slots(p, valid, cchild, cnch, cx, chidx, lane, w);
+  if (w == 0) par[lane] = (valid && cnch[p] > 0) ? p : -1;
+  const int r0 = blockIdx.y * kShiftRows;
+  const int cfirst = degree_of(r0) * degree_of(r0);
+  const int nstep = (PP - cfirst + kShiftCols - 1) / kShiftCols;
+  const double *T0 = T + (int64_t(8) + w) * PP * PP, *T1 = T + (int64_t(12) + w) * PP * PP;
+  double acc0[kShiftRows], acc1[kShiftRows];
+#pragma unroll
+  for (int r = 0; r < kShiftRows; r++) acc0[r] = acc1[r] = 0.0;
+  __syncthreads();
+  ShiftRegs g0, g1;
+  load_step(g0, L, PP, par, cfirst, PP, T0, r0, lane);
+#pragma unroll
+  for (int i = 0; i < kTPer; i++) {
+    const int e = i * kWave + lane;
+    const int col = cfirst + e / kShiftRows, r = r0 + e % kShiftRows;
+    g1.t[i] = (col < PP && r < PP) ? T1[int64_t(col) * PP + r] : 0.0;
+  }
+  for (int st = 0; st < nstep; st++) {
+    store_step(g0, X[w], Tw[w][0], lane);
+#pragma unroll
+    for (int i = 0; i < kTPer; i++) {
+      const int e = i * kWave + lane;
+      Tw[w][1][e / kShiftRows][e % kShiftRows] = g1.t[i];
+    }
+    wave_lds_sync();
+    if (st + 1 < nstep) {
+      const int c0 = cfirst + (st + 1) * kShiftCols;
+      load_step(g0, L, PP, par, c0, PP, T0, r0, lane);
+#pragma unroll
+      for (int i = 0; i < kTPer; i++) {
+        const int e = i * kWave + lane;
+        const int col = c0 + e / kShiftRows, r = r0 + e % kShiftRows;
+        g1.t[i] = (col < PP && r < PP) ? T1[int64_t(col) * PP + r] : 0.0;
+      }
+    }
+    shift_fma(X[w], Tw[w][0], lane, acc0);
+    shift_fma(X[w], Tw[w][1], lane, acc1);
+    wave_lds_sync();
+  }
+  const int ch0 = chidx[w][lane], ch1 = chidx[w + 4][lane];
+  if (ch0 >= 0) {
+#pragma unroll
+    for (int r = 0; r < kShiftRows; r++)
+      if (r0 + r < PP) L[int64_t(ch0) * PP + r0 + r] += acc0[r];
+  }
+  if (ch1 >= 0) {
+#pragma unroll
+    for (int r = 0; r < kShiftRows; r++)
+      if (r0 + r < PP) L[int64_t(ch1) * PP + r0 + r] += acc1[r];
   }
 }
 
 // ---------------------------------------------------------------- L2P
 // One workgroup per leaf, one thread per body.  With u = (x - Xc)/R and
-// Lh_n^m = L_n^m R^n, phi = Re sum_{n, m>=0} w_m Lh_n^m Y_n^m(u)
+// Lh_n^m = L_n^m R^n = L~/R, phi = Re sum_{n, m>=0} w_m Lh_n^m Y_n^m(u)
 // (w_0 = 1, w_m = 2, 3d/kernel.h:302-316).  The gradient uses
 //   dz Y_n^m = -Y_{n-1}^m,  (dx + i dy) Y_n^m = -Y_{n-1}^{m+1},
 //   (dx - i dy) Y_n^m = Y_{n-1}^{m-1} (m >= 1), = -conj(Y_{n-1}^1) (m = 0),
 // collected per harmonic Y_n^m so each column m is generated once.
-template <int MAXP>
-__global__ void __launch_bounds__(64) k_l2p(int P, int NT, const int *__restrict__ leaves,
-                                            const int *__restrict__ nleaves, const int *__restrict__ cbeg,
-                                            const int *__restrict__ ccnt, const double4 *__restrict__ cx,
-                                            const double2 *__restrict__ L, const double4 *__restrict__ xs,
-                                            exafmm_body *__restrict__ bodies) {
-  __shared__ float2 Lh[MAXP * (MAXP + 1) / 2];
+template <int MAXP, class T>
+__global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leaves, const int *__restrict__ nleaves,
+                                            const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                            const double4 *__restrict__ cx, const double *__restrict__ L,
+                                            const double4 *__restrict__ xs, exafmm_body *__restrict__ bodies) {
+  __shared__ cpx<T> Lh[MAXP * (MAXP + 1) / 2];
   const int li = blockIdx.x;
   if (li >= *nleaves) return;
   const int c = leaves[li];
   const double4 C = cx[c];
+  const double invR = 1.0 / C.w;
+  const int NT = P * (P + 1) / 2;
+  const double *Lc = L + int64_t(c) * P * P;
   for (int e = threadIdx.x; e < NT; e += 64) {
-    int n = 0;
-    while (nms(n + 1, 0) <= e) n++;
-    const double s = pow(C.w, double(n)) * (e == nms(n, 0) ? 1.0 : 2.0);  // fold w_m
-    const double2 v = L[int64_t(c) * NT + e];
-    Lh[e] = make_float2(float(v.x * s), float(v.y * s));
+    int n, m;
+    jk_of(e, n, m);
+    const double w = (m == 0 ? 1.0 : 2.0) * invR;  // fold w_m and L~ -> L R^n
+    Lh[e] = {T(Lc[rf(n, m, 0)] * w), T(m ? Lc[rf(n, m, 1)] * w : 0.0)};
   }
   __syncthreads();
   const int beg = cbeg[c], cnt = ccnt[c];
-  const double invR = 1.0 / C.w;
   for (int b = beg + threadIdx.x; b < beg + cnt; b += 64) {
     const double4 B = xs[b];
-    const float x = float((B.x - C.x) * invR), y = float((B.y - C.y) * invR), z = float((B.z - C.z) * invR);
-    const float r2 = x * x + y * y + z * z;
-    float pot = 0, Ar = 0, Ai = 0, Br = 0, Bi = 0, Dz = 0;
-    float ymr = 1, ymi = 0;  // Y_m^m
+    const T x = T((B.x - C.x) * invR), y = T((B.y - C.y) * invR), z = T((B.z - C.z) * invR);
+    const T r2 = x * x + y * y + z * z;
+    T pot = 0, Ar = 0, Ai = 0, Br = 0, Bi = 0, Dz = 0;
+    T ymr = 1, ymi = 0;  // Y_m^m
     for (int m = 0; m < P; m++) {
       if (m > 0) {
-        const float inv = 1.0f / float(2 * m);
-        const float tr = (ymr * x - ymi * y) * inv, ti = (ymr * y + ymi * x) * inv;
+        const T inv = T(1) / T(2 * m);
+        const T tr = (ymr * x - ymi * y) * inv, ti = (ymr * y + ymi * x) * inv;
         ymr = tr;
         ymi = ti;
       }
-      float pr = 0, pi = 0, cr = ymr, ci = ymi;
+      T pr = 0, pi = 0, cr = ymr, ci = ymi;
       for (int n = m; n < P; n++) {
         if (n > m) {
-          float nr, ni;
+          T nr, ni;
           if (n == m + 1) {
             nr = -z * cr;
             ni = -z * ci;
           } else {
-            const float s = -1.0f / float((n + m) * (n - m));
-            const float cz = float(2 * n - 1) * z;
+            const T s = T(-1) / T((n + m) * (n - m));
+            const T cz = T(2 * n - 1) * z;
             nr = (cz * cr + r2 * pr) * s;
             ni = (cz * ci + r2 * pi) * s;
           }
@@ -199,57 +449,50 @@ __global__ void __launch_bounds__(64) k_l2p(int P, int NT, const int *__restrict
           cr = nr;
           ci = ni;
         }
-        // potential: Re(Lh_n^m Y_n^m)  (weights folded into Lh)
-        const float2 l0 = Lh[nms(n, m)];
-        pot += l0.x * cr - l0.y * ci;
+        const cpx<T> l0 = Lh[nms(n, m)];
+        pot += l0.re * cr - l0.im * ci;
         if (n + 1 < P) {
-          // dz:  -Lh_{n+1}^m Y_n^m
-          const float2 lz = Lh[nms(n + 1, m)];
-          Dz -= lz.x * cr - lz.y * ci;
-          // (dx+i dy): -Lh_{n+1}^{m-1} Y_n^m  (m >= 1)
+          const cpx<T> lz = Lh[nms(n + 1, m)];
+          Dz -= lz.re * cr - lz.im * ci;
           if (m >= 1) {
-            const float2 la = Lh[nms(n + 1, m - 1)];
-            Ar -= la.x * cr - la.y * ci;
-            Ai -= la.x * ci + la.y * cr;
+            const cpx<T> la = Lh[nms(n + 1, m - 1)];
+            Ar -= la.re * cr - la.im * ci;
+            Ai -= la.re * ci + la.im * cr;
           }
-          // (dx-i dy): Lh_{n+1}^{m+1} Y_n^m
-          const float2 lb = Lh[nms(n + 1, m + 1)];
-          Br += lb.x * cr - lb.y * ci;
-          Bi += lb.x * ci + lb.y * cr;
-          // m = 1 also feeds (dx-i dy) of the m = 0 terms: -Lh_{n+1}^0 conj(Y_n^1)
-          if (m == 1) {
-            const float2 l00 = Lh[nms(n + 1, 0)];
-            Br -= l00.x * cr + l00.y * ci;
-            Bi -= l00.y * cr - l00.x * ci;
+          const cpx<T> lb = Lh[nms(n + 1, m + 1)];
+          Br += lb.re * cr - lb.im * ci;
+          Bi += lb.re * ci + lb.im * cr;
+          if (m == 1) {  // the m = 0 terms' (dx - i dy): -Lh_{n+1}^0 conj(Y_n^1)
+            const cpx<T> l00 = Lh[nms(n + 1, 0)];
+            Br -= l00.re * cr + l00.im * ci;
+            Bi -= l00.im * cr - l00.re * ci;
           }
         }
       }
     }
-    // weights w_m were folded into Lh; the m = 0 terms of Lh carry w = 1
-    const double Fx = 0.5 * double(Ar + Br) * invR;
-    const double Fy = 0.5 * double(Ai - Bi) * invR;
-    const double Fz = double(Dz) * invR;
     exafmm_body *o = bodies + b;
     o->p += double(pot);
-    o->F[0] += Fx;
-    o->F[1] += Fy;
-    o->F[2] += Fz;
+    o->F[0] += 0.5 * double(Ar + Br) * invR;
+    o->F[1] += 0.5 * double(Ai - Bi) * invR;
+    o->F[2] += double(Dz) * invR;
   }
 }
 
 void upward(Ctx &c) {
   Tree &t = c.t;
   hipStream_t s = c.s0;
-  const int P = c.P, NT = c.NT;
-  c.M.ensure(size_t(t.ncells) * NT);
+  const int P = c.P, PP = P * P;
+  shift_ops(c);
+  c.M.ensure(size_t(t.ncells) * PP);
   const int64_t nl = t.nleaves;
-  hipLaunchKernelGGL(k_p2m, dim3(ceil_div(nl * P, 256)), dim3(256), 0, s, P, t.leaves.p, t.misc.p + 17, t.cbeg.p,
-                     t.ccnt.p, t.cx.p, t.xs.p, c.M.p, NT);
+  hipLaunchKernelGGL((k_p2m<kMaxP>), dim3(ceil_div(nl, kP2MWaves)), dim3(kP2MWaves * kWave), 0, s, P, t.leaves.p,
+                     t.misc.p + 17,
+                     t.cbeg.p, t.ccnt.p, t.cx.p, t.xs.p, c.M.p);
   XF_LAUNCH();
   for (int l = t.depth - 1; l >= 0; l--) {
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_m2m, dim3(cnt), dim3(64), sizeof(cpx<double>) * 8 * NT, s, P, NT, t.lstart[l], cnt,
-                       t.cchild.p, t.cnch.p, t.cx.p, c.M.p);
+    hipLaunchKernelGGL(k_m2m, dim3(ceil_div(cnt, kWave), ceil_div(PP, kShiftRows)), dim3(kShiftWaves * kWave), 0, s, c.P,
+                       t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
     XF_LAUNCH();
   }
   c.have_M = true;
@@ -258,11 +501,12 @@ void upward(Ctx &c) {
 void l2l_pass(Ctx &c) {
   Tree &t = c.t;
   hipStream_t s = c.s0;
-  const int P = c.P, NT = c.NT;
-  for (int l = 0; l < t.depth; l++) {
+  const int PP = c.P * c.P;
+  shift_ops(c);
+  for (int l = 0; l < t.depth; l++) {  // parents at level l
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_l2l, dim3(cnt), dim3(64), sizeof(cpx<double>) * 8 * NT, s, P, NT, t.lstart[l], cnt,
-                       t.cchild.p, t.cnch.p, t.cx.p, c.L.p);
+    hipLaunchKernelGGL(k_l2l, dim3(ceil_div(cnt, kWave), ceil_div(PP, kShiftRows)), dim3(kShiftWaves * kWave), 0, s, c.P,
+                       t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.L.p);
     XF_LAUNCH();
   }
 }
@@ -270,9 +514,14 @@ void l2l_pass(Ctx &c) {
 void l2p_pass(Ctx &c) {
   Tree &t = c.t;
   hipStream_t s = c.s0;
-  const int P = c.P, NT = c.NT;
-  hipLaunchKernelGGL(k_l2p<kMaxP>, dim3(t.nleaves), dim3(64), 0, s, P, NT, t.leaves.p, t.misc.p + 17, t.cbeg.p,
-                     t.ccnt.p, t.cx.p, c.L.p, t.xs.p, c.bodies);
+  const int P = c.P;
+  // FP64 evaluation in parity mode, FP32 otherwise
+  if (c.cfg.m2l_precision == EXAFMM_PREC_FP64)
+    hipLaunchKernelGGL((k_l2p<kMaxP, double>), dim3(t.nleaves), dim3(64), 0, s, P, t.leaves.p, t.misc.p + 17,
+                       t.cbeg.p, t.ccnt.p, t.cx.p, c.L.p, t.xs.p, c.bodies);
+  else
+    hipLaunchKernelGGL((k_l2p<kMaxP, float>), dim3(t.nleaves), dim3(64), 0, s, P, t.leaves.p, t.misc.p + 17,
+                       t.cbeg.p, t.ccnt.p, t.cx.p, c.L.p, t.xs.p, c.bodies);
   XF_LAUNCH();
 }
 

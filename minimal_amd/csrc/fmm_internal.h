@@ -135,17 +135,29 @@ struct Ctx {
   Tree t;
   Lists lists;
   const Ctx *src = nullptr;        // source tree of the last horizontal pass
-  Buf<double2> M, L;               // FP64 complex [ncells][NT]
+  // expansions, radius-scaled real form [ncells][P^2]:
+  //   M~ = (Re, Im) M_n^m / R^n,  L~ = (Re, Im) L_j^k R^(j+1)
+  Buf<double> M, L;
   bool have_M = false, have_L = false;
+  Buf<double> shift_T;             // scaled M2M / L2L operators per octant
+  int shift_P = -1;
 
   // M2L fast path
   Buf<float> ops32;                // [nkeys][P^2][P^2] scaled real-form operators
+  // What ops32 holds: operators are pure functions of (key, P, truncation,
+  // precision) plus R0 and the cycle for periodic keys, so an unchanged key
+  // set is not generated again.
+  std::vector<unsigned long long> ops_keys;
+  int ops_P = -1, ops_fill = -1, ops_bf16 = -1;
+  double ops_R0 = 0, ops_cycle = 0;
   Buf<float> mt32;                 // scaled multipoles, real form [ncells][P^2]
   Buf<float> lt32;                 // scaled local accumulators [ncells][P^2]
   Buf<int> tiles;                  // M2L tile descriptors
   Buf<int> tile_mat;               // per tile [U][kTileN] source indices
   Buf<int> tile_keys;
   Buf<int> misc;
+  // periodic far images
+  Buf<double2> far_sup, far_part;
 
   float stage_ms[8] = {};
 };
@@ -158,9 +170,14 @@ void traverse(Ctx &tgt, const Ctx &src);
 void upward(Ctx &c);
 void m2l_apply(Ctx &tgt, const Ctx &src);
 void p2p_apply(Ctx &tgt, const Ctx &src, hipStream_t s);
+void shift_ops(Ctx &c);
 void l2l_pass(Ctx &c);
 void l2p_pass(Ctx &c);
 void direct_sum(int device, hipStream_t s, exafmm_body *t, int64_t nt, const exafmm_body *src, int64_t ns);
+void periodic_far(Ctx &c);
+void dipole_correction(Ctx &c);
+void ewald_sum(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns, int ksize, double alpha,
+               double sigma, double cutoff, double cycle);
 
 // single-operator entry points (FP64, one call == one reference call)
 void op_p2p(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns);

@@ -19,11 +19,13 @@
 namespace xf {
 using namespace xfd;
 
+// Expansions are stored radius-scaled in real form; the pair sums run on the
+// unscaled complex coefficients (M = M~ Rs^n in, L~ += L Rc^(j+1) out).
 __global__ void __launch_bounds__(64) k_m2l_fp64(int P, int NT, int fillP, int ncells, const double4 *__restrict__ cx,
                                                  const double4 *__restrict__ scx, const int *__restrict__ beg,
                                                  const int *__restrict__ cnt, const int2 *__restrict__ list,
-                                                 const double2 *__restrict__ M, double2 *__restrict__ L,
-                                                 const double *__restrict__ shifts) {
+                                                 const double *__restrict__ M, double *__restrict__ L,
+                                                 double cycle) {
   extern __shared__ double2 sm[];
   const int c = blockIdx.x;
   if (c >= ncells) return;
@@ -38,16 +40,21 @@ __global__ void __launch_bounds__(64) k_m2l_fp64(int P, int NT, int fillP, int n
   double accr[3] = {0, 0, 0}, acci[3] = {0, 0, 0};
   for (int p = 0; p < n_pairs; p++) {
     const int2 e = list[b0 + p];
-    const double4 Xs = scx[e.x];
-    double dx = Xc.x - Xs.x, dy = Xc.y - Xs.y, dz = Xc.z - Xs.z;
-    if (shifts) {
-      dx -= shifts[3 * e.y];
-      dy -= shifts[3 * e.y + 1];
-      dz -= shifts[3 * e.y + 2];
-    }
+    const int src = e.x >> 5, img = e.x & 31;
+    const double4 Xs = scx[src];
+    // 3dp/kernel.h:200: dX = Xi - Xj - iX * cycle
+    const double dx = Xc.x - Xs.x - ((img % 3) - 1) * cycle;
+    const double dy = Xc.y - Xs.y - (((img / 3) % 3) - 1) * cycle;
+    const double dz = Xc.z - Xs.z - ((img / 9) - 1) * cycle;
     __syncthreads();
     if (lane < fillP) singular_column(lane, fillP, dx, dy, dz, I);
-    for (int i = lane; i < NT; i += 64) Ms[i] = M[int64_t(e.x) * NT + i];
+    for (int i = lane; i < NT; i += 64) {
+      int n, m;
+      jk_of(i, n, m);
+      const double *Mc = M + int64_t(src) * P * P;
+      const double s = pow(Xs.w, double(n));
+      Ms[i] = make_double2(Mc[rf(n, m, 0)] * s, m ? Mc[rf(n, m, 1)] * s : 0.0);
+    }
     __syncthreads();
     for (int r = 0; r * 64 + lane < NT; r++) {
       int j, k;
@@ -58,9 +65,12 @@ __global__ void __launch_bounds__(64) k_m2l_fp64(int P, int NT, int fillP, int n
     }
   }
   for (int r = 0; r * 64 + lane < NT; r++) {
-    const int jk = r * 64 + lane;
-    double2 *o = L + int64_t(c) * NT + jk;
-    *o = make_double2(o->x + accr[r], o->y + acci[r]);
+    int j, k;
+    jk_of(r * 64 + lane, j, k);
+    const double s = pow(Xc.w, double(j + 1));
+    double *o = L + int64_t(c) * P * P;
+    o[rf(j, k, 0)] += accr[r] * s;
+    if (k > 0) o[rf(j, k, 1)] += acci[r] * s;
   }
 }
 
@@ -70,7 +80,6 @@ void m2l_apply(Ctx &tc, const Ctx &sc) {
   const int P = tc.P, NT = tc.NT;
   const int fillP = tc.cfg.m2l_truncation == EXAFMM_M2L_FULL ? 2 * P - 1 : P;
   Tree &t = tc.t;
-  tc.L.ensure(size_t(t.ncells) * NT);
   if (tc.cfg.m2l_precision != EXAFMM_PREC_FP64 && tc.lists.nkeys > 0) {
     m2l_fast(tc, sc);
     return;
@@ -78,7 +87,7 @@ void m2l_apply(Ctx &tc, const Ctx &sc) {
   const size_t lds = sizeof(double2) * (fillP * (fillP + 1) / 2 + NT);
   hipLaunchKernelGGL(k_m2l_fp64, dim3(t.ncells), dim3(64), lds, tc.s0, P, NT, fillP, int(t.ncells), t.cx.p,
                      sc.t.cx.p, tc.lists.m2l_beg.p, tc.lists.m2l_cnt.p, tc.lists.m2l.p, sc.M.p, tc.L.p,
-                     (const double *)nullptr);
+                     tc.cfg.images > 0 ? tc.cfg.cycle : 0.0);
   XF_LAUNCH();
 }
 

@@ -1,6 +1,634 @@
-// m2l_mfma.hip -- M2L grouped by translation key on MFMA (in progress).
+// m2l_mfma.hip -- M2L grouped by translation key, applied on MFMA.
+//
+// Reference: M2L 3d/kernel.h:233-260 (singular harmonics 3d/kernel.h:83-116).
+//
+// Radius scaling makes the translation operator of a pair depend only on
+// its key (integer offset d in units of the finer radius, and the level
+// difference): with M~_n^m = M_n^m / Rs^n and L~_j^k = L_j^k Rc^{j+1},
+//   L~ = T(d) M~,  T = reference coefficients x I_{j+n}(d) x 2^{a(j+1) + b n},
+// where Rc = Rf 2^a, Rs = Rf 2^b.  In real form (P^2 real degrees of freedom
+// per expansion: Re M_n^0, Re/Im M_n^m for m > 0) T is a dense P^2 x P^2
+// matrix, generated once per distinct key in FP64.
+//
+// Targets are grouped in tiles of 64 cells of one level and one octant
+// position (cells with the same position relative to their parent have the
+// same key set away from the domain boundary).  For each key in a tile's
+// key union the 64 source columns are gathered and multiplied by the key's
+// operator; all keys of a tile accumulate into the same MFMA accumulators,
+// so each target's L is written once, without atomics, in a fixed order.
+//
+// Precision: FP32 uses v_mfma_f32_16x16x4_f32.  BF16X3 splits operator and
+// multipoles into bf16 hi + lo parts and forms hi*hi + hi*lo + lo*hi with
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulate), ~16 significant bits per
+// product.
+#include <algorithm>
+
 #include "fmm_internal.h"
+#include "fmm_ops.h"
 
 namespace xf {
-void m2l_fast(Ctx &, const Ctx &) { fail(EXAFMM_ERR_UNSUPPORTED, "MFMA M2L path not built yet"); }
+using namespace xfd;
+
+constexpr int kTN = 256;  // targets per tile: 4 column groups x 4 column tiles of 16
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+struct Geom {
+  int P, PP, R;  // R = row tiles of 16 (rows padded to 16R)
+  int K4;        // FP32 k-steps of 4
+  int K32;       // BF16 k-steps of 32
+};
+static Geom geom(int P) {
+  Geom g;
+  g.P = P;
+  g.PP = P * P;
+  g.R = (g.PP + 15) / 16;
+  g.K4 = (g.PP + 3) / 4;
+  g.K32 = (g.PP + 31) / 32;
+  return g;
+}
+
+// ---------------------------------------------------------------- tiles
+__global__ void __launch_bounds__(256) k_group_keys(int ncells, const double4 *__restrict__ cx,
+                                                    const int *__restrict__ cparent, const int *__restrict__ clevel,
+                                                    const int *__restrict__ m2l_cnt,
+                                                    unsigned long long *__restrict__ key, int *__restrict__ val) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  if (c >= ncells) return;
+  unsigned g = 255;
+  if (m2l_cnt[c] > 0) {
+    int o = 0;
+    const int p = cparent[c];
+    if (p >= 0) {
+      const double4 a = cx[c], b = cx[p];
+      o = (a.x > b.x) + ((a.y > b.y) << 1) + ((a.z > b.z) << 2);
+    }
+    g = unsigned(min(clevel[c], 30)) * 8 + o;
+  }
+  key[c] = g;
+  val[c] = c;
+}
+
+__global__ void __launch_bounds__(256) k_group_hist(int ncells, const unsigned long long *__restrict__ key,
+                                                    int *__restrict__ cnt) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) atomicAdd(&h[int(key[c])], 1);
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
+}
+
+// Entries of a tile's targets flattened: offsets of each column's list in
+// LDS, entry e -> (column, index) by binary search, so all threads work on
+// the tile at once.
+__device__ inline int tile_offsets(const int *cells, int ncol, const int *cnt, int *off) {
+  // exclusive scan of the columns' list lengths (blockDim.x >= kTN)
+  const int t = threadIdx.x;
+  const int v = t < ncol ? cnt[cells[t]] : 0;
+  if (t < kTN) off[t + 1] = v;
+  if (t == 0) off[0] = 0;
+  __syncthreads();
+  for (int o = 1; o < kTN; o <<= 1) {
+    int a = 0;
+    if (t < kTN && t >= o) a = off[t + 1 - o];
+    __syncthreads();
+    if (t < kTN) off[t + 1] += a;
+    __syncthreads();
+  }
+  return off[ncol];
+}
+__device__ inline int col_of(const int *off, int ncol, int e) {
+  int lo = 0, hi = ncol;  // last t with off[t] <= e
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= e)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// Distinct keys of one tile: bitmap over key ids in LDS.
+__device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, const int *off, int total,
+                                   const int2 *list, const int *slot2id, unsigned *bm, int nwords) {
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) bm[i] = 0;
+  __syncthreads();
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int t = col_of(off, ncol, e);
+    const int id = slot2id[list[beg[cells[t]] + e - off[t]].y];
+    atomicOr(&bm[id >> 5], 1u << (id & 31));
+  }
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
+                                                    const int *__restrict__ sorted, const int *__restrict__ beg,
+                                                    const int *__restrict__ cnt, const int2 *__restrict__ list,
+                                                    const int *__restrict__ slot2id, int nkeys,
+                                                    int *__restrict__ tU) {
+  extern __shared__ unsigned bm[];
+  __shared__ int off[kTN + 1];
+  __shared__ int tot;
+  const int nwords = (nkeys + 31) / 32;
+  const int *cells = sorted + tfirst[blockIdx.x];
+  const int ncol = tcount[blockIdx.x];
+  const int total = tile_offsets(cells, ncol, cnt, off);
+  tile_bitmap(cells, ncol, beg, off, total, list, slot2id, bm, nwords);
+  if (threadIdx.x == 0) tot = 0;
+  __syncthreads();
+  int s = 0;
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) s += __popc(bm[i]);
+  atomicAdd(&tot, s);
+  __syncthreads();
+  if (threadIdx.x == 0) tU[blockIdx.x] = tot;
+}
+
+// Key list (ascending) and source matrix [U][64] (-1 = no pair) of a tile.
+__global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirst, const int *__restrict__ tcount,
+                                                   const int *__restrict__ sorted, const int *__restrict__ beg,
+                                                   const int *__restrict__ cnt, const int2 *__restrict__ list,
+                                                   const int *__restrict__ slot2id, int nkeys,
+                                                   const int *__restrict__ tbase, int *__restrict__ tkeys,
+                                                   int *__restrict__ tmat) {
+  extern __shared__ unsigned bm[];
+  __shared__ int off[kTN + 1];
+  const int nwords = (nkeys + 31) / 32;
+  unsigned *rank = bm + nwords;  // exclusive popcount prefix per word
+  const int *cells = sorted + tfirst[blockIdx.x];
+  const int ncol = tcount[blockIdx.x];
+  const int total = tile_offsets(cells, ncol, cnt, off);
+  tile_bitmap(cells, ncol, beg, off, total, list, slot2id, bm, nwords);
+  if (threadIdx.x == 0) {
+    unsigned r = 0;
+    for (int i = 0; i < nwords; i++) {
+      rank[i] = r;
+      r += __popc(bm[i]);
+    }
+  }
+  __syncthreads();
+  const int base = tbase[blockIdx.x];
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) {
+    unsigned w = bm[i];
+    int r = rank[i];
+    while (w) {
+      const int b = __ffs(w) - 1;
+      w &= w - 1;
+      tkeys[base + r++] = i * 32 + b;
+    }
+  }
+  int U = 0;
+  if (nwords) U = rank[nwords - 1] + __popc(bm[nwords - 1]);
+  for (int i = threadIdx.x; i < U * kTN; i += blockDim.x) tmat[int64_t(base) * kTN + i] = -1;
+  __threadfence();
+  __syncthreads();
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    const int t = col_of(off, ncol, e);
+    const int2 en = list[beg[cells[t]] + e - off[t]];
+    const int id = slot2id[en.y];
+    const unsigned w = bm[id >> 5];
+    const int r = rank[id >> 5] + __popc(w & ((1u << (id & 31)) - 1u));
+    tmat[(int64_t(base) + r) * kTN + t] = en.x >> 5;  // source cell (the image is part of the key)
+  }
+}
+
+// ---------------------------------------------------------------- operands
+// GEMM operands from the scaled FP64 multipoles M~ (P^2 per cell, padded to KP):
+// FP32 copy, or bf16 hi + lo split (hi = bf16(x), lo = bf16(x - hi)).
+__global__ void __launch_bounds__(256) k_mt(int PP, int KP, int ncells, const double *__restrict__ M,
+                                            float *__restrict__ mt32, __bf16 *__restrict__ mhi,
+                                            __bf16 *__restrict__ mlo) {
+  const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int c = int(g / KP), r = int(g % KP);
+  if (c >= ncells) return;
+  const float v = r < PP ? float(M[int64_t(c) * PP + r]) : 0.0f;
+  if (mt32) mt32[g] = v;
+  if (mhi) {
+    const __bf16 h = __bf16(v);
+    mhi[g] = h;
+    mlo[g] = __bf16(v - float(h));
+  }
+}
+
+// Decodes a translation key (layout of m2l_key in traverse.hip) into the
+// offset d = dX / Rf in units of the finer radius and the radius ratios
+// Rc = Rf 2^a, Rs = Rf 2^b.  For a periodic image the shift iX * cycle is
+// subtracted in the same units (Rf = R0 / 2^level).
+__device__ inline void key_at(const unsigned long long *keys, int id, double R0, double cycle, double &dx,
+                              double &dy, double &dz, int &a, int &b) {
+  const unsigned long long k = keys[id];
+  dx = double((long long)((k >> 48) & 0xFFFF) - 32768);
+  dy = double((long long)((k >> 32) & 0xFFFF) - 32768);
+  dz = double((long long)((k >> 16) & 0xFFFF) - 32768);
+  const int img = int((k >> 11) & 31), lf = int((k >> 6) & 31);
+  const int ld = int(k & 63) - 32;  // level(target) - level(source)
+  a = ld < 0 ? -ld : 0;
+  b = ld > 0 ? ld : 0;
+  if (img != 13) {
+    const double rf = R0 / (1 << lf);
+    dx -= ((img % 3) - 1) * cycle / rf;
+    dy -= (((img / 3) % 3) - 1) * cycle / rf;
+    dz -= ((img / 9) - 1) * cycle / rf;
+  }
+}
+
+// Operator entry T[(j,k,pr)][(n,m,pc)] of one key (harmonics I in LDS).
+__device__ inline double op_entry(int P, int fillP, int row, int col, int a, int b, const cpx<double> *I) {
+  const int PP = P * P;
+  if (row >= PP || col >= PP) return 0.0;
+  int j = 0;
+  while ((j + 1) * (j + 1) <= row) j++;
+  int rem = row - j * j;
+  const int k = (rem + 1) / 2;
+  const int pr = (rem > 0 && rem % 2 == 0) ? 1 : 0;
+  int n = 0;
+  while ((n + 1) * (n + 1) <= col) n++;
+  rem = col - n * n;
+  const int m = (rem + 1) / 2;
+  const int pc = (rem > 0 && rem % 2 == 0) ? 1 : 0;
+  if (j + n >= fillP) return 0.0;
+  const double s = ldexp(1.0, a * (j + 1) + b * n);
+  const double cj = sgn(j);
+  const double cp = cj * sgn(((k - m) * (k < m)) + m);
+  // Gp = I_{j+n}^{m-k} (conjugated for negative order)
+  cpx<double> gp = I[nms(j + n, abs(m - k))];
+  if (m - k < 0) gp.im = -gp.im;
+  gp.re *= s;
+  gp.im *= s;
+  if (m == 0) {
+    if (pc == 1) return 0.0;
+    return pr == 0 ? cp * gp.re : cp * gp.im;
+  }
+  // Gm = conj(I_{j+n}^{m+k})
+  cpx<double> gm = I[nms(j + n, m + k)];
+  gm.im = -gm.im;
+  gm.re *= s;
+  gm.im *= s;
+  if (pr == 0 && pc == 0) return cp * gp.re + cj * gm.re;
+  if (pr == 0 && pc == 1) return -cp * gp.im + cj * gm.im;
+  if (pr == 1 && pc == 0) return cp * gp.im + cj * gm.im;
+  return cp * gp.re - cj * gm.re;
+}
+
+// One workgroup per key: singular harmonics of d in FP64, then the operator
+// in MFMA fragment order.
+//  FP32 (16x16x4):  op32[key][kk][r][lane], element (row, kcol):
+//                   r = row/16, kk = kcol/4, lane = (kcol%4)*16 + row%16
+//  BF16 (16x16x32): ophi/oplo[key][kk][r][lane][8], element (row, kcol):
+//                   r = row/16, kk = kcol/32, lane = ((kcol%32)/8)*16 + row%16
+__global__ void __launch_bounds__(256) k_ops_gen(int P, int fillP, int R, int K4, int K32, double R0, double cycle,
+                                                 const unsigned long long *__restrict__ keys, float *__restrict__ op32,
+                                                 __bf16 *__restrict__ ophi, __bf16 *__restrict__ oplo) {
+  __shared__ cpx<double> I[(2 * kMaxP) * (2 * kMaxP + 1) / 2];
+  const int id = blockIdx.x;
+  double dx, dy, dz;
+  int a, b;
+  key_at(keys, id, R0, cycle, dx, dy, dz, a, b);
+  if (threadIdx.x < fillP) singular_column(threadIdx.x, fillP, dx, dy, dz, I);
+  __syncthreads();
+  if (op32) {
+    const int64_t per = int64_t(K4) * R * 64;
+    for (int64_t e = threadIdx.x; e < per; e += blockDim.x) {
+      const int lane = int(e % 64), r = int((e / 64) % R), kk = int(e / (64 * R));
+      const int row = r * 16 + lane % 16, kc = kk * 4 + lane / 16;
+      op32[id * per + e] = float(op_entry(P, fillP, row, kc, a, b, I));
+    }
+  }
+  if (ophi) {
+    const int64_t per = int64_t(K32) * R * 64 * 8;
+    for (int64_t e = threadIdx.x; e < per; e += blockDim.x) {
+      const int i = int(e % 8), lane = int((e / 8) % 64), r = int((e / 512) % R), kk = int(e / (512 * R));
+      const int row = r * 16 + lane % 16, kc = kk * 32 + (lane / 16) * 8 + i;
+      const float v = float(op_entry(P, fillP, row, kc, a, b, I));
+      const __bf16 h = __bf16(v);
+      ophi[id * per + e] = h;
+      oplo[id * per + e] = __bf16(v - float(h));
+    }
+  }
+}
+
+// ---------------------------------------------------------------- GEMM
+// Epilogue: accumulator rows are scaled local coefficients L~, added to the
+// target's FP64 L~ (one writer per cell).
+// ---------------------------------------------------------------- GEMM
+// One workgroup = one tile of up to 256 targets and a block of RTB row
+// tiles (blockIdx.y); 8 waves = 4 column groups (64 targets, 4 column tiles
+// of 16) x 2 row halves (RH row tiles each).  Per key the block's operator
+// rows are copied once into LDS and shared by all 8 waves (256 targets per
+// operator load); each lane gathers its target's source multipoles from
+// global memory (L2-resident).  All keys of a tile accumulate into the same
+// MFMA accumulators, so each target's L is written once, without atomics.
+constexpr int kGemmThreads = 512;
+constexpr int kGemmLds = 65536;  // operator rows staged per key
+
+template <int RH>
+__device__ inline void write_locals(int PP, int rbase, int nr, const int *cells, int ncol, int cg,
+                                    const v4f (&acc)[RH][4], int lane, double *L) {
+#pragma unroll
+  for (int ct = 0; ct < 4; ct++) {
+    const int col = cg * 64 + ct * 16 + (lane & 15);
+    if (col >= ncol) continue;
+    double *Lc = L + int64_t(cells[col]) * PP;
+#pragma unroll
+    for (int r = 0; r < RH; r++) {
+      if (r >= nr) break;  // row tiles of this wave only (another block owns the next ones)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = (rbase + r) * 16 + 4 * (lane >> 4) + i;
+        if (row < PP) Lc[row] += double(acc[r][ct][i]);
+      }
+    }
+  }
+}
+
+template <int RH>
+__global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RTB, int K4, int KP,
+                                                          const int *__restrict__ tfirst,
+                                                          const int *__restrict__ tcount,
+                                                          const int *__restrict__ tbase, const int *__restrict__ tU,
+                                                          const int *__restrict__ sorted,
+                                                          const int *__restrict__ tkeys,
+                                                          const int *__restrict__ tmat,
+                                                          const float *__restrict__ op32,
+                                                          const float *__restrict__ mt, double *__restrict__ L) {
+  extern __shared__ float As[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int cg = w & 3, rh = w >> 2;
+  const int tile = blockIdx.x;
+  const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
+  const int rt0 = blockIdx.y * RTB;
+  const int nrt = min(RTB, RT - rt0);
+  const int rl0 = rh * RH;  // first local row tile of this wave
+  const int kq = lane >> 4;
+  v4f acc[RH][4];
+#pragma unroll
+  for (int r = 0; r < RH; r++)
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) acc[r][ct] = v4f{0, 0, 0, 0};
+  const int64_t per = int64_t(K4) * RT * 64;
+  const int E4 = K4 * nrt * 16;  // float4 per staged operator block
+  for (int u = 0; u < U; u++) {
+    const int key = tkeys[base + u];
+    int src[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) src[ct] = tmat[(int64_t(base) + u) * kTN + cg * 64 + ct * 16 + (lane & 15)];
+    __syncthreads();
+    const float *Ak = op32 + key * per;
+    for (int e = threadIdx.x; e < E4; e += kGemmThreads) {
+      const int l4 = e % 16, rl = (e / 16) % nrt, kk = e / (16 * nrt);
+      reinterpret_cast<float4 *>(As)[e] =
+          *reinterpret_cast<const float4 *>(Ak + (int64_t(kk) * RT + rt0 + rl) * 64 + l4 * 4);
+    }
+    __syncthreads();
+    for (int kk = 0; kk < K4; kk++) {
+      float bv[4];
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) bv[ct] = src[ct] < 0 ? 0.0f : mt[int64_t(src[ct]) * KP + kk * 4 + kq];
+#pragma unroll
+      for (int r = 0; r < RH; r++) {
+        if (rl0 + r >= nrt) break;
+        const float a = As[(kk * nrt + rl0 + r) * 64 + lane];
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[ct], acc[r][ct], 0, 0, 0);
+      }
+    }
+  }
+  write_locals<RH>(PP, rt0 + rl0, nrt - rl0, sorted + tfirst[tile], ncol, cg, acc, lane, L);
+}
+
+template <int RH>
+__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int RTB, int K32, int KP,
+                                                             const int *__restrict__ tfirst,
+                                                             const int *__restrict__ tcount,
+                                                             const int *__restrict__ tbase,
+                                                             const int *__restrict__ tU,
+                                                             const int *__restrict__ sorted,
+                                                             const int *__restrict__ tkeys,
+                                                             const int *__restrict__ tmat,
+                                                             const v8bf *__restrict__ ophi,
+                                                             const v8bf *__restrict__ oplo,
+                                                             const __bf16 *__restrict__ mhi,
+                                                             const __bf16 *__restrict__ mlo,
+                                                             double *__restrict__ L) {
+  extern __shared__ v8bf Ab[];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int cg = w & 3, rh = w >> 2;
+  const int tile = blockIdx.x;
+  const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
+  const int rt0 = blockIdx.y * RTB;
+  const int nrt = min(RTB, RT - rt0);
+  const int rl0 = rh * RH;
+  const int kq = lane >> 4;
+  v4f acc[RH][4];
+#pragma unroll
+  for (int r = 0; r < RH; r++)
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) acc[r][ct] = v4f{0, 0, 0, 0};
+  const int64_t per = int64_t(K32) * RT * 64;  // v8bf per key and part
+  const int E = K32 * nrt * 64;                // v8bf per staged part
+  v8bf *Ahs = Ab, *Als = Ab + E;
+  const v8bf zero = {};
+  for (int u = 0; u < U; u++) {
+    const int key = tkeys[base + u];
+    const v8bf *bh[4], *bl[4];
+    bool have[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+      const int sc = tmat[(int64_t(base) + u) * kTN + cg * 64 + ct * 16 + (lane & 15)];
+      have[ct] = sc >= 0;
+      bh[ct] = reinterpret_cast<const v8bf *>(mhi + int64_t(have[ct] ? sc : 0) * KP) + kq;
+      bl[ct] = reinterpret_cast<const v8bf *>(mlo + int64_t(have[ct] ? sc : 0) * KP) + kq;
+    }
+    __syncthreads();
+    const v8bf *Ah = ophi + key * per, *Al = oplo + key * per;
+    for (int e = threadIdx.x; e < E; e += kGemmThreads) {
+      const int l = e % 64, rl = (e / 64) % nrt, kk = e / (64 * nrt);
+      const int64_t g = (int64_t(kk) * RT + rt0 + rl) * 64 + l;
+      Ahs[e] = Ah[g];
+      Als[e] = Al[g];
+    }
+    __syncthreads();
+    for (int kk = 0; kk < K32; kk++) {
+      v8bf b_hi[4], b_lo[4];
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) {
+        b_hi[ct] = have[ct] ? bh[ct][kk * 4] : zero;
+        b_lo[ct] = have[ct] ? bl[ct][kk * 4] : zero;
+      }
+#pragma unroll
+      for (int r = 0; r < RH; r++) {
+        if (rl0 + r >= nrt) break;
+        const int ai = (kk * nrt + rl0 + r) * 64 + lane;
+        const v8bf a_hi = Ahs[ai], a_lo = Als[ai];
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) {
+          acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi[ct], acc[r][ct], 0, 0, 0);
+          acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo[ct], acc[r][ct], 0, 0, 0);
+          acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi[ct], acc[r][ct], 0, 0, 0);
+        }
+      }
+    }
+  }
+  write_locals<RH>(PP, rt0 + rl0, nrt - rl0, sorted + tfirst[tile], ncol, cg, acc, lane, L);
+}
+
+// Row tiles per workgroup so the staged operator block fits kGemmLds.
+static int rows_per_block(const Geom &g, bool bf16) {
+  const int per_rt = bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4;
+  return std::max(1, std::min(g.R, kGemmLds / per_rt));
+}
+
+template <int RH>
+static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, int RTB) {
+  Ctx &c = tc;
+  const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ntiles, *tbase = c.tiles.p + 2 * ntiles,
+            *tU = c.tiles.p + 3 * ntiles;
+  const int *sorted = c.tiles.p + 4 * ntiles;
+  const dim3 grid(ntiles, ceil_div(g.R, RTB));
+  if (bf16) {
+    const v8bf *ophi = reinterpret_cast<const v8bf *>(c.ops32.p);
+    const size_t per = size_t(g.K32) * g.R * 64 * 8;
+    const v8bf *oplo = reinterpret_cast<const v8bf *>(reinterpret_cast<const __bf16 *>(c.ops32.p) +
+                                                      per * c.lists.nkeys);
+    const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
+    const __bf16 *mlo = mhi + size_t(sc.t.ncells) * KPb;
+    const size_t lds = size_t(g.K32) * RTB * 64 * 16 * 2;
+    hipLaunchKernelGGL((k_m2l_bf16x3<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, tfirst,
+                       tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi, mlo, c.L.p);
+  } else {
+    const size_t lds = size_t(g.K4) * RTB * 64 * 4;
+    hipLaunchKernelGGL((k_m2l_f32<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K4, KP, tfirst,
+                       tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, c.ops32.p, sc.mt32.p, c.L.p);
+  }
+  XF_LAUNCH();
+}
+
+static bool ops_cached(const Ctx &c, const std::vector<unsigned long long> &keys, int fillP, int bf16) {
+  const bool periodic = c.cfg.images > 0;
+  return c.ops_P == c.P && c.ops_fill == fillP && c.ops_bf16 == bf16 && c.ops_keys == keys &&
+         (!periodic || (c.ops_R0 == c.t.R0 && c.ops_cycle == c.cfg.cycle));
+}
+
+static void ops_remember(Ctx &c, std::vector<unsigned long long> keys, int fillP, int bf16) {
+  c.ops_keys = std::move(keys);
+  c.ops_P = c.P;
+  c.ops_fill = fillP;
+  c.ops_bf16 = bf16;
+  c.ops_R0 = c.t.R0;
+  c.ops_cycle = c.cfg.cycle;
+}
+
+void m2l_fast(Ctx &tc, const Ctx &sc) {
+  if (&tc != &sc) fail(EXAFMM_ERR_UNSUPPORTED, "keyed M2L needs source == target tree");
+  Ctx &c = tc;
+  hipStream_t s = c.s0;
+  const Geom g = geom(c.P);
+  const bool bf16 = c.cfg.m2l_precision == EXAFMM_PREC_BF16X3;
+  const int fillP = c.cfg.m2l_truncation == EXAFMM_M2L_FULL ? 2 * c.P - 1 : c.P;
+  const int nc = int(c.t.ncells);
+  const int nkeys = c.lists.nkeys;
+  if (nkeys > (1 << 16)) fail(EXAFMM_ERR_CAPACITY, "more than 65536 distinct M2L translations");
+
+  // 1. tiles: cells with pairs grouped by (level, octant), BFS order inside
+  c.t.key.ensure(nc);
+  c.t.key2.ensure(nc);
+  c.t.val.ensure(nc);
+  c.t.val2.ensure(nc);
+  hipLaunchKernelGGL(k_group_keys, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, c.t.cx.p, c.t.cparent.p,
+                     c.t.clevel.p, c.lists.m2l_cnt.p, c.t.key.p, c.t.val.p);
+  XF_LAUNCH();
+  unsigned long long *k1 = c.t.key.p, *k2 = c.t.key2.p;
+  int *v1 = c.t.val.p, *v2 = c.t.val2.p;
+  radix_sort_pairs(k1, v1, k2, v2, nc, 0, 8, c.t.rs_hist, s);
+  c.misc.ensure(512);
+  XF_HIP(hipMemsetAsync(c.misc.p, 0, 256 * sizeof(int), s));
+  hipLaunchKernelGGL(k_group_hist, dim3(std::min(ceil_div(nc, 256), 64)), dim3(256), 0, s, nc, k1, c.misc.p);
+  XF_LAUNCH();
+  std::vector<int> gc(256);
+  std::vector<unsigned long long> keys(nkeys);
+  XF_HIP(hipMemcpyAsync(gc.data(), c.misc.p, 256 * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (nkeys)
+    XF_HIP(hipMemcpyAsync(keys.data(), c.lists.keys.p, sizeof(unsigned long long) * nkeys, hipMemcpyDeviceToHost,
+                          s));
+  XF_HIP(hipStreamSynchronize(s));
+  std::vector<int> tfirst, tcount;
+  int off = 0;
+  for (int gi = 0; gi < 255; gi++) {
+    for (int i = 0; i < gc[gi]; i += kTN) {
+      tfirst.push_back(off + i);
+      tcount.push_back(std::min(kTN, gc[gi] - i));
+    }
+    off += gc[gi];
+  }
+  const int ntiles = int(tfirst.size());
+  if (ntiles == 0) return;
+  c.tiles.ensure(size_t(4) * ntiles + nc);
+  XF_HIP(hipMemcpyAsync(c.tiles.p, tfirst.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, s));
+  XF_HIP(hipMemcpyAsync(c.tiles.p + ntiles, tcount.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, s));
+  XF_HIP(hipMemcpyAsync(c.tiles.p + 4 * ntiles, v1, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
+  int *d_first = c.tiles.p, *d_count = c.tiles.p + ntiles, *d_base = c.tiles.p + 2 * ntiles,
+      *d_U = c.tiles.p + 3 * ntiles, *d_sorted = c.tiles.p + 4 * ntiles;
+  const int nwords = (nkeys + 31) / 32;
+  hipLaunchKernelGGL(k_tile_count, dim3(ntiles), dim3(256), sizeof(unsigned) * nwords, s, d_first, d_count,
+                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, c.lists.slot2id.p, nkeys, d_U);
+  XF_LAUNCH();
+  scan_exclusive(d_U, d_base, ntiles, c.misc.p + 256, s);
+  int totalU = 0;
+  XF_HIP(hipMemcpyAsync(&totalU, c.misc.p + 256, sizeof(int), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipStreamSynchronize(s));
+  c.tile_keys.ensure(std::max(totalU, 1));
+  c.tile_mat.ensure(size_t(std::max(totalU, 1)) * kTN);
+  hipLaunchKernelGGL(k_tile_fill, dim3(ntiles), dim3(256), sizeof(unsigned) * 2 * nwords, s, d_first, d_count,
+                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, c.lists.slot2id.p, nkeys, d_base,
+                     c.tile_keys.p, c.tile_mat.p);
+  XF_LAUNCH();
+
+  // 2. operands: scaled multipoles, operators
+  const int KP = g.K4 * 4, KPb = g.K32 * 32;
+  const double cyc = c.cfg.images > 0 ? c.cfg.cycle : 0.0;
+  if (bf16) {
+    c.mt32.ensure((size_t(nc) * KPb * 2 + 1) / 2 + 1);  // two bf16 arrays
+    __bf16 *mhi = reinterpret_cast<__bf16 *>(c.mt32.p);
+    __bf16 *mlo = mhi + size_t(nc) * KPb;
+    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc) * KPb, 256)), dim3(256), 0, s, c.P * c.P, KPb, nc, c.M.p,
+                       (float *)nullptr, mhi, mlo);
+    XF_LAUNCH();
+    const size_t per = size_t(g.K32) * g.R * 64 * 8;
+    if (!ops_cached(c, keys, fillP, 1)) {
+      c.ops32.ensure((per * nkeys * 2 + 1) / 2 + 1);
+      __bf16 *ophi = reinterpret_cast<__bf16 *>(c.ops32.p);
+      hipLaunchKernelGGL(k_ops_gen, dim3(nkeys), dim3(256), 0, s, c.P, fillP, g.R, g.K4, g.K32, c.t.R0, cyc,
+                         c.lists.keys.p, (float *)nullptr, ophi, ophi + per * nkeys);
+      XF_LAUNCH();
+      ops_remember(c, std::move(keys), fillP, 1);
+    }
+  } else {
+    c.mt32.ensure(size_t(nc) * KP);
+    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc) * KP, 256)), dim3(256), 0, s, c.P * c.P, KP, nc, c.M.p,
+                       c.mt32.p, (__bf16 *)nullptr, (__bf16 *)nullptr);
+    XF_LAUNCH();
+    const size_t per = size_t(g.K4) * g.R * 64;
+    if (!ops_cached(c, keys, fillP, 0)) {
+      c.ops32.ensure(per * nkeys);
+      hipLaunchKernelGGL(k_ops_gen, dim3(nkeys), dim3(256), 0, s, c.P, fillP, g.R, g.K4, g.K32, c.t.R0, cyc,
+                         c.lists.keys.p, c.ops32.p, (__bf16 *)nullptr, (__bf16 *)nullptr);
+      XF_LAUNCH();
+      ops_remember(c, std::move(keys), fillP, 0);
+    }
+  }
+
+  // 3. GEMM over tiles
+  XF_HIP(hipEventRecord(c.ev[13], s));
+  const int RTB = rows_per_block(g, bf16);
+  switch ((RTB + 1) / 2) {
+    case 1: launch_gemm<1>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
+    case 2: launch_gemm<2>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
+    case 3: launch_gemm<3>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
+    case 4: launch_gemm<4>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
+    case 5: launch_gemm<5>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
+    default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
+  }
+}
+
 }  // namespace xf

@@ -1,0 +1,280 @@
+// periodic.hip -- far periodic images, dipole correction, Ewald oracle.
+//
+// Far images (3dp/traverse_lazy.h:75-121, eager 3dp/traverse_eager.h:46-92):
+// a super-cell starts as the root (centre X0, multipole M_root); at each of
+// images-1 levels the root's local expansion receives M2L from the 26 x 27
+// copies at offsets (3i + c) * cycle (i != 0), then the super-cell absorbs
+// its 26 neighbours by M2M and cycle triples.  Each level is one launch of
+// 702 workgroups writing partial locals, summed in a fixed order.
+//
+// Dipole correction (3dp/fmm.cxx:63-75): p -= 4 pi/(3 cycle^3) |D|^2 / N / q,
+// F -= 4 pi/(3 cycle^3) D, D = sum q X.  Bodies with q = 0 keep p (the
+// reference divides by q).
+//
+// Ewald sum (3dp/ewald.h:22-150) in FP64 on the device: wave part by
+// DFT/IDFT over the half space |k| <= ksize, real part with erfc over each
+// body pair's nearest image within the cutoff, self term.
+#include <cmath>
+
+#include "fmm_internal.h"
+#include "fmm_ops.h"
+
+namespace xf {
+using namespace xfd;
+
+__global__ void __launch_bounds__(64) k_far_m2l(int P, int NT, int fillP, double cyc, const double2 *__restrict__ Msup,
+                                                double2 *__restrict__ part) {
+  __shared__ cpx<double> I[(2 * kMaxP) * (2 * kMaxP + 1) / 2];
+  __shared__ double2 Ms[kMaxP * (kMaxP + 1) / 2];
+  const int e = blockIdx.x;  // 26 neighbour blocks x 27 sub-images, reference loop order
+  const int ib = e / 27, cidx = e % 27;
+  const int nb = ib < 13 ? ib : ib + 1;  // skip the centre block
+  const int ix = nb / 9 - 1, iy = (nb / 3) % 3 - 1, iz = nb % 3 - 1;
+  const int cx = cidx / 9 - 1, cy = (cidx / 3) % 3 - 1, cz = cidx % 3 - 1;
+  // dX = X_root - X_super - iX * cycle, with X_super = X_root
+  const double dx = -(ix * 3 + cx) * cyc, dy = -(iy * 3 + cy) * cyc, dz = -(iz * 3 + cz) * cyc;
+  if (threadIdx.x < fillP) singular_column(threadIdx.x, fillP, dx, dy, dz, I);
+  for (int i = threadIdx.x; i < NT; i += 64) Ms[i] = Msup[i];
+  __syncthreads();
+  for (int jk = threadIdx.x; jk < NT; jk += 64) {
+    int j, k;
+    jk_of(jk, j, k);
+    part[int64_t(e) * NT + jk] = m2l_coef(P, fillP, j, k, Ms, I);
+  }
+}
+
+// Sum of the partial locals in launch order, added to the root's scaled L~.
+__global__ void __launch_bounds__(64) k_far_sum(int P, int nparts, double R0, const double2 *__restrict__ part,
+                                                double *__restrict__ Lroot) {
+  const int NT = P * (P + 1) / 2;
+  for (int jk = threadIdx.x; jk < NT; jk += 64) {
+    double ax = 0, ay = 0;
+    for (int e = 0; e < nparts; e++) {
+      ax += part[int64_t(e) * NT + jk].x;
+      ay += part[int64_t(e) * NT + jk].y;
+    }
+    int j, k;
+    jk_of(jk, j, k);
+    const double s = pow(R0, double(j + 1));
+    Lroot[rf(j, k, 0)] += ax * s;
+    if (k > 0) Lroot[rf(j, k, 1)] += ay * s;
+  }
+}
+
+// Root multipole M = M~ R0^n (complex) as the first super-cell.
+__global__ void __launch_bounds__(64) k_far_init(int P, double R0, const double *__restrict__ Mroot,
+                                                 double2 *__restrict__ sup) {
+  const int NT = P * (P + 1) / 2;
+  for (int jk = threadIdx.x; jk < NT; jk += 64) {
+    int n, m;
+    jk_of(jk, n, m);
+    const double s = pow(R0, double(n));
+    sup[jk] = make_double2(Mroot[rf(n, m, 0)] * s, m ? Mroot[rf(n, m, 1)] * s : 0.0);
+  }
+}
+
+// Msup += sum over the 26 neighbours (child centre X + i*cycle) of M2M(Msup).
+__global__ void __launch_bounds__(64) k_far_m2m(int P, int NT, double cyc, double2 *__restrict__ Msup) {
+  __shared__ cpx<double> Y[kMaxP * (kMaxP + 1) / 2];
+  __shared__ double2 Mc[kMaxP * (kMaxP + 1) / 2];
+  for (int i = threadIdx.x; i < NT; i += 64) Mc[i] = Msup[i];
+  double2 acc[3] = {};
+  for (int nb = 0; nb < 27; nb++) {
+    if (nb == 13) continue;
+    const int ix = nb / 9 - 1, iy = (nb / 3) % 3 - 1, iz = nb % 3 - 1;
+    __syncthreads();
+    if (threadIdx.x == 0) regular_all<double>(P, -ix * cyc, -iy * cyc, -iz * cyc, Y);  // Xparent - Xchild
+    __syncthreads();
+    for (int r = 0; r * 64 + int(threadIdx.x) < NT; r++) {
+      int j, k;
+      jk_of(r * 64 + threadIdx.x, j, k);
+      const double2 v = m2m_coef(j, k, Mc, Y);
+      acc[r].x += v.x;
+      acc[r].y += v.y;
+    }
+  }
+  __syncthreads();
+  for (int r = 0; r * 64 + int(threadIdx.x) < NT; r++) {
+    const int jk = r * 64 + threadIdx.x;
+    Msup[jk].x = Mc[jk].x + acc[r].x;
+    Msup[jk].y = Mc[jk].y + acc[r].y;
+  }
+}
+
+void periodic_far(Ctx &c) {
+  const int P = c.P, NT = c.NT;
+  const int fillP = c.cfg.m2l_truncation == EXAFMM_M2L_FULL ? 2 * P - 1 : P;
+  if (c.cfg.images < 2) return;
+  Buf<double2> &sup = c.far_sup, &part = c.far_part;
+  sup.ensure(NT);
+  part.ensure(size_t(702) * NT);
+  hipLaunchKernelGGL(k_far_init, dim3(1), dim3(64), 0, c.s0, P, c.t.R0, c.M.p, sup.p);
+  XF_LAUNCH();
+  double cyc = c.cfg.cycle;
+  for (int lev = 0; lev < c.cfg.images - 1; lev++) {
+    hipLaunchKernelGGL(k_far_m2l, dim3(702), dim3(64), 0, c.s0, P, NT, fillP, cyc, sup.p, part.p);
+    XF_LAUNCH();
+    hipLaunchKernelGGL(k_far_sum, dim3(1), dim3(64), 0, c.s0, P, 702, c.t.R0, part.p, c.L.p);
+    XF_LAUNCH();
+    hipLaunchKernelGGL(k_far_m2m, dim3(1), dim3(64), 0, c.s0, P, NT, cyc, sup.p);
+    XF_LAUNCH();
+    cyc *= 3;
+  }
+}
+
+// ---------------------------------------------------------------- dipole
+__global__ void __launch_bounds__(256) k_dipole(int64_t n, const exafmm_body *__restrict__ b, double *__restrict__ out) {
+  __shared__ double s[3][256];
+  double d[3] = {0, 0, 0};
+  for (int64_t i = threadIdx.x; i < n; i += 256)
+    for (int k = 0; k < 3; k++) d[k] += b[i].X[k] * b[i].q;
+  for (int k = 0; k < 3; k++) s[k][threadIdx.x] = d[k];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 3; k++) s[k][threadIdx.x] += s[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_dipole_apply(int64_t n, double coef, const double *__restrict__ D,
+                                                      exafmm_body *__restrict__ b) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double dn = D[0] * D[0] + D[1] * D[1] + D[2] * D[2];
+  if (b[i].q != 0) b[i].p -= coef * dn / double(n) / b[i].q;
+  for (int k = 0; k < 3; k++) b[i].F[k] -= coef * D[k];
+}
+
+void dipole_correction(Ctx &c) {
+  c.far_sup.ensure(c.NT > 2 ? c.NT : 2);
+  double *D = reinterpret_cast<double *>(c.far_sup.p);
+  hipLaunchKernelGGL(k_dipole, dim3(1), dim3(256), 0, c.s0, c.t.n, c.bodies, D);
+  XF_LAUNCH();
+  const double cyc = c.cfg.cycle;
+  const double coef = 4 * M_PI / (3 * cyc * cyc * cyc);
+  hipLaunchKernelGGL(k_dipole_apply, dim3(ceil_div(c.t.n, 256)), dim3(256), 0, c.s0, c.t.n, coef, D, c.bodies);
+  XF_LAUNCH();
+}
+
+// ---------------------------------------------------------------- Ewald
+struct Wave {
+  double K[3];
+  double re, im;
+};
+
+__global__ void __launch_bounds__(256) k_ewald_dft(int nw, Wave *__restrict__ waves, const exafmm_body *__restrict__ s,
+                                                   int64_t ns, double scale, double coef, double coef2) {
+  __shared__ double red[2][256];
+  const int w = blockIdx.x;
+  if (w >= nw) return;
+  const double K0 = waves[w].K[0], K1 = waves[w].K[1], K2 = waves[w].K[2];
+  double re = 0, im = 0;
+  for (int64_t b = threadIdx.x; b < ns; b += 256) {
+    const double th = K0 * s[b].X[0] * scale + K1 * s[b].X[1] * scale + K2 * s[b].X[2] * scale;
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    re += s[b].q * cs;
+    im += s[b].q * sn;
+  }
+  red[0][threadIdx.x] = re;
+  red[1][threadIdx.x] = im;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + o];
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double k0 = K0 * scale, k1 = K1 * scale, k2 = K2 * scale;
+    const double KK = k0 * k0 + k1 * k1 + k2 * k2;
+    const double factor = coef * exp(-KK * coef2) / KK;
+    waves[w].re = red[0][0] * factor;
+    waves[w].im = red[1][0] * factor;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ewald_target(int nw, const Wave *__restrict__ waves, exafmm_body *__restrict__ t,
+                                                      int64_t nt, const exafmm_body *__restrict__ s, int64_t ns,
+                                                      double scale, double alpha, double cutoff, double cyc) {
+  __shared__ double red[4][256];
+  const int64_t ti = blockIdx.x;
+  if (ti >= nt) return;
+  const double x = t[ti].X[0], y = t[ti].X[1], z = t[ti].X[2];
+  double p = 0, F[3] = {0, 0, 0};
+  // wave part (3dp/ewald.h:36-51)
+  for (int w = threadIdx.x; w < nw; w += 256) {
+    const double th = waves[w].K[0] * x * scale + waves[w].K[1] * y * scale + waves[w].K[2] * z * scale;
+    double sn, cs;
+    sincos(th, &sn, &cs);
+    const double dtmp = waves[w].re * sn - waves[w].im * cs;
+    p += waves[w].re * cs + waves[w].im * sn;
+    for (int d = 0; d < 3; d++) F[d] -= dtmp * waves[w].K[d] * scale;
+  }
+  // real part (3dp/ewald.h:82-102), nearest image of each pair
+  for (int64_t j = threadIdx.x; j < ns; j += 256) {
+    double d[3] = {x - s[j].X[0], y - s[j].X[1], z - s[j].X[2]};
+    for (int k = 0; k < 3; k++) {
+      if (d[k] < -cyc / 2) d[k] += cyc;
+      if (d[k] > cyc / 2) d[k] -= cyc;
+    }
+    const double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (0 < R2 && R2 < cutoff * cutoff) {
+      const double R2s = R2 * alpha * alpha;
+      const double Rs = sqrt(R2s);
+      const double invRs = 1 / Rs;
+      const double invR2s = invRs * invRs;
+      const double invR3s = invR2s * invRs;
+      const double ec = erfc(Rs);
+      double dtmp = s[j].q * (M_2_SQRTPI * exp(-R2s) * invR2s + ec * invR3s);
+      dtmp *= alpha * alpha * alpha;
+      p += s[j].q * ec * invRs * alpha;
+      for (int k = 0; k < 3; k++) F[k] -= d[k] * dtmp;
+    }
+  }
+  red[0][threadIdx.x] = p;
+  for (int k = 0; k < 3; k++) red[1 + k][threadIdx.x] = F[k];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    t[ti].p += red[0][0] - M_2_SQRTPI * t[ti].q * alpha;  // self term (3dp/ewald.h:146-150)
+    for (int k = 0; k < 3; k++) t[ti].F[k] += red[1 + k][0];
+  }
+}
+
+void ewald_sum(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns, int ksize, double alpha,
+               double sigma, double cutoff, double cyc) {
+  std::vector<Wave> w;
+  const int kmax = ksize, kmaxsq = ksize * ksize;
+  for (int l = 0; l <= kmax; l++) {
+    const int mmin = l == 0 ? 0 : -kmax;
+    for (int m = mmin; m <= kmax; m++) {
+      const int nmin = (l == 0 && m == 0) ? 1 : -kmax;
+      for (int n = nmin; n <= kmax; n++)
+        if (l * l + m * m + n * n <= kmaxsq) w.push_back({{double(l), double(m), double(n)}, 0, 0});
+    }
+  }
+  const int nw = int(w.size());
+  Buf<Wave> dw;
+  dw.ensure(std::max(nw, 1));
+  if (nw) XF_HIP(hipMemcpyAsync(dw.p, w.data(), sizeof(Wave) * nw, hipMemcpyHostToDevice, c.s0));
+  const double scale = 2 * M_PI / cyc;
+  const double coef = 2 / sigma / cyc / cyc / cyc;
+  const double coef2 = 1 / (4 * alpha * alpha);
+  if (nw) {
+    hipLaunchKernelGGL(k_ewald_dft, dim3(nw), dim3(256), 0, c.s0, nw, dw.p, s, ns, scale, coef, coef2);
+    XF_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_ewald_target, dim3(nt), dim3(256), 0, c.s0, nw, dw.p, t, nt, s, ns, scale, alpha, cutoff, cyc);
+  XF_LAUNCH();
+  XF_HIP(hipStreamSynchronize(c.s0));
+}
+
+}  // namespace xf

@@ -45,7 +45,7 @@ void scan_exclusive_dev(const int *in, int *out, const int *count_dev, int *tota
 
 // ---------------------------------------------------------------- radix sort
 constexpr int kRsThreads = 256;
-constexpr int kRsItems = 16;
+constexpr int kRsItems = 4;
 constexpr int kRsTile = kRsThreads * kRsItems;
 
 __global__ void __launch_bounds__(kRsThreads) k_rs_hist(const unsigned long long *__restrict__ key, int n, int shift,
@@ -71,13 +71,26 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_hist(const unsigned long long
 __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const unsigned long long *__restrict__ key,
                                                            const int *__restrict__ val, int n, int shift, int nbits,
                                                            const int *__restrict__ off, int nblocks,
+                                                           const int *__restrict__ dtot,
                                                            unsigned long long *__restrict__ key_out,
                                                            int *__restrict__ val_out) {
+  static_assert(kRsThreads == 256, "one thread per digit");
   __shared__ int run[256];
   __shared__ int wcnt[kRsThreads / kWave][256];
   const int t = threadIdx.x;
   const int lane = t & (kWave - 1), w = t / kWave;
-  run[t] = off[t * nblocks + blockIdx.x];
+  // digit bases: exclusive scan of the digit totals
+  run[t] = dtot[t];
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const int v = t >= o ? run[t - o] : 0;
+    __syncthreads();
+    run[t] += v;
+    __syncthreads();
+  }
+  const int dbase = run[t] - dtot[t];
+  __syncthreads();
+  run[t] = dbase + off[t * nblocks + blockIdx.x];
   const unsigned mask = (1u << nbits) - 1u;
   const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const int base = blockIdx.x * kRsTile;
@@ -119,21 +132,49 @@ __global__ void __launch_bounds__(kRsThreads) k_rs_scatter(const unsigned long l
   }
 }
 
+// Offsets for the scatter: one workgroup per digit scans that digit's
+// per-block counts (exclusive) and records the digit total; each scatter
+// workgroup adds the digit bases (scan of the totals) itself.
+__global__ void __launch_bounds__(256) k_rs_scan_digit(const int *__restrict__ h, int *__restrict__ o, int nblocks,
+                                                       int *__restrict__ dtot) {
+  __shared__ int part[256];
+  const int d = blockIdx.x, t = threadIdx.x;
+  const int chunk = (nblocks + 255) / 256;
+  const int b = t * chunk, e = min(b + chunk, nblocks);
+  const int *hd = h + d * nblocks;
+  int s = 0;
+  for (int i = b; i < e; i++) s += hd[i];
+  part[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 256; off <<= 1) {
+    const int v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t ? part[t - 1] : 0;
+  for (int i = b; i < e; i++) {
+    o[d * nblocks + i] = run;
+    run += hd[i];
+  }
+  if (t == 255) dtot[d] = part[255];
+}
+
 // Sorts (k, v) by key bits [lo_bit, hi_bit); the result ends in (k, v)
 // (the pointers are swapped as the passes ping-pong).
 void radix_sort_pairs(unsigned long long *&k, int *&v, unsigned long long *&k2, int *&v2, int n, int lo_bit,
                       int hi_bit, Buf<int> &hist, hipStream_t s) {
   if (n <= 1 || hi_bit <= lo_bit) return;
   const int nb = ceil_div(n, kRsTile);
-  hist.ensure(size_t(256) * nb * 2 + 4);
-  int *h = hist.p, *o = hist.p + 256 * nb;
+  hist.ensure(size_t(256) * nb * 2 + 256);
+  int *h = hist.p, *o = hist.p + 256 * nb, *dt = hist.p + 512 * nb;
   for (int sh = lo_bit; sh < hi_bit; sh += 8) {
     const int bits = min(8, hi_bit - sh);
     hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(kRsThreads), 0, s, k, n, sh, bits, h, nb);
     XF_LAUNCH();
-    hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, h, o, 256 * nb, nullptr, nullptr);
+    hipLaunchKernelGGL(k_rs_scan_digit, dim3(256), dim3(256), 0, s, h, o, nb, dt);
     XF_LAUNCH();
-    hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kRsThreads), 0, s, k, v, n, sh, bits, o, nb, k2, v2);
+    hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(kRsThreads), 0, s, k, v, n, sh, bits, o, nb, dt, k2, v2);
     XF_LAUNCH();
     std::swap(k, k2);
     std::swap(v, v2);

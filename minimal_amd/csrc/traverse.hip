@@ -15,6 +15,11 @@
 // exclusive scan, and a fill pass writing at the scanned offsets.
 // The MAC is evaluated in FP64 with contraction disabled, so each pair is
 // classified exactly as the reference classifies it.
+//
+// Periodic boundary (3dp/traverse_lazy.h:39-57, 124-135): the source tree
+// is visited in its 27 near images; every list entry carries its image
+// index (packed as cell * 32 + image, image = ix+1 + 3(iy+1) + 9(iz+1)) and
+// the MAC uses dX = Xc - Xb - iX * cycle.  Free space uses image 13 (0,0,0).
 #include "fmm_internal.h"
 
 namespace xf {
@@ -31,6 +36,8 @@ struct TravArgs {
   const double4 *scx;
   const int *scnch, *scchild, *sclevel;
   double theta;
+  double cycle;
+  int periodic;
   int lstart, lcount;
   // hand-down lists
   const int *in_beg, *in_cnt, *in;
@@ -56,18 +63,24 @@ __device__ inline unsigned long long mix64(unsigned long long x) {
   return x;
 }
 
-// Key of the M2L translation (c <- b): offset in units of the finer of the
-// two radii (an integer vector for cells of one tree) and the level
-// difference.  Under radius scaling of M and L the operator depends on
-// nothing else.
-__device__ inline bool m2l_key(double4 Xc, int lc, double4 Xb, int lb, unsigned long long &key) {
+// Key of the M2L translation (c <- b): integer offset (Xc - Xb) / Rf in units
+// of the finer radius Rf, level difference, and -- periodic only -- the
+// image and the level of Rf (the image shift is not a whole number of
+// cell widths).  Under radius scaling of M and L the operator depends on
+// nothing else.  Layout: dx,dy,dz 16 bits each, image 5, level 5, diff 6.
+__device__ inline bool m2l_key(double4 Xc, int lc, double4 Xb, int lb, int img, int periodic,
+                               unsigned long long &key) {
   const double rf = fmin(Xc.w, Xb.w);
   const double dx = rint((Xc.x - Xb.x) / rf), dy = rint((Xc.y - Xb.y) / rf), dz = rint((Xc.z - Xb.z) / rf);
-  const double lim = 131071.0;
-  if (fabs(dx) > lim || fabs(dy) > lim || fabs(dz) > lim) return false;
-  const long long ix = (long long)dx + 131072, iy = (long long)dy + 131072, iz = (long long)dz + 131072;
-  key = ((unsigned long long)ix << 46) | ((unsigned long long)iy << 28) | ((unsigned long long)iz << 10) |
-        (unsigned long long)(lc - lb + 64);
+  const double lim = 32767.0;
+  const int ld = lc - lb;
+  if (fabs(dx) > lim || fabs(dy) > lim || fabs(dz) > lim || ld < -31 || ld > 31) return false;
+  const unsigned long long ix = (unsigned long long)((long long)dx + 32768);
+  const unsigned long long iy = (unsigned long long)((long long)dy + 32768);
+  const unsigned long long iz = (unsigned long long)((long long)dz + 32768);
+  const unsigned long long im = periodic ? unsigned(img) : 13u;
+  const unsigned long long lf = periodic ? unsigned(max(lc, lb)) : 0u;
+  key = (ix << 48) | (iy << 32) | (iz << 16) | (im << 11) | (lf << 6) | (unsigned long long)(ld + 32);
   return true;
 }
 
@@ -113,17 +126,21 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
     const int par = a.cparent[c];
     const int ib = a.in_beg[par];
     tail = a.in_cnt[par];
-    if (tail > kQCap) {
+    if (tail > kQCap || int64_t(ib) + tail > a.cap_out) {
       if (lane == 0) atomicOr(a.flags, 4);
       return;
     }
     for (int i = lane; i < tail; i += kWave) q[i] = a.in[ib + i];
+  } else if (a.periodic) {
+    tail = 27;
+    if (lane < 27) q[lane] = lane;  // (root, image) for the 27 near images
   } else {
     tail = 1;
-    if (lane == 0) q[0] = 0;
+    if (lane == 0) q[0] = 13;  // (root, no shift)
   }
-  __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 
   const double4 Xc = a.cx[c];
   const int leafc = a.cnch[c] == 0;
@@ -141,14 +158,19 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
     const int k = min(kWave, tail - head);
     int b = -1, kind = -1, nchb = 0;
     double4 Xb;
+    int img = 13;
     if (lane < k) {
-      b = q[(head + lane) % kQCap];
+      const int e = q[(head + lane) % kQCap];
+      b = e >> 5;
+      img = e & 31;
       Xb = a.scx[b];
       nchb = a.scnch[b];
+      const double sx = ((img % 3) - 1) * a.cycle, sy = (((img / 3) % 3) - 1) * a.cycle,
+                   sz = ((img / 9) - 1) * a.cycle;
       bool mac;
       {
 #pragma clang fp contract(off)
-        const double dx = Xc.x - Xb.x, dy = Xc.y - Xb.y, dz = Xc.z - Xb.z;
+        const double dx = Xc.x - Xb.x - sx, dy = Xc.y - Xb.y - sy, dz = Xc.z - Xb.z - sz;
         const double R2 = (dx * dx + dy * dy + dz * dz) * th2 * th2;
         const double Rs = Xc.w + Xb.w;
         mac = R2 > Rs * Rs;
@@ -169,18 +191,18 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
         int slot = 0;
         if (a.want_keys) {
           unsigned long long key;
-          if (m2l_key(Xc, lc, Xb, a.sclevel[b], key))
+          if (m2l_key(Xc, lc, Xb, a.sclevel[b], img, a.periodic, key))
             slot = key_insert(a.hslots, a.hmask, key, a.flags);
           else
             atomicOr(a.flags, 16);
         }
-        if (pos < a.cap_m2l) a.m2l[pos] = make_int2(b, slot);
+        if (pos < a.cap_m2l) a.m2l[pos] = make_int2((b << 5) | img, slot);
       } else if (kind == 1) {
         const int pos = bp + np + __popcll(mp & lt);
-        if (pos < a.cap_p2p) a.p2p[pos] = b;
+        if (pos < a.cap_p2p) a.p2p[pos] = (b << 5) | img;
       } else if (kind == 2) {
         const int pos = bo + no + __popcll(mo & lt);
-        if (pos < a.cap_out) a.out[pos] = b;
+        if (pos < a.cap_out) a.out[pos] = (b << 5) | img;
       }
     }
     nm += __popcll(mm);
@@ -196,7 +218,7 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
     }
     if (nadd) {
       const int first = a.scchild[b];
-      for (int i = 0; i < nadd; i++) q[(tail + off + i) % kQCap] = first + i;
+      for (int i = 0; i < nadd; i++) q[(tail + off + i) % kQCap] = ((first + i) << 5) | img;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -288,7 +310,8 @@ void traverse(Ctx &tc, const Ctx &sc) {
   Lists &Lr = tc.lists;
   hipStream_t s = tc.s0;
   const int64_t nc = T.ncells;
-  const bool keys = (&tc == &sc) && tc.cfg.images == 0;
+  const bool keys = (&tc == &sc);
+  if (S.ncells >= (int64_t(1) << 26)) fail(EXAFMM_ERR_CAPACITY, "more than 2^26 cells");
   Lr.m2l_beg.ensure(nc);
   Lr.m2l_cnt.ensure(nc);
   Lr.p2p_beg.ensure(nc);
@@ -321,6 +344,8 @@ void traverse(Ctx &tc, const Ctx &sc) {
     a.scchild = S.cchild.p;
     a.sclevel = S.clevel.p;
     a.theta = tc.cfg.theta;
+    a.periodic = tc.cfg.images > 0;
+    a.cycle = tc.cfg.images > 0 ? tc.cfg.cycle : 0.0;
     a.out_cnt = Lr.out_cnt.p;
     a.out_beg = Lr.out_beg.p;
     a.out = Lr.out.p;

@@ -22,7 +22,6 @@ __host__ __device__ constexpr int hist_off(int l) {  // offset of level l (1..7)
   return l <= 1 ? 0 : hist_off(l - 1) + (1 << (3 * (l - 1)));
 }
 constexpr int kHistTotal = hist_off(kHistLevels + 1);
-constexpr int kHistShared = hist_off(5);  // levels 1..4 aggregated in LDS
 
 // ---------------------------------------------------------------- bounds
 __global__ void __launch_bounds__(256) k_bounds_part(const exafmm_body *__restrict__ b, int64_t n,
@@ -58,20 +57,34 @@ __global__ void __launch_bounds__(256) k_bounds_part(const exafmm_body *__restri
 }
 
 // X0 = (max+min)/2, R0 = max half-extent * 1.00001 (3d/build_tree.h:7-21).
-__global__ void k_bounds_final(const double *__restrict__ part, int nparts, double *__restrict__ root) {
-  if (threadIdx.x != 0) return;
-  double lo[3], hi[3];
-  for (int d = 0; d < 3; d++) {
-    lo[d] = part[d];
-    hi[d] = part[3 + d];
-  }
-  for (int p = 1; p < nparts; p++)
+__global__ void __launch_bounds__(256) k_bounds_final(const double *__restrict__ part, int nparts,
+                                                      double *__restrict__ root) {
+  __shared__ double s[6][256];
+  const int t = threadIdx.x;
+  double v[6];
+  for (int d = 0; d < 6; d++) v[d] = part[d];
+  for (int p = t; p < nparts; p += 256)
     for (int d = 0; d < 3; d++) {
-      lo[d] = fmin(part[p * 6 + d], lo[d]);
-      hi[d] = fmax(part[p * 6 + 3 + d], hi[d]);
+      v[d] = fmin(part[p * 6 + d], v[d]);
+      v[3 + d] = fmax(part[p * 6 + 3 + d], v[3 + d]);
     }
-  double X0[3];
-  for (int d = 0; d < 3; d++) X0[d] = (hi[d] + lo[d]) / 2;
+  for (int d = 0; d < 6; d++) s[d][t] = v[d];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int d = 0; d < 3; d++) {
+        s[d][t] = fmin(s[d][t], s[d][t + o]);
+        s[3 + d][t] = fmax(s[3 + d][t], s[3 + d][t + o]);
+      }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  double lo[3], hi[3], X0[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = s[d][0];
+    hi[d] = s[3 + d][0];
+    X0[d] = (hi[d] + lo[d]) / 2;
+  }
   double R0 = 0;
   for (int d = 0; d < 3; d++) {
     R0 = fmax(X0[d] - lo[d], R0);
@@ -85,51 +98,87 @@ __global__ void k_bounds_final(const double *__restrict__ part, int nparts, doub
 }
 
 // ---------------------------------------------------------------- keys
-// Octant path of each body through the reference's centre sequence.
+// Octant path of each body through the reference's centre sequence, and
+// the population of its level-7 prefix (dense, 8^7 bins: spread-out atomics).
 __global__ void __launch_bounds__(256) k_keys(const exafmm_body *__restrict__ b, int64_t n,
                                               const double *__restrict__ root, unsigned long long *__restrict__ key,
-                                              int *__restrict__ val, int *__restrict__ hist) {
-  __shared__ int sh[kHistShared];
-  for (int i = threadIdx.x; i < kHistShared; i += 256) sh[i] = 0;
-  __syncthreads();
+                                              int *__restrict__ val, int *__restrict__ hist7) {
   const double R0 = root[3];
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i < n) {
-    const double x = b[i].X[0], y = b[i].X[1], z = b[i].X[2];
-    double cx = root[0], cy = root[1], cz = root[2];
-    unsigned long long k = 0;
-    for (int l = 0; l < kMaxLevel; l++) {
-      const int bx = x > cx, by = y > cy, bz = z > cz;
-      const int oct = bx + (by << 1) + (bz << 2);
-      k = (k << 3) | unsigned(oct);
-      const double r = R0 / (1 << (l + 1));
-      cx += bx ? r : -r;
-      cy += by ? r : -r;
-      cz += bz ? r : -r;
-    }
-    key[i] = k;
-    val[i] = int(i);
-    for (int l = 1; l <= kHistLevels; l++) {
-      const int pre = int(k >> (3 * (kMaxLevel - l)));
-      if (l <= 4)
-        atomicAdd(&sh[hist_off(l) + pre], 1);
-      else
-        atomicAdd(&hist[hist_off(l) + pre], 1);
-    }
+  if (i >= n) return;
+  const double x = b[i].X[0], y = b[i].X[1], z = b[i].X[2];
+  double cx = root[0], cy = root[1], cz = root[2];
+  unsigned long long k = 0;
+  for (int l = 0; l < kMaxLevel; l++) {
+    const int bx = x > cx, by = y > cy, bz = z > cz;
+    const int oct = bx + (by << 1) + (bz << 2);
+    k = (k << 3) | unsigned(oct);
+    const double r = R0 / (1 << (l + 1));
+    cx += bx ? r : -r;
+    cy += by ? r : -r;
+    cz += bz ? r : -r;
   }
-  __syncthreads();
-  for (int j = threadIdx.x; j < kHistShared; j += 256)
-    if (sh[j]) atomicAdd(&hist[j], sh[j]);
+  key[i] = k;
+  val[i] = int(i);
+  atomicAdd(&hist7[int(k >> (3 * (kMaxLevel - kHistLevels)))], 1);
 }
 
-// Largest prefix population of each histogram level.
-__global__ void __launch_bounds__(256) k_hist_max(const int *__restrict__ hist, int *__restrict__ lmax) {
-  for (int j = blockIdx.x * 256 + threadIdx.x; j < kHistTotal; j += gridDim.x * 256) {
-    const int v = hist[j];
-    if (!v) continue;
-    int l = 1;
-    while (l < kHistLevels && j >= hist_off(l + 1)) l++;
-    atomicMax(&lmax[l], v);
+// Prefix populations of levels 6 and 5 by summing children (prefixes are
+// in Morton order, so a level-l prefix p has children 8p..8p+7), and the
+// largest population per level: block maxima, one atomic per block.
+__global__ void __launch_bounds__(256) k_hist_up(int *__restrict__ hist, int *__restrict__ lmax) {
+  __shared__ int m[3][256];
+  const int p5 = blockIdx.x * 256 + threadIdx.x;  // one thread per level-5 prefix
+  const int *h7 = hist + hist_off(7) + p5 * 64;
+  int s5 = 0, m6 = 0, m7 = 0;
+  for (int c6 = 0; c6 < 8; c6++) {
+    int s6 = 0;
+    for (int c7 = 0; c7 < 8; c7++) {
+      const int v = h7[c6 * 8 + c7];
+      s6 += v;
+      m7 = max(m7, v);
+    }
+    hist[hist_off(6) + p5 * 8 + c6] = s6;
+    m6 = max(m6, s6);
+    s5 += s6;
+  }
+  hist[hist_off(5) + p5] = s5;
+  m[0][threadIdx.x] = s5;
+  m[1][threadIdx.x] = m6;
+  m[2][threadIdx.x] = m7;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o)
+      for (int k = 0; k < 3; k++) m[k][threadIdx.x] = max(m[k][threadIdx.x], m[k][threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicMax(&lmax[5], m[0][0]);
+    atomicMax(&lmax[6], m[1][0]);
+    atomicMax(&lmax[7], m[2][0]);
+  }
+}
+
+// Levels 4..1 from level 5 in one workgroup.
+__global__ void __launch_bounds__(1024) k_hist_top(int *__restrict__ hist, int *__restrict__ lmax) {
+  __shared__ int red[1024];
+  for (int l = 4; l >= 1; l--) {
+    const int nb = 1 << (3 * l);
+    int mx = 0;
+    for (int p = threadIdx.x; p < nb; p += 1024) {
+      int s = 0;
+      for (int c = 0; c < 8; c++) s += hist[hist_off(l + 1) + p * 8 + c];
+      hist[hist_off(l) + p] = s;
+      mx = max(mx, s);
+    }
+    red[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + o]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) lmax[l] = red[0];
+    __syncthreads();
   }
 }
 
@@ -314,7 +363,7 @@ void tree_build(Ctx &c) {
 
   hipLaunchKernelGGL(k_bounds_part, dim3(256), dim3(256), 0, s, c.bodies, n, t.bounds_part.p, t.misc.p);
   XF_LAUNCH();
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, s, t.bounds_part.p, 256, root);
+  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, t.bounds_part.p, 256, root);
   XF_LAUNCH();
 
   t.key.ensure(n);
@@ -322,10 +371,13 @@ void tree_build(Ctx &c) {
   t.val.ensure(n);
   t.val2.ensure(n);
   t.hist.ensure(kHistTotal);
-  XF_HIP(hipMemsetAsync(t.hist.p, 0, sizeof(int) * kHistTotal, s));
-  hipLaunchKernelGGL(k_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, c.bodies, n, root, t.key.p, t.val.p, t.hist.p);
+  XF_HIP(hipMemsetAsync(t.hist.p + hist_off(kHistLevels), 0, sizeof(int) * (1 << (3 * kHistLevels)), s));
+  hipLaunchKernelGGL(k_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, c.bodies, n, root, t.key.p, t.val.p,
+                     t.hist.p + hist_off(kHistLevels));
   XF_LAUNCH();
-  hipLaunchKernelGGL(k_hist_max, dim3(512), dim3(256), 0, s, t.hist.p, t.misc.p);
+  hipLaunchKernelGGL(k_hist_up, dim3((1 << 15) / 256), dim3(256), 0, s, t.hist.p, t.misc.p);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_hist_top, dim3(1), dim3(1024), 0, s, t.hist.p, t.misc.p);
   XF_LAUNCH();
 
   // sync 1: bounds, input check, depth needed

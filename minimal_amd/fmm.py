@@ -38,6 +38,7 @@ EXPORTS = [
     "exafmm_horizontal", "exafmm_downward", "exafmm_evaluate", "exafmm_direct", "exafmm_list_sizes",
     "exafmm_get_lists", "exafmm_get_multipoles", "exafmm_get_locals", "exafmm_stage_times", "exafmm_p2p",
     "exafmm_p2m", "exafmm_m2m", "exafmm_m2l", "exafmm_l2l", "exafmm_l2p", "exafmm_work_counts",
+    "exafmm_dipole_correction", "exafmm_ewald",
 ]
 
 
@@ -85,6 +86,8 @@ class Library:
         L.exafmm_get_locals.argtypes = [_P, _P]
         L.exafmm_stage_times.argtypes = [_P, _P, _I]
         L.exafmm_work_counts.argtypes = [_P, _P, _I]
+        L.exafmm_dipole_correction.argtypes = [_P]
+        L.exafmm_ewald.argtypes = [_P, _P, _L, _P, _L, _I, _D, _D, _D, _I]
         L.exafmm_p2p.argtypes = [_P, _P, _L, _P, _L]
         L.exafmm_p2m.argtypes = [_P, _P, _P, _L, _P]
         L.exafmm_m2m.argtypes = [_P, _P, _P, _P, _P]
@@ -178,6 +181,15 @@ class FMM:
         ns = len(sources) if ns is None else ns
         self._chk(self.L.lib.exafmm_direct(self.h, _ptr(targets), nt, _ptr(sources), ns, int(on_device)))
 
+    def dipole_correction(self):
+        self._chk(self.L.lib.exafmm_dipole_correction(self.h))
+
+    def ewald(self, targets, sources, ksize, alpha, sigma, cutoff, nt=None, ns=None, on_device=False):
+        nt = len(targets) if nt is None else nt
+        ns = len(sources) if ns is None else ns
+        self._chk(self.L.lib.exafmm_ewald(self.h, _ptr(targets), nt, _ptr(sources), ns, ksize, alpha, sigma,
+                                          cutoff, int(on_device)))
+
     def synchronize(self):
         self._chk(self.L.lib.exafmm_synchronize(self.h))
 
@@ -220,7 +232,7 @@ class FMM:
     def stage_times(self) -> dict:
         v = np.zeros(8)
         self._chk(self.L.lib.exafmm_stage_times(self.h, _ptr(v), 8))
-        keys = ["tree", "lists", "upward", "m2l", "p2p", "downward", "total"]
+        keys = ["tree", "lists", "upward", "m2l", "p2p", "downward", "total", "m2l_kernel"]
         return {k: float(x) for k, x in zip(keys, v)}
 
     def work_counts(self) -> dict:

@@ -1,0 +1,45 @@
+"""The reference's C++ surface (include/exafmm_mi355x.hpp): a driver written
+with the reference's names compiles with g++, links the HIP library, and
+reports errors as exceptions rather than aborting."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+SRC = os.path.join(ROOT, "tests", "cpp", "dropin_driver.cxx")
+
+
+def _build(tmp_path):
+    exe = os.path.join(str(tmp_path), "dropin")
+    lib = os.path.join(ROOT, "minimal_amd")
+    cmd = ["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"), SRC, "-o", exe, "-L" + lib,
+           "-lexafmm_mi355x", "-Wl,-rpath," + lib]
+    subprocess.run(cmd, check=True, capture_output=True)
+    return exe
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ missing")
+def test_dropin_compiles_and_fails_cleanly_without_device(tmp_path):
+    from minimal_amd import fmm
+    exe = _build(tmp_path)
+    if fmm.load().device_count() > 0:
+        pytest.skip("device present: covered by the gpu test")
+    r = subprocess.run([exe, "1000"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3
+    assert "no HIP device" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(shutil.which("g++") is None, reason="g++ missing")
+def test_dropin_driver_on_gpu(tmp_path, gpu):
+    exe = _build(tmp_path)
+    r = subprocess.run([exe, "10000"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    parts = r.stdout.split()
+    err_p = float(parts[parts.index("err_p") + 1])
+    err_F = float(parts[parts.index("err_F") + 1])
+    # full truncation, FP32 P2P, BF16x3 M2L on the driver set of 3d/fmm.cxx
+    assert err_p < 1e-5 and err_F < 1e-4

@@ -153,3 +153,74 @@ def test_errors_are_status_codes(gpu):
     with pytest.raises(fmm.ExaFMMError) as e:
         g.build_tree(c)
     assert e.value.code == -3
+
+
+@pytest.mark.parametrize("name", TREES)
+@pytest.mark.parametrize("prec,bound", [(fmm.PREC_FP32, 2e-5), (fmm.PREC_BF16X3, 5e-4)])
+def test_keyed_m2l_locals(gpu, name, prec, bound):
+    """Keyed MFMA M2L (radius-scaled operators, tiles of 64 targets) against
+    the FP64 per-pair path on the same tree, full truncation.  Compared in
+    the scaled form L_j^k R^(j+1) that sets each term's size inside the cell
+    (the field at |x - Xc| <= sqrt(3) R): per cell, max |dL~| / max |L~|.
+    This pins the operator (a wrong index, sign or scale gives an O(1)
+    error); the bounds are FP32 / BF16x3 rounding under the cancellation of
+    signed charges.  Precision of the whole method is asserted end to end
+    against the FP64 oracle below."""
+    g = golden(name)
+    P = int(g["P"])
+    out = {}
+    for p in (fmm.PREC_FP64, prec):
+        f = _ctx(g, m2l_precision=p, m2l_truncation=fmm.M2L_FULL)
+        b = g["input"].copy()
+        f.build_tree(b)
+        f.upward()
+        f.horizontal()
+        out[p] = f.locals()
+        R = f.cells()["R"]
+    deg = np.array([j for j in range(P) for _ in range(j + 1)])
+    sc = R[:, None] ** (deg[None, :] + 1)
+    ref, got = out[fmm.PREC_FP64] * sc, out[prec] * sc
+    scale = np.max(np.abs(ref), axis=1, keepdims=True) + 1e-300
+    assert np.max(np.abs(got - ref) / scale) <= bound
+
+
+@pytest.mark.parametrize("prec,bound", [(fmm.PREC_FP32, 3e-6), (fmm.PREC_BF16X3, 5e-6)])
+def test_fast_path_vs_oracle_full(gpu, orc, prec, bound):
+    """Whole evaluation (FP32 P2P, keyed M2L) against the FP64 oracle in full
+    truncation, config-1 body set: relative L2 of p and F."""
+    g = golden("config1_n10000.npz")
+    ref = g["input"].copy()
+    orc.fmm(ref, 10, 64, 0.4, full=True)
+    f = _ctx(g, m2l_precision=prec, m2l_truncation=fmm.M2L_FULL)
+    b = g["input"].copy()
+    f.evaluate(b)
+    assert rel_l2(b["p"], ref["p"]) <= bound
+    assert rel_l2(b["F"], ref["F"]) <= bound
+
+
+@pytest.mark.parametrize("name", ["uniform_n1000_p4.npz", "plummer_n4096_p6.npz"])
+@pytest.mark.parametrize("prec", [fmm.PREC_FP32, fmm.PREC_BF16X3])
+def test_fast_path_vs_oracle_signed(gpu, orc, name, prec):
+    """Signed charges, uniform and clustered sets: the keyed M2L + FP32 P2P
+    evaluation stays within 3e-6 (FP32 M2L) / 1e-5 (BF16x3 M2L) relative L2
+    of the FP64 oracle run in the same truncation."""
+    g = golden(name)
+    P, nc, th = int(g["P"]), int(g["ncrit"]), float(g["theta"])
+    ref = g["input"].copy()
+    orc.fmm(ref, P, nc, th, full=True)
+    f = _ctx(g, m2l_precision=prec, m2l_truncation=fmm.M2L_FULL)
+    b = g["input"].copy()
+    f.evaluate(b)
+    bound = 3e-6 if prec == fmm.PREC_FP32 else 1e-5
+    assert rel_l2(b["p"], ref["p"]) <= bound
+    assert rel_l2(b["F"], ref["F"]) <= bound
+
+
+def test_fast_path_reference_truncation(gpu):
+    """Keyed M2L in the reference's truncation reproduces the reference p, F."""
+    g = golden("config1_n10000.npz")
+    f = _ctx(g, m2l_precision=fmm.PREC_FP32, m2l_truncation=fmm.M2L_REFERENCE)
+    b = g["input"].copy()
+    f.evaluate(b)
+    assert rel_l2(b["p"], g["p"]) <= 3e-6
+    assert rel_l2(b["F"], g["F"]) <= 3e-6
