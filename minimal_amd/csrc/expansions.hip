@@ -48,6 +48,12 @@ __global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m(int P, const int *__re
   const double4 C = cx[c];
   const double invR = 1.0 / C.w;
   const int m = lane & 15, g = lane >> 4;
+  // per-lane constants (no divisions per body): 1/(2^m m!) and the
+  // recurrence factors -1/((n+m)(n-m)), n = m+2..P-1
+  double cm = 1.0, sn[PMAX];
+  for (int k = 1; k <= m; k++) cm /= double(2 * k);
+#pragma unroll
+  for (int n = 0; n < PMAX; n++) sn[n] = (n >= m + 2 && n < P) ? -1.0 / double((n + m) * (n - m)) : 0.0;
   double ar[PMAX], ai[PMAX];
 #pragma unroll
   for (int n = 0; n < PMAX; n++) ar[n] = ai[n] = 0;
@@ -62,14 +68,14 @@ __global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m(int P, const int *__re
         const double4 B = sb[w][bi];
         const double x = (B.x - C.x) * invR, y = -(B.y - C.y) * invR, z = (B.z - C.z) * invR;  // conjugate
         const double r2 = x * x + y * y + z * z;
-        double yr = B.w, yi = 0;  // q * Y_m^m
+        double yr = 1.0, yi = 0;  // (x + iy)^m
         for (int k = 1; k <= m; k++) {
-          const double inv = 1.0 / (2 * k);
-          const double tr = (yr * x - yi * y) * inv, ti = (yr * y + yi * x) * inv;
+          const double tr = yr * x - yi * y, ti = yr * y + yi * x;
           yr = tr;
           yi = ti;
         }
-        double pr = 0, pi = 0, cr = yr, ci = yi;
+        const double qc = B.w * cm;  // q * Y_m^m = q (x+iy)^m / (2^m m!)
+        double pr = 0, pi = 0, cr = yr * qc, ci = yi * qc;
 #pragma unroll
         for (int n = 0; n < PMAX; n++) {
           if (n < m || n >= P) continue;
@@ -79,10 +85,9 @@ __global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m(int P, const int *__re
               nr = -z * cr;
               ni = -z * ci;
             } else {
-              const double s = -1.0 / double((n + m) * (n - m));
               const double cz = double(2 * n - 1) * z;
-              nr = (cz * cr + r2 * pr) * s;
-              ni = (cz * ci + r2 * pi) * s;
+              nr = (cz * cr + r2 * pr) * sn[n];
+              ni = (cz * ci + r2 * pi) * sn[n];
             }
             pr = cr;
             pi = ci;
@@ -158,10 +163,6 @@ __global__ void k_shift_ops(int P, double *__restrict__ T) {
   }
 }
 
-// Rows per workgroup and staged columns of the M2M / L2L GEMMs.
-constexpr int kShiftRows = 16;
-constexpr int kShiftCols = 16;
-
 void shift_ops(Ctx &c) {
   if (c.shift_P == c.P) return;
   const int PP = c.P * c.P;
@@ -172,21 +173,18 @@ void shift_ops(Ctx &c) {
 }
 
 // ---------------------------------------------------------------- M2M
-// M2M and L2L as per-octant GEMMs over the cells of one level.  A workgroup
-// is 4 waves over the same 64 parents (lane = parent) and one block of
-// kShiftRows operator rows (blockIdx.y); wave w takes octants w and w+4.
-// Each wave walks its (octant, column chunk) steps on its own: the values
-// of the next step are loaded into registers while the current one, staged
-// in the wave's LDS slice (expansions X[col][lane], operator Tw[col][row]
-// read back as broadcasts), is multiplied, so every step exposes at most
-// one memory latency.  Degree structure: an M2M row of degree j reads
-// columns of degree <= j, an L2L row of degree j reads columns of degree
-// >= j, so a row block stops (starts) at the column bound of its last
-// (first) row.
-constexpr int kShiftWaves = 4;
-constexpr int kShiftPad = kWave + 1;
-constexpr int kXPer = kShiftCols * kWave / kWave;           // staged expansion values per lane
-constexpr int kTPer = kShiftCols * kShiftRows / kWave;      // staged operator values per lane
+// M2M and L2L as FP64 MFMA GEMMs over the cells of one level
+// (v_mfma_f64_16x16x4_f64: A lane l = row l%16, k l/16; B lane l = column
+// l%16, k l/16; D register i of lane l = row l/16 + 4i, column l%16).  A wave owns one
+// 16-row tile of the operator and 16 parents; its operands stream straight
+// from L2 into registers (no LDS round trip: every loaded value feeds a
+// 16x16x4 product), four k-steps per batch so the loads of a batch overlap.
+// Degree structure: an M2M row of degree j reads columns of degree <= j, an
+// L2L row of degree j reads columns of degree >= j, so a row tile stops
+// (starts) at the column bound of its last (first) row.
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kShiftWaves = 4;  // row tiles per workgroup
+constexpr int kShiftBatch = 4;  // k-steps of 4 columns loaded together
 
 __device__ inline int degree_of(int row) {
   int j = 0;
@@ -194,199 +192,115 @@ __device__ inline int degree_of(int row) {
   return j;
 }
 
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// chidx[o][lane] = child of the lane's parent in octant o, or -1; the
-// octant is recomputed from the centres (strict >, 3d/build_tree.h:50-56).
-__device__ inline void child_slots(int p, bool valid, const int *__restrict__ cchild, const int *__restrict__ cnch,
-                                   const double4 *__restrict__ cx, int (*chidx)[kWave], int lane, int w) {
-  if (w != 0) return;
-  int slot[8];
-#pragma unroll
-  for (int o = 0; o < 8; o++) slot[o] = -1;
-  if (valid) {
-    const int nch = cnch[p], first = cchild[p];
-    const double4 Cp = cx[p];
-    for (int ch = first; ch < first + nch; ch++) {
-      const double4 Cc = cx[ch];
-      const int o = (Cc.x > Cp.x) + ((Cc.y > Cp.y) << 1) + ((Cc.z > Cp.z) << 2);
-#pragma unroll
-      for (int q = 0; q < 8; q++)
-        if (q == o) slot[q] = ch;
-    }
+// Child of parent p in octant o, or -1 (the octant is recomputed from the
+// centres, strict >, 3d/build_tree.h:50-56).
+__device__ inline int child_in(int p, int o, const int *__restrict__ cchild, const int *__restrict__ cnch,
+                               const double4 *__restrict__ cx) {
+  const int nch = cnch[p], first = cchild[p];
+  const double4 Cp = cx[p];
+  for (int ch = first; ch < first + nch; ch++) {
+    const double4 Cc = cx[ch];
+    if ((Cc.x > Cp.x) + ((Cc.y > Cp.y) << 1) + ((Cc.z > Cp.z) << 2) == o) return ch;
   }
-#pragma unroll
-  for (int o = 0; o < 8; o++) chidx[o][lane] = slot[o];
-}
-
-// Register stage of one step: expansion values (cell src[e / kShiftCols],
-// column c0 + e % kShiftCols for e = i * 64 + lane) and operator values
-// To[c0 + e / kShiftRows][r0 + e % kShiftRows].
-struct ShiftRegs {
-  double x[kXPer];
-  double t[kTPer];
-};
-
-__device__ inline void load_step(ShiftRegs &g, const double *__restrict__ E, int PP, const int *src, int c0,
-                                 int ncol, const double *__restrict__ To, int r0, int lane) {
-#pragma unroll
-  for (int i = 0; i < kXPer; i++) {
-    const int e = i * kWave + lane;
-    const int cell = src[e / kShiftCols];
-    const int col = c0 + e % kShiftCols;
-    g.x[i] = (cell >= 0 && col < ncol) ? E[int64_t(cell) * PP + col] : 0.0;
-  }
-#pragma unroll
-  for (int i = 0; i < kTPer; i++) {
-    const int e = i * kWave + lane;
-    const int col = c0 + e / kShiftRows, r = r0 + e % kShiftRows;
-    g.t[i] = (col < ncol && r < PP) ? To[int64_t(col) * PP + r] : 0.0;
-  }
-}
-
-__device__ inline void store_step(const ShiftRegs &g, double (*X)[kShiftPad], double (*Tw)[kShiftRows], int lane) {
-#pragma unroll
-  for (int i = 0; i < kXPer; i++) {
-    const int e = i * kWave + lane;
-    X[e % kShiftCols][e / kShiftCols] = g.x[i];
-  }
-#pragma unroll
-  for (int i = 0; i < kTPer; i++) {
-    const int e = i * kWave + lane;
-    Tw[e / kShiftRows][e % kShiftRows] = g.t[i];
-  }
-}
-
-__device__ inline void shift_fma(const double (*X)[kShiftPad], const double (*Tw)[kShiftRows], int lane,
-                                 double *acc) {
-#pragma unroll 4
-  for (int cc = 0; cc < kShiftCols; cc++) {
-    const double x = X[cc][lane];
-#pragma unroll
-    for (int r = 0; r < kShiftRows; r++) acc[r] = fma(Tw[cc][r], x, acc[r]);
-  }
+  return -1;
 }
 
 // M~_parent = sum over children of T_M2M[octant] M~_child (3d/kernel.h:206).
+// grid: (parent tiles of 16, row blocks of 64); K runs over (octant, column).
 __global__ void __launch_bounds__(kShiftWaves *kWave) k_m2m(int P, int lstart, int lcount,
                                                             const int *__restrict__ cchild,
                                                             const int *__restrict__ cnch,
                                                             const double4 *__restrict__ cx,
                                                             const double *__restrict__ T, double *__restrict__ M) {
-  __shared__ int chidx[8][kWave];
-  __shared__ double X[kShiftWaves][kShiftRows > kShiftCols ? kShiftRows : kShiftCols][kShiftPad];
-  __shared__ double Tw[kShiftWaves][kShiftCols][kShiftRows];
+  __shared__ int chl[8][16];
   const int PP = P * P;
   const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  const int pl = blockIdx.x * kWave + lane;
-  const bool valid = pl < lcount;
-  const int p = lstart + pl;
-  child_slots(p, valid, cchild, cnch, cx, chidx, lane, w);
-  const int r0 = blockIdx.y * kShiftRows;
-  const int jmax = degree_of(min(r0 + kShiftRows, PP) - 1);
-  const int ncol = (jmax + 1) * (jmax + 1);
-  const int nchunk = (ncol + kShiftCols - 1) / kShiftCols;
-  const int nstep = 2 * nchunk;
-  double acc[kShiftRows];
-#pragma unroll
-  for (int r = 0; r < kShiftRows; r++) acc[r] = 0.0;
-  __syncthreads();
-  ShiftRegs g;
-  load_step(g, M, PP, chidx[w], 0, ncol, T + int64_t(w) * PP * PP, r0, lane);
-  for (int st = 0; st < nstep; st++) {
-    store_step(g, X[w], Tw[w], lane);
-    wave_lds_sync();
-    if (st + 1 < nstep) {
-      const int o = w + 4 * ((st + 1) / nchunk), c0 = ((st + 1) % nchunk) * kShiftCols;
-      load_step(g, M, PP, chidx[o], c0, ncol, T + int64_t(o) * PP * PP, r0, lane);
-    }
-    shift_fma(X[w], Tw[w], lane, acc);
-    wave_lds_sync();
+  const int j16 = lane & 15, kq = lane >> 4;
+  const int p0 = blockIdx.x * 16;
+  if (threadIdx.x < 128) {
+    const int o = threadIdx.x / 16, q = threadIdx.x % 16;
+    chl[o][q] = p0 + q < lcount ? child_in(lstart + p0 + q, o, cchild, cnch, cx) : -1;
   }
-  // sum the four waves' partial rows
   __syncthreads();
+  const int r0 = (blockIdx.y * kShiftWaves + w) * 16;
+  if (r0 >= PP) return;
+  const int jmax = degree_of(min(r0 + 16, PP) - 1);
+  const int ncol = (jmax + 1) * (jmax + 1);
+  const int ksteps = (ncol + 3) / 4;
+  const int row = min(r0 + j16, PP - 1);  // rows past PP compute garbage that is never stored
+  v4d acc = {0, 0, 0, 0};
+  for (int o = 0; o < 8; o++) {
+    const int ch = chl[o][j16];
+    const double *To = T + int64_t(o) * PP * PP;
+    const double *Mc = M + int64_t(ch < 0 ? 0 : ch) * PP;
+    for (int k0 = 0; k0 < ksteps; k0 += kShiftBatch) {
+      double a[kShiftBatch], b[kShiftBatch];
 #pragma unroll
-  for (int r = 0; r < kShiftRows; r++) X[w][r][lane] = acc[r];
-  __syncthreads();
-  if (!valid || cnch[p] == 0) return;
-  for (int r = w; r < kShiftRows; r += kShiftWaves) {
-    if (r0 + r >= PP) break;
-    const double v = ((X[0][r][lane] + X[1][r][lane]) + X[2][r][lane]) + X[3][r][lane];
-    M[int64_t(p) * PP + r0 + r] = v;
+      for (int s = 0; s < kShiftBatch; s++) {
+        const int col = (k0 + s) * 4 + kq;
+        const bool in = col < ncol;
+        a[s] = in ? To[int64_t(col) * PP + row] : 0.0;
+        b[s] = (in && ch >= 0) ? Mc[col] : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < kShiftBatch; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+    }
+  }
+  const int pl = p0 + j16;
+  if (pl >= lcount) return;
+  const int p = lstart + pl;
+  if (cnch[p] == 0) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int r = r0 + kq + 4 * i;
+    if (r < PP) M[int64_t(p) * PP + r] = acc[i];
   }
 }
 
 // L~_child += T_L2L[octant] L~_parent for the children of one level of
-// parents (3d/kernel.h:265-280).  Wave w produces the children in octants
-// w and w+4 from the same staged parent columns.
+// parents (3d/kernel.h:265-280).  grid: (parent tiles of 16, row blocks of
+// 64, octant).
 __global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l(int P, int lstart, int lcount,
                                                             const int *__restrict__ cchild,
                                                             const int *__restrict__ cnch,
                                                             const double4 *__restrict__ cx,
                                                             const double *__restrict__ T, double *__restrict__ L) {
-  __shared__ int chidx[8][kWave];
-  __shared__ int par[kWave];
-  __shared__ double X[kShiftWaves][kShiftCols][kShiftPad];
-  __shared__ double Tw[kShiftWaves][2][kShiftCols][kShiftRows];
+  __shared__ int chl[16];
   const int PP = P * P;
   const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  const int pl = blockIdx.x * kWave + lane;
-  const bool valid = pl < lcount;
-  const int p = lstart + pl;
-  child_slots(p, valid, cchild, cnch, cx, chidx, lane, w);
-  if (w == 0) par[lane] = (valid && cnch[p] > 0) ? p : -1;
-  const int r0 = blockIdx.y * kShiftRows;
-  const int cfirst = degree_of(r0) * degree_of(r0);
-  const int nstep = (PP - cfirst + kShiftCols - 1) / kShiftCols;
-  const double *T0 = T + (int64_t(8) + w) * PP * PP, *T1 = T + (int64_t(12) + w) * PP * PP;
-  double acc0[kShiftRows], acc1[kShiftRows];
-#pragma unroll
-  for (int r = 0; r < kShiftRows; r++) acc0[r] = acc1[r] = 0.0;
+  const int j16 = lane & 15, kq = lane >> 4;
+  const int p0 = blockIdx.x * 16, o = blockIdx.z;
+  if (threadIdx.x < 16)
+    chl[threadIdx.x] = p0 + threadIdx.x < lcount ? child_in(lstart + p0 + threadIdx.x, o, cchild, cnch, cx) : -1;
   __syncthreads();
-  ShiftRegs g0, g1;
-  load_step(g0, L, PP, par, cfirst, PP, T0, r0, lane);
+  const int r0 = (blockIdx.y * kShiftWaves + w) * 16;
+  if (r0 >= PP) return;
+  const int jmin = degree_of(r0);
+  const int c0 = jmin * jmin;
+  const int ksteps = (PP - c0 + 3) / 4;
+  const int row = min(r0 + j16, PP - 1);
+  const int ch = chl[j16];
+  const double *To = T + (int64_t(8) + o) * PP * PP;
+  const int pl = min(p0 + j16, lcount - 1);
+  const double *Lp = L + int64_t(lstart + pl) * PP;
+  v4d acc = {0, 0, 0, 0};
+  for (int k0 = 0; k0 < ksteps; k0 += kShiftBatch) {
+    double a[kShiftBatch], b[kShiftBatch];
 #pragma unroll
-  for (int i = 0; i < kTPer; i++) {
-    const int e = i * kWave + lane;
-    const int col = cfirst + e / kShiftRows, r = r0 + e % kShiftRows;
-    g1.t[i] = (col < PP && r < PP) ? T1[int64_t(col) * PP + r] : 0.0;
-  }
-  for (int st = 0; st < nstep; st++) {
-    store_step(g0, X[w], Tw[w][0], lane);
-#pragma unroll
-    for (int i = 0; i < kTPer; i++) {
-      const int e = i * kWave + lane;
-      Tw[w][1][e / kShiftRows][e % kShiftRows] = g1.t[i];
+    for (int s = 0; s < kShiftBatch; s++) {
+      const int col = c0 + (k0 + s) * 4 + kq;
+      const bool in = col < PP;
+      a[s] = in ? To[int64_t(col) * PP + row] : 0.0;
+      b[s] = (in && ch >= 0) ? Lp[col] : 0.0;
     }
-    wave_lds_sync();
-    if (st + 1 < nstep) {
-      const int c0 = cfirst + (st + 1) * kShiftCols;
-      load_step(g0, L, PP, par, c0, PP, T0, r0, lane);
 #pragma unroll
-      for (int i = 0; i < kTPer; i++) {
-        const int e = i * kWave + lane;
-        const int col = c0 + e / kShiftRows, r = r0 + e % kShiftRows;
-        g1.t[i] = (col < PP && r < PP) ? T1[int64_t(col) * PP + r] : 0.0;
-      }
-    }
-    shift_fma(X[w], Tw[w][0], lane, acc0);
-    shift_fma(X[w], Tw[w][1], lane, acc1);
-    wave_lds_sync();
+    for (int s = 0; s < kShiftBatch; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
   }
-  const int ch0 = chidx[w][lane], ch1 = chidx[w + 4][lane];
-  if (ch0 >= 0) {
+  if (ch < 0) return;
 #pragma unroll
-    for (int r = 0; r < kShiftRows; r++)
-      if (r0 + r < PP) L[int64_t(ch0) * PP + r0 + r] += acc0[r];
-  }
-  if (ch1 >= 0) {
-#pragma unroll
-    for (int r = 0; r < kShiftRows; r++)
-      if (r0 + r < PP) L[int64_t(ch1) * PP + r0 + r] += acc1[r];
+  for (int i = 0; i < 4; i++) {
+    const int r = r0 + kq + 4 * i;
+    if (r < PP) L[int64_t(ch) * PP + r] += acc[i];
   }
 }
 
@@ -403,6 +317,7 @@ __global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leave
                                             const double4 *__restrict__ cx, const double *__restrict__ L,
                                             const double4 *__restrict__ xs, exafmm_body *__restrict__ bodies) {
   __shared__ cpx<T> Lh[MAXP * (MAXP + 1) / 2];
+  __shared__ T rec[MAXP * (MAXP + 1) / 2];  // 1/(2m) at (m, m), -1/((n+m)(n-m)) at (n, m), n >= m + 2
   const int li = blockIdx.x;
   if (li >= *nleaves) return;
   const int c = leaves[li];
@@ -415,6 +330,7 @@ __global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leave
     jk_of(e, n, m);
     const double w = (m == 0 ? 1.0 : 2.0) * invR;  // fold w_m and L~ -> L R^n
     Lh[e] = {T(Lc[rf(n, m, 0)] * w), T(m ? Lc[rf(n, m, 1)] * w : 0.0)};
+    rec[e] = n == m ? T(m ? 1.0 / (2 * m) : 0.0) : (n >= m + 2 ? T(-1.0 / double((n + m) * (n - m))) : T(0));
   }
   __syncthreads();
   const int beg = cbeg[c], cnt = ccnt[c];
@@ -426,7 +342,7 @@ __global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leave
     T ymr = 1, ymi = 0;  // Y_m^m
     for (int m = 0; m < P; m++) {
       if (m > 0) {
-        const T inv = T(1) / T(2 * m);
+        const T inv = rec[nms(m, m)];
         const T tr = (ymr * x - ymi * y) * inv, ti = (ymr * y + ymi * x) * inv;
         ymr = tr;
         ymi = ti;
@@ -439,7 +355,7 @@ __global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leave
             nr = -z * cr;
             ni = -z * ci;
           } else {
-            const T s = T(-1) / T((n + m) * (n - m));
+            const T s = rec[nms(n, m)];
             const T cz = T(2 * n - 1) * z;
             nr = (cz * cr + r2 * pr) * s;
             ni = (cz * ci + r2 * pi) * s;
@@ -491,8 +407,8 @@ void upward(Ctx &c) {
   XF_LAUNCH();
   for (int l = t.depth - 1; l >= 0; l--) {
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_m2m, dim3(ceil_div(cnt, kWave), ceil_div(PP, kShiftRows)), dim3(kShiftWaves * kWave), 0, s, c.P,
-                       t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
+    hipLaunchKernelGGL(k_m2m, dim3(ceil_div(cnt, 16), ceil_div(PP, 16 * kShiftWaves)), dim3(kShiftWaves * kWave), 0,
+                       s, c.P, t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
     XF_LAUNCH();
   }
   c.have_M = true;
@@ -505,8 +421,8 @@ void l2l_pass(Ctx &c) {
   shift_ops(c);
   for (int l = 0; l < t.depth; l++) {  // parents at level l
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_l2l, dim3(ceil_div(cnt, kWave), ceil_div(PP, kShiftRows)), dim3(kShiftWaves * kWave), 0, s, c.P,
-                       t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.L.p);
+    hipLaunchKernelGGL(k_l2l, dim3(ceil_div(cnt, 16), ceil_div(PP, 16 * kShiftWaves), 8), dim3(kShiftWaves * kWave),
+                       0, s, c.P, t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.L.p);
     XF_LAUNCH();
   }
 }

@@ -79,24 +79,34 @@ __global__ void __launch_bounds__(256) k_group_hist(int ncells, const unsigned l
   if (h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
 
-// Entries of a tile's targets flattened: offsets of each column's list in
-// LDS, entry e -> (column, index) by binary search, so all threads work on
-// the tile at once.
-__device__ inline int tile_offsets(const int *cells, int ncol, const int *cnt, int *off) {
-  // exclusive scan of the columns' list lengths (blockDim.x >= kTN)
-  const int t = threadIdx.x;
-  const int v = t < ncol ? cnt[cells[t]] : 0;
-  if (t < kTN) off[t + 1] = v;
-  if (t == 0) off[0] = 0;
+// Entries of a tile's targets flattened: each column's list start and the
+// exclusive prefix of the list lengths are staged in LDS, entry e ->
+// (column, index) by binary search, so all threads work on the tile at once.
+// M2L entries hold (source << 5 | image, key id).
+struct TileCols {
+  int off[kTN + 1];
+  int beg[kTN];
+};
+
+__device__ inline int tile_offsets(const int *cells, int ncol, const int *beg, const int *cnt, TileCols &tc) {
+  const int t = threadIdx.x;  // blockDim.x >= kTN
+  int v = 0;
+  if (t < ncol) {
+    const int c = cells[t];
+    v = cnt[c];
+    tc.beg[t] = beg[c];
+  }
+  if (t < kTN) tc.off[t + 1] = v;
+  if (t == 0) tc.off[0] = 0;
   __syncthreads();
   for (int o = 1; o < kTN; o <<= 1) {
     int a = 0;
-    if (t < kTN && t >= o) a = off[t + 1 - o];
+    if (t < kTN && t >= o) a = tc.off[t + 1 - o];
     __syncthreads();
-    if (t < kTN) off[t + 1] += a;
+    if (t < kTN) tc.off[t + 1] += a;
     __syncthreads();
   }
-  return off[ncol];
+  return tc.off[ncol];
 }
 __device__ inline int col_of(const int *off, int ncol, int e) {
   int lo = 0, hi = ncol;  // last t with off[t] <= e
@@ -110,15 +120,27 @@ __device__ inline int col_of(const int *off, int ncol, int e) {
   return lo;
 }
 
+constexpr int kTileUnroll = 4;  // entries in flight per thread
+
 // Distinct keys of one tile: bitmap over key ids in LDS.
-__device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, const int *off, int total,
-                                   const int2 *list, const int *slot2id, unsigned *bm, int nwords) {
+__device__ inline void tile_bitmap(int ncol, const TileCols &tc, int total, const int2 *list, unsigned *bm,
+                                   int nwords) {
   for (int i = threadIdx.x; i < nwords; i += blockDim.x) bm[i] = 0;
   __syncthreads();
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int t = col_of(off, ncol, e);
-    const int id = slot2id[list[beg[cells[t]] + e - off[t]].y];
-    atomicOr(&bm[id >> 5], 1u << (id & 31));
+  for (int e0 = threadIdx.x; e0 < total; e0 += blockDim.x * kTileUnroll) {
+    int id[kTileUnroll];
+#pragma unroll
+    for (int k = 0; k < kTileUnroll; k++) {
+      const int e = e0 + k * blockDim.x;
+      id[k] = -1;
+      if (e < total) {
+        const int t = col_of(tc.off, ncol, e);
+        id[k] = list[tc.beg[t] + e - tc.off[t]].y;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kTileUnroll; k++)
+      if (id[k] >= 0) atomicOr(&bm[id[k] >> 5], 1u << (id[k] & 31));
   }
   __syncthreads();
 }
@@ -126,16 +148,15 @@ __device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, c
 __global__ void __launch_bounds__(256) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                     const int *__restrict__ sorted, const int *__restrict__ beg,
                                                     const int *__restrict__ cnt, const int2 *__restrict__ list,
-                                                    const int *__restrict__ slot2id, int nkeys,
-                                                    int *__restrict__ tU) {
+                                                    int nkeys, int *__restrict__ tU) {
   extern __shared__ unsigned bm[];
-  __shared__ int off[kTN + 1];
+  __shared__ TileCols tc;
   __shared__ int tot;
   const int nwords = (nkeys + 31) / 32;
   const int *cells = sorted + tfirst[blockIdx.x];
   const int ncol = tcount[blockIdx.x];
-  const int total = tile_offsets(cells, ncol, cnt, off);
-  tile_bitmap(cells, ncol, beg, off, total, list, slot2id, bm, nwords);
+  const int total = tile_offsets(cells, ncol, beg, cnt, tc);
+  tile_bitmap(ncol, tc, total, list, bm, nwords);
   if (threadIdx.x == 0) tot = 0;
   __syncthreads();
   int s = 0;
@@ -145,21 +166,20 @@ __global__ void __launch_bounds__(256) k_tile_count(const int *__restrict__ tfir
   if (threadIdx.x == 0) tU[blockIdx.x] = tot;
 }
 
-// Key list (ascending) and source matrix [U][64] (-1 = no pair) of a tile.
+// Key list (ascending) and source matrix [U][kTN] (-1 = no pair) of a tile.
 __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                    const int *__restrict__ sorted, const int *__restrict__ beg,
                                                    const int *__restrict__ cnt, const int2 *__restrict__ list,
-                                                   const int *__restrict__ slot2id, int nkeys,
-                                                   const int *__restrict__ tbase, int *__restrict__ tkeys,
-                                                   int *__restrict__ tmat) {
+                                                   int nkeys, const int *__restrict__ tbase,
+                                                   int *__restrict__ tkeys, int *__restrict__ tmat) {
   extern __shared__ unsigned bm[];
-  __shared__ int off[kTN + 1];
+  __shared__ TileCols tc;
   const int nwords = (nkeys + 31) / 32;
   unsigned *rank = bm + nwords;  // exclusive popcount prefix per word
   const int *cells = sorted + tfirst[blockIdx.x];
   const int ncol = tcount[blockIdx.x];
-  const int total = tile_offsets(cells, ncol, cnt, off);
-  tile_bitmap(cells, ncol, beg, off, total, list, slot2id, bm, nwords);
+  const int total = tile_offsets(cells, ncol, beg, cnt, tc);
+  tile_bitmap(ncol, tc, total, list, bm, nwords);
   if (threadIdx.x == 0) {
     unsigned r = 0;
     for (int i = 0; i < nwords; i++) {
@@ -181,15 +201,28 @@ __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirs
   int U = 0;
   if (nwords) U = rank[nwords - 1] + __popc(bm[nwords - 1]);
   for (int i = threadIdx.x; i < U * kTN; i += blockDim.x) tmat[int64_t(base) * kTN + i] = -1;
-  __threadfence();
   __syncthreads();
-  for (int e = threadIdx.x; e < total; e += blockDim.x) {
-    const int t = col_of(off, ncol, e);
-    const int2 en = list[beg[cells[t]] + e - off[t]];
-    const int id = slot2id[en.y];
-    const unsigned w = bm[id >> 5];
-    const int r = rank[id >> 5] + __popc(w & ((1u << (id & 31)) - 1u));
-    tmat[(int64_t(base) + r) * kTN + t] = en.x >> 5;  // source cell (the image is part of the key)
+  for (int e0 = threadIdx.x; e0 < total; e0 += blockDim.x * kTileUnroll) {
+    int2 en[kTileUnroll];
+    int col[kTileUnroll];
+#pragma unroll
+    for (int k = 0; k < kTileUnroll; k++) {
+      const int e = e0 + k * blockDim.x;
+      col[k] = -1;
+      if (e < total) {
+        const int t = col_of(tc.off, ncol, e);
+        col[k] = t;
+        en[k] = list[tc.beg[t] + e - tc.off[t]];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kTileUnroll; k++) {
+      if (col[k] < 0) continue;
+      const int id = en[k].y;
+      const unsigned w = bm[id >> 5];
+      const int r = rank[id >> 5] + __popc(w & ((1u << (id & 31)) - 1u));
+      tmat[(int64_t(base) + r) * kTN + col[k]] = en[k].x >> 5;  // source cell (the image is part of the key)
+    }
   }
 }
 
@@ -309,18 +342,19 @@ __global__ void __launch_bounds__(256) k_ops_gen(int P, int fillP, int R, int K4
 }
 
 // ---------------------------------------------------------------- GEMM
-// Epilogue: accumulator rows are scaled local coefficients L~, added to the
-// target's FP64 L~ (one writer per cell).
-// ---------------------------------------------------------------- GEMM
 // One workgroup = one tile of up to 256 targets and a block of RTB row
-// tiles (blockIdx.y); 8 waves = 4 column groups (64 targets, 4 column tiles
-// of 16) x 2 row halves (RH row tiles each).  Per key the block's operator
-// rows are copied once into LDS and shared by all 8 waves (256 targets per
-// operator load); each lane gathers its target's source multipoles from
-// global memory (L2-resident).  All keys of a tile accumulate into the same
+// tiles (blockIdx.y).  8 waves = 4 column groups (64 targets = 4 column
+// tiles of 16) x 2 k-parities: wave (g, kp) multiplies the k-steps kk = kp,
+// kp+2, ... of every key for its 64 targets and all RTB row tiles, so each
+// source multipole is fetched once per tile; the two parities are added
+// through LDS at the end.  Per key the block's operator rows are copied
+// global -> LDS (global_load_lds, no register round trip) into the second
+// of two buffers while the first is multiplied; one wait + barrier per key.
+// Each lane's source multipoles for its next k-step are loaded while the
+// current one is multiplied.  All keys of a tile accumulate into the same
 // MFMA accumulators, so each target's L is written once, without atomics.
 constexpr int kGemmThreads = 512;
-constexpr int kGemmLds = 65536;  // operator rows staged per key
+constexpr int kGemmMaxRT = 8;  // row tiles per wave (accumulator registers)
 
 template <int RH>
 __device__ inline void write_locals(int PP, int rbase, int nr, const int *cells, int ncol, int cg,
@@ -332,7 +366,7 @@ __device__ inline void write_locals(int PP, int rbase, int nr, const int *cells,
     double *Lc = L + int64_t(cells[col]) * PP;
 #pragma unroll
     for (int r = 0; r < RH; r++) {
-      if (r >= nr) break;  // row tiles of this wave only (another block owns the next ones)
+      if (r >= nr) break;  // row tiles of this block only (another block owns the next ones)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         const int row = (rbase + r) * 16 + 4 * (lane >> 4) + i;
@@ -340,6 +374,34 @@ __device__ inline void write_locals(int PP, int rbase, int nr, const int *cells,
       }
     }
   }
+}
+
+// Parity 1 hands its accumulators to parity 0 through LDS (the operator
+// buffers are free by then); parity 0 adds them and writes the locals.
+template <int RH>
+__device__ inline void reduce_parities(v4f (&acc)[RH][4], int kp, int cg, int lane, void *lds) {
+  v4f *red = reinterpret_cast<v4f *>(lds);
+  __syncthreads();
+  if (kp == 1) {
+#pragma unroll
+    for (int r = 0; r < RH; r++)
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) red[((cg * RH + r) * 4 + ct) * 64 + lane] = acc[r][ct];
+  }
+  __syncthreads();
+  if (kp == 0) {
+#pragma unroll
+    for (int r = 0; r < RH; r++)
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) acc[r][ct] += red[((cg * RH + r) * 4 + ct) * 64 + lane];
+  }
+}
+
+// Row of the full operator layout [kk][row tile][lane] for run c of a
+// block's staged copy (c = kk * nrt + local row tile).
+__device__ inline int op_run(int c, int nrt, int RT, int rt0) {
+  const int kk = c / nrt, rl = c % nrt;
+  return kk * RT + rt0 + rl;
 }
 
 template <int RH>
@@ -354,12 +416,11 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
                                                           const float *__restrict__ mt, double *__restrict__ L) {
   extern __shared__ float As[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int cg = w & 3, rh = w >> 2;
+  const int cg = w & 3, kp = w >> 2;
   const int tile = blockIdx.x;
   const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
   const int rt0 = blockIdx.y * RTB;
   const int nrt = min(RTB, RT - rt0);
-  const int rl0 = rh * RH;  // first local row tile of this wave
   const int kq = lane >> 4;
   v4f acc[RH][4];
 #pragma unroll
@@ -367,34 +428,57 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) acc[r][ct] = v4f{0, 0, 0, 0};
   const int64_t per = int64_t(K4) * RT * 64;
-  const int E4 = K4 * nrt * 16;  // float4 per staged operator block
-  for (int u = 0; u < U; u++) {
-    const int key = tkeys[base + u];
-    int src[4];
+  const int NC = K4 * nrt;  // runs of 64 floats per staged block
+  const int BUF = NC * 64;  // floats per LDS buffer
+  if (U > 0) {
+    auto issue = [&](int u, float *dst) {
+      const float *Ak = op32 + int64_t(tkeys[base + u]) * per;
+      for (int c = w; c < NC; c += kGemmThreads / 64)
+        __builtin_amdgcn_global_load_lds(Ak + int64_t(op_run(c, nrt, RT, rt0)) * 64 + lane, dst + c * 64, 4, 0, 0);
+    };
+    auto load_b = [&](const int (&sc)[4], int kk, float (&bv)[4]) {
 #pragma unroll
-    for (int ct = 0; ct < 4; ct++) src[ct] = tmat[(int64_t(base) + u) * kTN + cg * 64 + ct * 16 + (lane & 15)];
+      for (int ct = 0; ct < 4; ct++) bv[ct] = sc[ct] < 0 ? 0.0f : mt[int64_t(sc[ct]) * KP + kk * 4 + kq];
+    };
+    int src[4], nsrc[4];
+    float bcur[4], bnext[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) src[ct] = tmat[int64_t(base) * kTN + cg * 64 + ct * 16 + (lane & 15)];
+    if (kp < K4) load_b(src, kp, bnext);
+    issue(0, As);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    const float *Ak = op32 + key * per;
-    for (int e = threadIdx.x; e < E4; e += kGemmThreads) {
-      const int l4 = e % 16, rl = (e / 16) % nrt, kk = e / (16 * nrt);
-      reinterpret_cast<float4 *>(As)[e] =
-          *reinterpret_cast<const float4 *>(Ak + (int64_t(kk) * RT + rt0 + rl) * 64 + l4 * 4);
-    }
-    __syncthreads();
-    for (int kk = 0; kk < K4; kk++) {
-      float bv[4];
+    for (int u = 0; u < U; u++) {
+      const float *Acur = As + (u & 1) * BUF;
+      if (u + 1 < U) {
+        issue(u + 1, As + ((u + 1) & 1) * BUF);
 #pragma unroll
-      for (int ct = 0; ct < 4; ct++) bv[ct] = src[ct] < 0 ? 0.0f : mt[int64_t(src[ct]) * KP + kk * 4 + kq];
-#pragma unroll
-      for (int r = 0; r < RH; r++) {
-        if (rl0 + r >= nrt) break;
-        const float a = As[(kk * nrt + rl0 + r) * 64 + lane];
-#pragma unroll
-        for (int ct = 0; ct < 4; ct++) acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bv[ct], acc[r][ct], 0, 0, 0);
+        for (int ct = 0; ct < 4; ct++) nsrc[ct] = tmat[(int64_t(base) + u + 1) * kTN + cg * 64 + ct * 16 + (lane & 15)];
       }
+      for (int kk = kp; kk < K4; kk += 2) {
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) bcur[ct] = bnext[ct];
+        if (kk + 2 < K4)
+          load_b(src, kk + 2, bnext);
+        else if (u + 1 < U && kp < K4)
+          load_b(nsrc, kp, bnext);
+#pragma unroll
+        for (int r = 0; r < RH; r++) {
+          if (r >= nrt) break;
+          const float a = Acur[(kk * nrt + r) * 64 + lane];
+#pragma unroll
+          for (int ct = 0; ct < 4; ct++)
+            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bcur[ct], acc[r][ct], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) src[ct] = nsrc[ct];
+      __builtin_amdgcn_s_waitcnt(0);  // next operator block landed in LDS
+      __syncthreads();               // and everyone is done with this one
     }
   }
-  write_locals<RH>(PP, rt0 + rl0, nrt - rl0, sorted + tfirst[tile], ncol, cg, acc, lane, L);
+  reduce_parities<RH>(acc, kp, cg, lane, As);
+  if (kp == 0) write_locals<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane, L);
 }
 
 template <int RH>
@@ -413,12 +497,11 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
                                                              double *__restrict__ L) {
   extern __shared__ v8bf Ab[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int cg = w & 3, rh = w >> 2;
+  const int cg = w & 3, kp = w >> 2;
   const int tile = blockIdx.x;
   const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
   const int rt0 = blockIdx.y * RTB;
   const int nrt = min(RTB, RT - rt0);
-  const int rl0 = rh * RH;
   const int kq = lane >> 4;
   v4f acc[RH][4];
 #pragma unroll
@@ -426,57 +509,98 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) acc[r][ct] = v4f{0, 0, 0, 0};
   const int64_t per = int64_t(K32) * RT * 64;  // v8bf per key and part
-  const int E = K32 * nrt * 64;                // v8bf per staged part
-  v8bf *Ahs = Ab, *Als = Ab + E;
+  const int NC = K32 * nrt;                    // runs of 64 v8bf per staged part
+  const int E = NC * 64;                       // v8bf per staged part
   const v8bf zero = {};
-  for (int u = 0; u < U; u++) {
-    const int key = tkeys[base + u];
-    const v8bf *bh[4], *bl[4];
-    bool have[4];
-#pragma unroll
-    for (int ct = 0; ct < 4; ct++) {
-      const int sc = tmat[(int64_t(base) + u) * kTN + cg * 64 + ct * 16 + (lane & 15)];
-      have[ct] = sc >= 0;
-      bh[ct] = reinterpret_cast<const v8bf *>(mhi + int64_t(have[ct] ? sc : 0) * KP) + kq;
-      bl[ct] = reinterpret_cast<const v8bf *>(mlo + int64_t(have[ct] ? sc : 0) * KP) + kq;
-    }
-    __syncthreads();
-    const v8bf *Ah = ophi + key * per, *Al = oplo + key * per;
-    for (int e = threadIdx.x; e < E; e += kGemmThreads) {
-      const int l = e % 64, rl = (e / 64) % nrt, kk = e / (64 * nrt);
-      const int64_t g = (int64_t(kk) * RT + rt0 + rl) * 64 + l;
-      Ahs[e] = Ah[g];
-      Als[e] = Al[g];
-    }
-    __syncthreads();
-    for (int kk = 0; kk < K32; kk++) {
-      v8bf b_hi[4], b_lo[4];
+  if (U > 0) {
+    // buffer b: hi part at Ab + 2 b E, lo part at Ab + (2 b + 1) E
+    auto issue = [&](int u, v8bf *dst) {
+      const int64_t k = tkeys[base + u];
+      for (int c = w; c < 2 * NC; c += kGemmThreads / 64) {
+        const int part = c / NC, cc = c % NC;
+        const v8bf *A = (part ? oplo : ophi) + k * per;
+        __builtin_amdgcn_global_load_lds(A + int64_t(op_run(cc, nrt, RT, rt0)) * 64 + lane,
+                                         dst + part * E + cc * 64, 16, 0, 0);
+      }
+    };
+    // columns past P^2 are zero in the cast multipoles: not fetched
+    auto load_b = [&](const int (&sc)[4], int kk, v8bf (&bh)[4], v8bf (&bl)[4]) {
+      const int k0 = kk * 32 + kq * 8;
 #pragma unroll
       for (int ct = 0; ct < 4; ct++) {
-        b_hi[ct] = have[ct] ? bh[ct][kk * 4] : zero;
-        b_lo[ct] = have[ct] ? bl[ct][kk * 4] : zero;
+        const bool have = sc[ct] >= 0 && k0 < PP;
+        const int64_t o = int64_t(have ? sc[ct] : 0) * KP + k0;
+        bh[ct] = have ? *reinterpret_cast<const v8bf *>(mhi + o) : zero;
+        bl[ct] = have ? *reinterpret_cast<const v8bf *>(mlo + o) : zero;
       }
+    };
+    int src[4], nsrc[4];
+    v8bf bh[4], bl[4], nbh[4], nbl[4];
 #pragma unroll
-      for (int r = 0; r < RH; r++) {
-        if (rl0 + r >= nrt) break;
-        const int ai = (kk * nrt + rl0 + r) * 64 + lane;
-        const v8bf a_hi = Ahs[ai], a_lo = Als[ai];
+    for (int ct = 0; ct < 4; ct++) src[ct] = tmat[int64_t(base) * kTN + cg * 64 + ct * 16 + (lane & 15)];
+    if (kp < K32) load_b(src, kp, nbh, nbl);
+    issue(0, Ab);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    for (int u = 0; u < U; u++) {
+      const v8bf *Ahs = Ab + 2 * (u & 1) * E, *Als = Ahs + E;
+      if (u + 1 < U) {
+        issue(u + 1, Ab + 2 * ((u + 1) & 1) * E);
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) nsrc[ct] = tmat[(int64_t(base) + u + 1) * kTN + cg * 64 + ct * 16 + (lane & 15)];
+      }
+      for (int kk = kp; kk < K32; kk += 2) {
 #pragma unroll
         for (int ct = 0; ct < 4; ct++) {
-          acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi[ct], acc[r][ct], 0, 0, 0);
-          acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo[ct], acc[r][ct], 0, 0, 0);
-          acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi[ct], acc[r][ct], 0, 0, 0);
+          bh[ct] = nbh[ct];
+          bl[ct] = nbl[ct];
+        }
+        if (kk + 2 < K32)
+          load_b(src, kk + 2, nbh, nbl);
+        else if (u + 1 < U && kp < K32)
+          load_b(nsrc, kp, nbh, nbl);
+#pragma unroll
+        for (int r = 0; r < RH; r++) {
+          if (r >= nrt) break;
+          const int ai = (kk * nrt + r) * 64 + lane;
+          const v8bf a_hi = Ahs[ai], a_lo = Als[ai];
+#pragma unroll
+          for (int ct = 0; ct < 4; ct++) {
+            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, bh[ct], acc[r][ct], 0, 0, 0);
+            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, bl[ct], acc[r][ct], 0, 0, 0);
+            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, bh[ct], acc[r][ct], 0, 0, 0);
+          }
         }
       }
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++) src[ct] = nsrc[ct];
+      __builtin_amdgcn_s_waitcnt(0);  // next operator block landed in LDS
+      __syncthreads();               // and everyone is done with this one
     }
   }
-  write_locals<RH>(PP, rt0 + rl0, nrt - rl0, sorted + tfirst[tile], ncol, cg, acc, lane, L);
+  reduce_parities<RH>(acc, kp, cg, lane, Ab);
+  if (kp == 0) write_locals<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane, L);
 }
 
-// Row tiles per workgroup so the staged operator block fits kGemmLds.
-static int rows_per_block(const Geom &g, bool bf16) {
-  const int per_rt = bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4;
-  return std::max(1, std::min(g.R, kGemmLds / per_rt));
+// Row tiles per workgroup: two operator buffers within the LDS budget, at
+// most kGemmMaxRT accumulator row tiles per wave, and room for the parity
+// reduction (4 x RTB x 4 KB).
+static int rows_per_block(const Geom &g, bool bf16, int device) {
+  int max_lds = 0;
+  XF_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
+  const int per_rt = 2 * (bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4);  // both buffers
+  const int need = std::max(per_rt, 4 * 4 * 64 * 16);                   // or the reduction
+  if (need > max_lds) fail(EXAFMM_ERR_UNSUPPORTED, "expansion order too large for the LDS operator stage");
+  int rtb = std::min({g.R, kGemmMaxRT, max_lds / need});
+  // balance the row blocks
+  const int nblk = ceil_div(g.R, rtb);
+  return ceil_div(g.R, nblk);
+}
+
+static size_t gemm_lds(const Geom &g, bool bf16, int RTB) {
+  const size_t ops = size_t(2) * RTB * (bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4);
+  const size_t red = size_t(4) * RTB * 4 * 64 * 16;
+  return std::max(ops, red);
 }
 
 template <int RH>
@@ -493,11 +617,15 @@ static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool 
                                                       per * c.lists.nkeys);
     const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
     const __bf16 *mlo = mhi + size_t(sc.t.ncells) * KPb;
-    const size_t lds = size_t(g.K32) * RTB * 64 * 16 * 2;
+    const size_t lds = gemm_lds(g, true, RTB);
+    XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3<RH>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL((k_m2l_bf16x3<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, tfirst,
                        tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi, mlo, c.L.p);
   } else {
-    const size_t lds = size_t(g.K4) * RTB * 64 * 4;
+    const size_t lds = gemm_lds(g, false, RTB);
+    XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_f32<RH>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL((k_m2l_f32<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K4, KP, tfirst,
                        tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, c.ops32.p, sc.mt32.p, c.L.p);
   }
@@ -571,7 +699,7 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
       *d_U = c.tiles.p + 3 * ntiles, *d_sorted = c.tiles.p + 4 * ntiles;
   const int nwords = (nkeys + 31) / 32;
   hipLaunchKernelGGL(k_tile_count, dim3(ntiles), dim3(256), sizeof(unsigned) * nwords, s, d_first, d_count,
-                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, c.lists.slot2id.p, nkeys, d_U);
+                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_U);
   XF_LAUNCH();
   scan_exclusive(d_U, d_base, ntiles, c.misc.p + 256, s);
   int totalU = 0;
@@ -580,7 +708,7 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   c.tile_keys.ensure(std::max(totalU, 1));
   c.tile_mat.ensure(size_t(std::max(totalU, 1)) * kTN);
   hipLaunchKernelGGL(k_tile_fill, dim3(ntiles), dim3(256), sizeof(unsigned) * 2 * nwords, s, d_first, d_count,
-                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, c.lists.slot2id.p, nkeys, d_base,
+                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_base,
                      c.tile_keys.p, c.tile_mat.p);
   XF_LAUNCH();
 
@@ -620,13 +748,16 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
 
   // 3. GEMM over tiles
   XF_HIP(hipEventRecord(c.ev[13], s));
-  const int RTB = rows_per_block(g, bf16);
-  switch ((RTB + 1) / 2) {
+  const int RTB = rows_per_block(g, bf16, c.cfg.device);
+  switch (RTB) {
     case 1: launch_gemm<1>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
     case 2: launch_gemm<2>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
     case 3: launch_gemm<3>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
     case 4: launch_gemm<4>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
     case 5: launch_gemm<5>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
+    case 6: launch_gemm<6>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
+    case 7: launch_gemm<7>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
+    case 8: launch_gemm<8>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
     default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
   }
 }

@@ -5,24 +5,24 @@
 // Both add p += q/r and F -= q*dX/r^3 (dX = x_target - x_source) and skip
 // pairs at zero distance.
 //
-// FP32 path: one wavefront per target leaf.  The leaf's targets sit in lanes;
-// when a leaf holds n_i <= 32 bodies the wave splits into S = 64/n_i lane
-// groups that take every S-th source, so small leaves still fill the wave.
-// Sources of consecutive list entries are staged 256 at a time in LDS,
-// already shifted into the target leaf's frame (coordinates are stored
-// relative to their own leaf centre, so every difference is formed from
-// small FP32 numbers).  1/r is the hardware reciprocal square root refined
-// by one Newton step.  Each staged round is summed in FP32 and folded into
-// FP64 accumulators (compensated accumulation: the FP64 carry keeps the
-// low-order bits a long FP32 chain would drop), and lane groups are
-// reduced through LDS at the end.
+// FP32 path: one wavefront per target leaf, sources in lanes.  The leaf's
+// list entries are staged in LDS (source range and the shift of the source
+// leaf into the target leaf's frame, periodic image included) with a prefix
+// of their body counts; lane l takes source 64k + l of the concatenated
+// list in round k.  Targets are wave-uniform (scalar registers), 16 at a
+// time, so each interaction reads nothing from LDS: per lane and target it
+// accumulates p and the three force components over its sources, and the
+// 64 lane partials are added with a fixed butterfly at the end.
+// Coordinates are stored relative to their own leaf centre, so every
+// difference is formed from small FP32 numbers.  1/r is the hardware
+// reciprocal square root (optionally refined by one Newton step).
 #include "fmm_internal.h"
 
 namespace xf {
 
-constexpr int kP2PWaves = 4;
-
-constexpr int kP2PStage = 256;  // sources staged per round, per wave
+constexpr int kP2PWaves = 4;   // target leaves per workgroup, one wave each
+constexpr int kP2PSeg = 256;   // list entries staged per segment
+constexpr int kP2PTC = 16;     // targets per register block
 
 template <bool NEWTON>
 __device__ inline void interact(const float4 xi, const float4 sj, float &pot, float &ax, float &ay, float &az) {
@@ -45,6 +45,104 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
+__device__ inline float wave_sum(float v) {
+  v += __shfl_xor(v, 32);
+  v += __shfl_xor(v, 16);
+  v += __shfl_xor(v, 8);
+  v += __shfl_xor(v, 4);
+  v += __shfl_xor(v, 2);
+  v += __shfl_xor(v, 1);
+  return v;
+}
+
+// One block of TC wave-uniform targets against the whole list of the leaf.
+// Every target slot is computed (slots past tc repeat the last target and
+// are not written), so the TC interactions per source interleave freely.
+template <bool NEWTON, int TC>
+__device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int b0, int nl, double4 Cc,
+                                 const int *__restrict__ list, const int *__restrict__ scbeg,
+                                 const int *__restrict__ sccnt, const double4 *__restrict__ scx,
+                                 const float4 *__restrict__ sxq, double cycle, int *pre, int *sbg, float4 *off,
+                                 int lane, exafmm_body *__restrict__ out) {
+  float4 xt[TC];
+#pragma unroll
+  for (int t = 0; t < TC; t++) xt[t] = xt_src[min(t, tc - 1)];
+  float ap[TC], ax[TC], ay[TC], az[TC];
+#pragma unroll
+  for (int t = 0; t < TC; t++) ap[t] = ax[t] = ay[t] = az[t] = 0.0f;
+  for (int seg = 0; seg < nl; seg += kP2PSeg) {
+    const int ne = min(kP2PSeg, nl - seg);
+    wave_sync();
+    // entries of this segment: counts, then a wave prefix into pre[]
+    constexpr int kPer = kP2PSeg / kWave;
+    int cnt[kPer], run = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int e = lane * kPer + k;
+      cnt[k] = 0;
+      if (e < ne) {
+        const int en = list[b0 + seg + e];
+        const int sc = en >> 5, img = en & 31;
+        cnt[k] = sccnt[sc];
+        sbg[e] = scbeg[sc];
+        const double4 Cs = scx[sc];
+        off[e] = make_float4(float(Cs.x + ((img % 3) - 1) * cycle - Cc.x),
+                             float(Cs.y + (((img / 3) % 3) - 1) * cycle - Cc.y),
+                             float(Cs.z + ((img / 9) - 1) * cycle - Cc.z), 0.0f);
+      }
+      run += cnt[k];
+    }
+    int incl = run;
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    int acc = incl - run;
+    if (lane == 0) pre[0] = 0;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int e = lane * kPer + k;
+      acc += cnt[k];
+      if (e < ne) pre[e + 1] = acc;
+    }
+    wave_sync();
+    const int total = pre[ne];
+    // round k: lane takes source k*64 + lane; its entry cursor only moves forward
+    int e = 0;
+    float4 cur = make_float4(0.f, 0.f, 0.f, 0.f), o = cur;
+    if (lane < total) {
+      while (pre[e + 1] <= lane) e++;
+      cur = sxq[sbg[e] + lane - pre[e]];
+      o = off[e];
+    }
+    for (int k0 = 0; k0 < total; k0 += kWave) {
+      const float4 sj = make_float4(cur.x + o.x, cur.y + o.y, cur.z + o.z, cur.w);
+      const int g = k0 + kWave + lane;
+      cur = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (g < total) {  // prefetch the next round's source
+        while (pre[e + 1] <= g) e++;
+        cur = sxq[sbg[e] + g - pre[e]];
+        o = off[e];
+      }
+#pragma unroll
+      for (int t = 0; t < TC; t++) interact<NEWTON>(xt[t], sj, ap[t], ax[t], ay[t], az[t]);
+    }
+  }
+  // add the 64 lane partials (butterfly, same order everywhere) and write
+#pragma unroll
+  for (int t = 0; t < TC; t++) {
+    const float p = wave_sum(ap[t]), fx = wave_sum(ax[t]), fy = wave_sum(ay[t]), fz = wave_sum(az[t]);
+    if (lane == t && t < tc) {
+      exafmm_body *ob = out + t;
+      ob->p += p;
+      ob->F[0] -= fx;
+      ob->F[1] -= fy;
+      ob->F[2] -= fz;
+    }
+  }
+}
+
 template <bool NEWTON>
 __global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32(
     const int *__restrict__ leaves, const int *__restrict__ nleaves, const int *__restrict__ cbeg,
@@ -52,88 +150,26 @@ __global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32(
     const int *__restrict__ lbeg, const int *__restrict__ lcnt, const int *__restrict__ list,
     const int *__restrict__ scbeg, const int *__restrict__ sccnt, const double4 *__restrict__ scx,
     const float4 *__restrict__ sxq, double cycle, exafmm_body *__restrict__ bodies) {
-  __shared__ float4 src[kP2PWaves][kP2PStage];
-  __shared__ double red[kP2PWaves][4][kWave];
+  __shared__ int s_pre[kP2PWaves][kP2PSeg + 1];  // prefix of the entries' body counts
+  __shared__ int s_beg[kP2PWaves][kP2PSeg];      // first source body of each entry
+  __shared__ float4 s_off[kP2PWaves][kP2PSeg];   // source leaf frame -> target leaf frame
+  // (targets stay in vector registers: a block of them in scalar registers
+  // would spill)
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
   const int li = blockIdx.x * kP2PWaves + w;
-  if (li >= *nleaves) return;
+  if (li >= *nleaves) return;  // whole wave; the kernel uses wave-level syncs only
   const int c = leaves[li];
   const int tb = cbeg[c], tn = ccnt[c];
   const double4 Cc = cx[c];
   const int b0 = lbeg[c], nl = lcnt[c];
-  float4 *stage = src[w];
-  for (int t0 = 0; t0 < tn; t0 += kWave) {
-    const int ni = min(kWave, tn - t0);
-    const int S = kWave / ni;
-    const int i = lane % ni, g = lane / ni;
-    const bool active = g < S;
-    const float4 xi = xq[tb + t0 + i];
-    double P64 = 0, X64 = 0, Y64 = 0, Z64 = 0;
-    int s = 0, j_in = 0;  // next source leaf of the list, offset inside it
-    while (s < nl) {
-      // stage up to kP2PStage sources from consecutive list entries, shifted
-      // into the target leaf's frame (periodic image included)
-      int nfill = 0;
-      while (nfill < kP2PStage && s < nl) {
-        const int e = list[b0 + s];
-        const int sc = e >> 5, img = e & 31;
-        const int sb = scbeg[sc], sn = sccnt[sc];
-        const double4 Cs = scx[sc];
-        const float ox = float(Cs.x + ((img % 3) - 1) * cycle - Cc.x);
-        const float oy = float(Cs.y + (((img / 3) % 3) - 1) * cycle - Cc.y);
-        const float oz = float(Cs.z + ((img / 9) - 1) * cycle - Cc.z);
-        const int take = min(sn - j_in, kP2PStage - nfill);
-        for (int k = lane; k < take; k += kWave) {
-          const float4 v = sxq[sb + j_in + k];
-          stage[nfill + k] = make_float4(v.x + ox, v.y + oy, v.z + oz, v.w);
-        }
-        nfill += take;
-        j_in += take;
-        if (j_in == sn) {
-          s++;
-          j_in = 0;
-        }
-      }
-      wave_sync();
-      float pot = 0, ax = 0, ay = 0, az = 0;
-      if (active) {
-        int j = g;
-        for (; j + 3 * S < nfill; j += 4 * S) {
-          const float4 s0 = stage[j], s1 = stage[j + S], s2 = stage[j + 2 * S], s3 = stage[j + 3 * S];
-          interact<NEWTON>(xi, s0, pot, ax, ay, az);
-          interact<NEWTON>(xi, s1, pot, ax, ay, az);
-          interact<NEWTON>(xi, s2, pot, ax, ay, az);
-          interact<NEWTON>(xi, s3, pot, ax, ay, az);
-        }
-        for (; j < nfill; j += S) interact<NEWTON>(xi, stage[j], pot, ax, ay, az);
-      }
-      // fold the round's FP32 partials into the FP64 accumulators
-      P64 += pot;
-      X64 += ax;
-      Y64 += ay;
-      Z64 += az;
-      wave_sync();
-    }
-    red[w][0][lane] = P64;
-    red[w][1][lane] = X64;
-    red[w][2][lane] = Y64;
-    red[w][3][lane] = Z64;
-    wave_sync();
-    if (lane < ni) {
-      double p = 0, fx = 0, fy = 0, fz = 0;
-      for (int gg = 0; gg < S; gg++) {
-        p += red[w][0][gg * ni + lane];
-        fx += red[w][1][gg * ni + lane];
-        fy += red[w][2][gg * ni + lane];
-        fz += red[w][3][gg * ni + lane];
-      }
-      exafmm_body *o = bodies + tb + t0 + lane;
-      o->p += p;
-      o->F[0] -= fx;
-      o->F[1] -= fy;
-      o->F[2] -= fz;
-    }
-    wave_sync();
+  for (int t0 = 0; t0 < tn; t0 += kP2PTC) {
+    const int tc = min(kP2PTC, tn - t0);
+    if (tc > kP2PTC / 2)
+      p2p_block<NEWTON, kP2PTC>(tc, xq + tb + t0, b0, nl, Cc, list, scbeg, sccnt, scx, sxq, cycle, s_pre[w],
+                                s_beg[w], s_off[w], lane, bodies + tb + t0);
+    else
+      p2p_block<NEWTON, kP2PTC / 2>(tc, xq + tb + t0, b0, nl, Cc, list, scbeg, sccnt, scx, sxq, cycle, s_pre[w],
+                                    s_beg[w], s_off[w], lane, bodies + tb + t0);
   }
 }
 
@@ -193,7 +229,7 @@ void p2p_apply(Ctx &tc, const Ctx &sc, hipStream_t s) {
     hipLaunchKernelGGL(k_p2p64, dim3(T.nleaves), dim3(64), 0, s, T.leaves.p, T.misc.p + 17, T.cbeg.p, T.ccnt.p,
                        T.xs.p, L.p2p_beg.p, L.p2p_cnt.p, L.p2p.p, S.cbeg.p, S.ccnt.p, S.xs.p, cycle, tc.bodies);
   } else {
-    hipLaunchKernelGGL(k_p2p32<true>, dim3(ceil_div(T.nleaves, kP2PWaves)), dim3(kP2PWaves * kWave), 0, s,
+    hipLaunchKernelGGL(k_p2p32<false>, dim3(ceil_div(T.nleaves, kP2PWaves)), dim3(kP2PWaves * kWave), 0, s,
                        T.leaves.p, T.misc.p + 17, T.cbeg.p, T.ccnt.p, T.cx.p, T.xq.p, L.p2p_beg.p, L.p2p_cnt.p,
                        L.p2p.p, S.cbeg.p, S.ccnt.p, S.cx.p, S.xq.p, cycle, tc.bodies);
   }

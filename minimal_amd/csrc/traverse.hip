@@ -245,10 +245,21 @@ __global__ void __launch_bounds__(1024) k_trav_scan(int lstart, int lcount, cons
   const int chunk = (lcount + 1023) / 1024;
   const int b = t * chunk, e = min(b + chunk, lcount);
   int s0 = 0, s1 = 0, s2 = 0;
-  for (int i = b; i < e; i++) {
-    s0 += m2l_cnt[lstart + i];
-    s1 += p2p_cnt[lstart + i];
-    s2 += out_cnt[lstart + i];
+  for (int i0 = b; i0 < e; i0 += 8) {  // eight loads in flight per counter
+    int v0[8], v1[8], v2[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const bool in = i0 + k < e;
+      v0[k] = in ? m2l_cnt[lstart + i0 + k] : 0;
+      v1[k] = in ? p2p_cnt[lstart + i0 + k] : 0;
+      v2[k] = in ? out_cnt[lstart + i0 + k] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      s0 += v0[k];
+      s1 += v1[k];
+      s2 += v2[k];
+    }
   }
   part[0][t] = s0;
   part[1][t] = s1;
@@ -267,13 +278,26 @@ __global__ void __launch_bounds__(1024) k_trav_scan(int lstart, int lcount, cons
   int r0 = tot[0] + (t ? part[0][t - 1] : 0);
   int r1 = tot[1] + (t ? part[1][t - 1] : 0);
   int r2 = tot[2] + (t ? part[2][t - 1] : 0);
-  for (int i = b; i < e; i++) {
-    m2l_beg[lstart + i] = r0;
-    p2p_beg[lstart + i] = r1;
-    out_beg[lstart + i] = r2;
-    r0 += m2l_cnt[lstart + i];
-    r1 += p2p_cnt[lstart + i];
-    r2 += out_cnt[lstart + i];
+  for (int i0 = b; i0 < e; i0 += 8) {
+    int v0[8], v1[8], v2[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const bool in = i0 + k < e;
+      v0[k] = in ? m2l_cnt[lstart + i0 + k] : 0;
+      v1[k] = in ? p2p_cnt[lstart + i0 + k] : 0;
+      v2[k] = in ? out_cnt[lstart + i0 + k] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (i0 + k < e) {
+        m2l_beg[lstart + i0 + k] = r0;
+        p2p_beg[lstart + i0 + k] = r1;
+        out_beg[lstart + i0 + k] = r2;
+      }
+      r0 += v0[k];
+      r1 += v1[k];
+      r2 += v2[k];
+    }
   }
   __syncthreads();
   if (t == 0) {
@@ -297,6 +321,13 @@ __global__ void k_hash_collect(const unsigned long long *__restrict__ slots, int
     keys[p] = slots[i];
     sl[p] = i;
   }
+}
+
+// M2L entries carry their hash slot out of the traversal; after the keys are
+// sorted the slot becomes the key's rank (the id the operators are stored by).
+__global__ void k_entry_ids(int2 *__restrict__ m2l, int64_t n, const int *__restrict__ slot2id) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) m2l[i].y = slot2id[m2l[i].y];
 }
 
 __global__ void k_slot_ids(const int *__restrict__ sl, int n, int *__restrict__ slot2id) {
@@ -414,10 +445,14 @@ void traverse(Ctx &tc, const Ctx &sc) {
     int nk = 0;
     XF_HIP(hipMemcpyAsync(&nk, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
     XF_HIP(hipStreamSynchronize(s));
-    Buf<int> rh;
-    radix_sort_pairs(k1, v1, k2, v2, nk, 0, 64, rh, s);
+    radix_sort_pairs(k1, v1, k2, v2, nk, 0, 64, T.rs_hist, s);
     hipLaunchKernelGGL(k_slot_ids, dim3(ceil_div(std::max(nk, 1), 256)), dim3(256), 0, s, v1, nk, Lr.slot2id.p);
     XF_LAUNCH();
+    if (Lr.nm2l > 0) {
+      hipLaunchKernelGGL(k_entry_ids, dim3(ceil_div(Lr.nm2l, 256)), dim3(256), 0, s, Lr.m2l.p, Lr.nm2l,
+                         Lr.slot2id.p);
+      XF_LAUNCH();
+    }
     if (k1 != Lr.keys.p)
       XF_HIP(hipMemcpyAsync(Lr.keys.p, k1, sizeof(unsigned long long) * nk, hipMemcpyDeviceToDevice, s));
     XF_HIP(hipStreamSynchronize(s));
