@@ -16,6 +16,8 @@
 // Coordinates are stored relative to their own leaf centre, so every
 // difference is formed from small FP32 numbers.  1/r is the hardware
 // reciprocal square root (optionally refined by one Newton step).
+#include <type_traits>
+
 #include "fmm_internal.h"
 
 namespace xf {
@@ -23,21 +25,6 @@ namespace xf {
 constexpr int kP2PWaves = 4;   // target leaves per workgroup, one wave each
 constexpr int kP2PSeg = 256;   // list entries staged per segment
 constexpr int kP2PTC = 16;     // targets per register block
-
-template <bool NEWTON>
-__device__ inline void interact(const float4 xi, const float4 sj, float &pot, float &ax, float &ay, float &az) {
-  const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
-  const float r2 = dx * dx + dy * dy + dz * dz;
-  float inv = __builtin_amdgcn_rsqf(r2);
-  if (NEWTON) inv = inv * (1.5f - 0.5f * r2 * inv * inv);
-  inv = r2 > 0.0f ? inv : 0.0f;
-  const float qi = sj.w * inv;
-  const float q3 = qi * inv * inv;
-  pot += qi;
-  ax += dx * q3;
-  ay += dy * q3;
-  az += dz * q3;
-}
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
@@ -55,41 +42,110 @@ __device__ inline float wave_sum(float v) {
   return v;
 }
 
+// Sum over the 64 lanes of TC per-lane values by recursive halving: at each
+// step a lane keeps one half of its values and sends the other half to its
+// partner (TC - 1 shuffles instead of 6 TC).  Afterwards lane l holds the
+// partial sum of value l >> (6 - log2 TC) over the lanes that share its low
+// bits; the caller finishes with a butterfly over those bits.
+template <int TC>
+__device__ inline float halving(float (&v)[TC], int lane) {
+  int c = TC;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    if (c > 1) {
+      const bool up = lane & m;
+#pragma unroll
+      for (int i = 0; i < TC / 2; i++) {
+        if (i < c / 2) {
+          const float lo = v[i], hi = v[i + c / 2];
+          v[i] = (up ? hi : lo) + __shfl_xor(up ? lo : hi, m);
+        }
+      }
+      c >>= 1;
+    }
+  }
+  return v[0];
+}
+
+// The interactions of one staged source per lane with the TC targets (the
+// TC chains are independent, so they interleave).  CHECK skips pairs at zero distance (3d/kernel.h:139); only
+// rounds that can hold such pairs (the target leaf's own bodies, or any
+// round when sources and targets come from different trees) use it.
+template <bool CHECK, int TC>
+__device__ inline void interact_block(const float3 (&xt)[TC], float4 sj, float (&ap)[TC], float (&ax)[TC],
+                                      float (&ay)[TC], float (&az)[TC]) {
+#pragma unroll
+  for (int t = 0; t < TC; t++) {
+    const float dx = xt[t].x - sj.x, dy = xt[t].y - sj.y, dz = xt[t].z - sj.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float v = __builtin_amdgcn_rsqf(r2);
+    const float inv = CHECK ? (r2 > 0.0f ? v : 0.0f) : v;
+    const float qi = sj.w * inv;
+    const float q3 = qi * inv * inv;
+    ap[t] += qi;
+    ax[t] += dx * q3;
+    ay[t] += dy * q3;
+    az[t] += dz * q3;
+  }
+}
+
 // One block of TC wave-uniform targets against the whole list of the leaf.
 // Every target slot is computed (slots past tc repeat the last target and
-// are not written), so the TC interactions per source interleave freely.
-template <bool NEWTON, int TC>
+// are not written).  The entry of the target leaf itself is staged first,
+// so only the first rounds need the zero-distance check.  Unused source
+// lanes sit at 1e30 with q = 0: r^2 overflows to inf, rsqrt gives 0 and the
+// pair adds exact zeros.
+template <int TC>
 __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int b0, int nl, double4 Cc,
-                                 const int *__restrict__ list, const int *__restrict__ scbeg,
+                                 int self_cell, const int *__restrict__ list, const int *__restrict__ scbeg,
                                  const int *__restrict__ sccnt, const double4 *__restrict__ scx,
                                  const float4 *__restrict__ sxq, double cycle, int *pre, int *sbg, float4 *off,
-                                 int lane, exafmm_body *__restrict__ out) {
-  float4 xt[TC];
+                                 int *cnt_s, int lane, exafmm_body *__restrict__ out) {
+  float3 xt[TC];
 #pragma unroll
-  for (int t = 0; t < TC; t++) xt[t] = xt_src[min(t, tc - 1)];
+  for (int t = 0; t < TC; t++) {
+    const float4 v = xt_src[min(t, tc - 1)];
+    xt[t] = make_float3(v.x, v.y, v.z);
+  }
   float ap[TC], ax[TC], ay[TC], az[TC];
 #pragma unroll
   for (int t = 0; t < TC; t++) ap[t] = ax[t] = ay[t] = az[t] = 0.0f;
+  constexpr int kPer = kP2PSeg / kWave;
   for (int seg = 0; seg < nl; seg += kP2PSeg) {
     const int ne = min(kP2PSeg, nl - seg);
     wave_sync();
-    // entries of this segment: counts, then a wave prefix into pre[]
-    constexpr int kPer = kP2PSeg / kWave;
+    // where is the target leaf's own entry (image 13 = no shift)?
+    int self = -1;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int e = lane * kPer + k;
+      if (e < ne && list[b0 + seg + e] == ((self_cell << 5) | 13)) self = e;
+    }
+    const unsigned long long hit = __ballot(self >= 0);
+    const int sidx = hit ? __shfl(self, __ffsll((long long)hit) - 1) : -1;
+    // stage entry e at position 0 if it is the leaf itself (entry 0 takes its place)
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const int e = lane * kPer + k;
+      if (e < ne) {
+        const int pos = e == sidx ? 0 : (e == 0 && sidx > 0 ? sidx : e);
+        const int en = list[b0 + seg + e];
+        const int sc = en >> 5, img = en & 31;
+        cnt_s[pos] = sccnt[sc];
+        sbg[pos] = scbeg[sc];
+        const double4 Cs = scx[sc];
+        off[pos] = make_float4(float(Cs.x + ((img % 3) - 1) * cycle - Cc.x),
+                               float(Cs.y + (((img / 3) % 3) - 1) * cycle - Cc.y),
+                               float(Cs.z + ((img / 9) - 1) * cycle - Cc.z), 0.0f);
+      }
+    }
+    wave_sync();
+    // prefix of the staged counts into pre[]
     int cnt[kPer], run = 0;
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       const int e = lane * kPer + k;
-      cnt[k] = 0;
-      if (e < ne) {
-        const int en = list[b0 + seg + e];
-        const int sc = en >> 5, img = en & 31;
-        cnt[k] = sccnt[sc];
-        sbg[e] = scbeg[sc];
-        const double4 Cs = scx[sc];
-        off[e] = make_float4(float(Cs.x + ((img % 3) - 1) * cycle - Cc.x),
-                             float(Cs.y + (((img / 3) % 3) - 1) * cycle - Cc.y),
-                             float(Cs.z + ((img / 9) - 1) * cycle - Cc.z), 0.0f);
-      }
+      cnt[k] = e < ne ? cnt_s[e] : 0;
       run += cnt[k];
     }
     int incl = run;
@@ -108,50 +164,67 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
     }
     wave_sync();
     const int total = pre[ne];
+    const int nchecked = self_cell == -2 ? total : (sidx >= 0 ? pre[1] : 0);
     // round k: lane takes source k*64 + lane; its entry cursor only moves forward
+    const float4 far = make_float4(1e30f, 1e30f, 1e30f, 0.0f);
     int e = 0;
-    float4 cur = make_float4(0.f, 0.f, 0.f, 0.f), o = cur;
+    float4 cur = far, o = make_float4(0.f, 0.f, 0.f, 0.f);
     if (lane < total) {
       while (pre[e + 1] <= lane) e++;
       cur = sxq[sbg[e] + lane - pre[e]];
       o = off[e];
     }
-    for (int k0 = 0; k0 < total; k0 += kWave) {
+    // rounds that may hold zero-distance pairs first, then the rest (two
+    // loops, one interaction variant each, keep the register allocation of
+    // one variant)
+    auto round = [&](int k0, auto check) {
       const float4 sj = make_float4(cur.x + o.x, cur.y + o.y, cur.z + o.z, cur.w);
       const int g = k0 + kWave + lane;
-      cur = make_float4(0.f, 0.f, 0.f, 0.f);
+      cur = far;
+      o = make_float4(0.f, 0.f, 0.f, 0.f);
       if (g < total) {  // prefetch the next round's source
         while (pre[e + 1] <= g) e++;
         cur = sxq[sbg[e] + g - pre[e]];
         o = off[e];
       }
-#pragma unroll
-      for (int t = 0; t < TC; t++) interact<NEWTON>(xt[t], sj, ap[t], ax[t], ay[t], az[t]);
-    }
+      interact_block<decltype(check)::value, TC>(xt, sj, ap, ax, ay, az);
+    };
+    int k0 = 0;
+    for (; k0 < total && k0 < nchecked; k0 += kWave) round(k0, std::true_type());
+    for (; k0 < total; k0 += kWave) round(k0, std::false_type());
   }
-  // add the 64 lane partials (butterfly, same order everywhere) and write
+  // 64-lane sums: halving over the targets, then a butterfly over the lanes
+  // that share a target; lane l writes component l % 4 of target l / 4
+  // (TC = 16) or component l % 4 of target l / 8 for the lanes l % 8 < 4 (TC = 8)
+  constexpr int kLow = TC == 16 ? 4 : 8;  // lanes sharing a target after the halving
+  float sp = halving<TC>(ap, lane), sx = halving<TC>(ax, lane), sy = halving<TC>(ay, lane),
+        sz = halving<TC>(az, lane);
 #pragma unroll
-  for (int t = 0; t < TC; t++) {
-    const float p = wave_sum(ap[t]), fx = wave_sum(ax[t]), fy = wave_sum(ay[t]), fz = wave_sum(az[t]);
-    if (lane == t && t < tc) {
-      exafmm_body *ob = out + t;
-      ob->p += p;
-      ob->F[0] -= fx;
-      ob->F[1] -= fy;
-      ob->F[2] -= fz;
-    }
+  for (int m = 1; m < kLow; m <<= 1) {
+    sp += __shfl_xor(sp, m);
+    sx += __shfl_xor(sx, m);
+    sy += __shfl_xor(sy, m);
+    sz += __shfl_xor(sz, m);
+  }
+  const int t = lane / kLow, comp = lane % kLow;
+  if (comp < 4 && t < tc) {
+    exafmm_body *ob = out + t;
+    if (comp == 0)
+      ob->p += sp;
+    else
+      ob->F[comp - 1] -= comp == 1 ? sx : (comp == 2 ? sy : sz);
   }
 }
 
-template <bool NEWTON>
 __global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32(
     const int *__restrict__ leaves, const int *__restrict__ nleaves, const int *__restrict__ cbeg,
     const int *__restrict__ ccnt, const double4 *__restrict__ cx, const float4 *__restrict__ xq,
     const int *__restrict__ lbeg, const int *__restrict__ lcnt, const int *__restrict__ list,
     const int *__restrict__ scbeg, const int *__restrict__ sccnt, const double4 *__restrict__ scx,
-    const float4 *__restrict__ sxq, double cycle, exafmm_body *__restrict__ bodies) {
+    const float4 *__restrict__ sxq, double cycle, int same_tree, exafmm_body *__restrict__ bodies) {
   __shared__ int s_pre[kP2PWaves][kP2PSeg + 1];  // prefix of the entries' body counts
   __shared__ int s_beg[kP2PWaves][kP2PSeg];      // first source body of each entry
+  __shared__ int s_cnt[kP2PWaves][kP2PSeg];      // body count of each entry
   __shared__ float4 s_off[kP2PWaves][kP2PSeg];   // source leaf frame -> target leaf frame
   // (targets stay in vector registers: a block of them in scalar registers
   // would spill)
@@ -162,14 +235,15 @@ __global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32(
   const int tb = cbeg[c], tn = ccnt[c];
   const double4 Cc = cx[c];
   const int b0 = lbeg[c], nl = lcnt[c];
+  const int self_cell = same_tree ? c : -2;  // -2: every round checks zero distances
   for (int t0 = 0; t0 < tn; t0 += kP2PTC) {
     const int tc = min(kP2PTC, tn - t0);
     if (tc > kP2PTC / 2)
-      p2p_block<NEWTON, kP2PTC>(tc, xq + tb + t0, b0, nl, Cc, list, scbeg, sccnt, scx, sxq, cycle, s_pre[w],
-                                s_beg[w], s_off[w], lane, bodies + tb + t0);
+      p2p_block<kP2PTC>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle, s_pre[w],
+                        s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
     else
-      p2p_block<NEWTON, kP2PTC / 2>(tc, xq + tb + t0, b0, nl, Cc, list, scbeg, sccnt, scx, sxq, cycle, s_pre[w],
-                                    s_beg[w], s_off[w], lane, bodies + tb + t0);
+      p2p_block<kP2PTC / 2>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle,
+                            s_pre[w], s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
   }
 }
 
@@ -229,9 +303,9 @@ void p2p_apply(Ctx &tc, const Ctx &sc, hipStream_t s) {
     hipLaunchKernelGGL(k_p2p64, dim3(T.nleaves), dim3(64), 0, s, T.leaves.p, T.misc.p + 17, T.cbeg.p, T.ccnt.p,
                        T.xs.p, L.p2p_beg.p, L.p2p_cnt.p, L.p2p.p, S.cbeg.p, S.ccnt.p, S.xs.p, cycle, tc.bodies);
   } else {
-    hipLaunchKernelGGL(k_p2p32<false>, dim3(ceil_div(T.nleaves, kP2PWaves)), dim3(kP2PWaves * kWave), 0, s,
+    hipLaunchKernelGGL(k_p2p32, dim3(ceil_div(T.nleaves, kP2PWaves)), dim3(kP2PWaves * kWave), 0, s,
                        T.leaves.p, T.misc.p + 17, T.cbeg.p, T.ccnt.p, T.cx.p, T.xq.p, L.p2p_beg.p, L.p2p_cnt.p,
-                       L.p2p.p, S.cbeg.p, S.ccnt.p, S.cx.p, S.xq.p, cycle, tc.bodies);
+                       L.p2p.p, S.cbeg.p, S.ccnt.p, S.cx.p, S.xq.p, cycle, int(&tc == &sc), tc.bodies);
   }
   XF_LAUNCH();
 }
