@@ -309,9 +309,24 @@ int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_devi
     set_bodies(c, bodies, n, on_device);
     do_build(c);
     do_lists(c, c);
-    do_upward(c);
-    do_p2p(c, c);
-    do_far(c, c);
+    if (c.cfg.m2l_precision != EXAFMM_PREC_FP64 && c.lists.nkeys > 0) {
+      // s0: upward pass, then the keyed M2L once its tiles are ready;
+      // s1: M2L tiles and operators (geometry only), then P2P
+      rec(c, 15, c.s0);
+      XF_HIP(hipStreamWaitEvent(c.s1, c.ev[15], 0));
+      do_upward(c);
+      m2l_prepare(c, c.s1);
+      rec(c, 7, c.s1);
+      rec(c, 5, c.s1);
+      p2p_apply(c, c, c.s1);
+      rec(c, 6, c.s1);
+      XF_HIP(hipStreamWaitEvent(c.s0, c.ev[7], 0));
+      do_far(c, c);
+    } else {
+      do_upward(c);
+      do_p2p(c, c);
+      do_far(c, c);
+    }
     do_downward(c);
     bodies_to_host(c);
   });

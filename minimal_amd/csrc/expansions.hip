@@ -24,6 +24,17 @@
 namespace xf {
 using namespace xfd;
 
+// 1/i for i <= 1024 (the harmonic recurrences need 1/(2m) and
+// 1/((n+m)(n-m)), n, m < 16): correctly rounded at compile time, so a table
+// read replaces an FP64 division in the kernels.
+struct InvTable {
+  double v[1025];
+  constexpr InvTable() : v() {
+    for (int i = 1; i <= 1024; i++) v[i] = 1.0 / i;
+  }
+};
+static __constant__ InvTable kInv = InvTable();
+
 // ---------------------------------------------------------------- P2M
 // One wave per leaf: the leaf's bodies are staged in LDS 64 at a time; lane
 // (g, m) = (lane / 16, lane % 16) sums column n = m..P-1 of conj(Y_n^m(u)),
@@ -51,9 +62,9 @@ __global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m(int P, const int *__re
   // per-lane constants (no divisions per body): 1/(2^m m!) and the
   // recurrence factors -1/((n+m)(n-m)), n = m+2..P-1
   double cm = 1.0, sn[PMAX];
-  for (int k = 1; k <= m; k++) cm /= double(2 * k);
+  for (int k = 1; k <= m; k++) cm *= kInv.v[2 * k];
 #pragma unroll
-  for (int n = 0; n < PMAX; n++) sn[n] = (n >= m + 2 && n < P) ? -1.0 / double((n + m) * (n - m)) : 0.0;
+  for (int n = 0; n < PMAX; n++) sn[n] = (n >= m + 2 && n < P) ? -kInv.v[(n + m) * (n - m)] : 0.0;
   double ar[PMAX], ai[PMAX];
 #pragma unroll
   for (int n = 0; n < PMAX; n++) ar[n] = ai[n] = 0;
@@ -330,7 +341,7 @@ __global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leave
     jk_of(e, n, m);
     const double w = (m == 0 ? 1.0 : 2.0) * invR;  // fold w_m and L~ -> L R^n
     Lh[e] = {T(Lc[rf(n, m, 0)] * w), T(m ? Lc[rf(n, m, 1)] * w : 0.0)};
-    rec[e] = n == m ? T(m ? 1.0 / (2 * m) : 0.0) : (n >= m + 2 ? T(-1.0 / double((n + m) * (n - m))) : T(0));
+    rec[e] = n == m ? T(m ? kInv.v[2 * m] : 0.0) : (n >= m + 2 ? T(-kInv.v[(n + m) * (n - m)]) : T(0));
   }
   __syncthreads();
   const int beg = cbeg[c], cnt = ccnt[c];

@@ -97,6 +97,7 @@ struct Tree {
   Buf<int> lcount, lstart_d, tmp_nch, tmp_off, tmp_bnd, misc;
   Buf<double> bounds_part;
   Buf<int> rs_hist;
+  Buf<int> scan_tmp;  // device-wide scan partials
   bool built = false;
 };
 
@@ -115,6 +116,8 @@ struct Lists {
   Buf<unsigned long long> hslots;  // open-addressing key set
   int hcap = 0;
   Buf<int> slot2id;
+  Buf<int> scan_tmp;       // device-wide scan partials
+  Buf<int64_t> capd;       // list capacities on the device (scan overflow check)
   Buf<unsigned long long> keys;    // distinct keys, ascending
   Buf<int> kscratch;
   int nkeys = 0;
@@ -148,6 +151,10 @@ struct Ctx {
   // precision) plus R0 and the cycle for periodic keys, so an unchanged key
   // set is not generated again.
   std::vector<unsigned long long> ops_keys;
+  // keyed M2L tiles built by m2l_prepare for the current lists
+  bool m2l_ready = false;
+  int m2l_ntiles = 0;
+  Buf<int> m2l_misc, m2l_rs_hist, m2l_scan;  // scratch of m2l_prepare (may run on the side stream)
   int ops_P = -1, ops_fill = -1, ops_bf16 = -1;
   double ops_R0 = 0, ops_cycle = 0;
   Buf<float> mt32;                 // scaled multipoles, real form [ncells][P^2]
@@ -169,6 +176,7 @@ void tree_ref_order(Ctx &c, std::vector<int64_t> &bfs2ref, std::vector<int64_t> 
 void traverse(Ctx &tgt, const Ctx &src);
 void upward(Ctx &c);
 void m2l_apply(Ctx &tgt, const Ctx &src);
+void m2l_prepare(Ctx &c, hipStream_t s);  // keyed-M2L tiles and operators (fast precisions only)
 void p2p_apply(Ctx &tgt, const Ctx &src, hipStream_t s);
 void shift_ops(Ctx &c);
 void l2l_pass(Ctx &c);
@@ -188,8 +196,13 @@ void op_l2l(Ctx &c, const double *Cp, const double *Lp, const double *Cc, double
 void op_l2p(Ctx &c, const double *C, const double *L, exafmm_body *b, int64_t nb);
 
 // generic helpers (scan.hip)
-void scan_exclusive(const int *in, int *out, int count, int *total, hipStream_t s);
-void scan_exclusive_dev(const int *in, int *out, const int *count_dev, int *total, hipStream_t s);
+void scan_exclusive(const int *in, int *out, int count, int *total, Buf<int> &scratch, hipStream_t s);
+void scan_exclusive_dev(const int *in, int *out, const int *count_dev, int count_bound, int *total,
+                        Buf<int> &scratch, hipStream_t s);
+// three arrays at once; running[k] is added to every output and receives the
+// new totals; bit 1 of *flag is raised when a total exceeds cap[k]
+void scan_exclusive3(const int *in0, const int *in1, const int *in2, int *out0, int *out1, int *out2, int count,
+                     int *running, const int64_t *cap, int *flag, Buf<int> &scratch, hipStream_t s);
 void radix_sort_pairs(unsigned long long *&k, int *&v, unsigned long long *&k2, int *&v2, int n, int lo_bit,
                       int hi_bit, Buf<int> &hist, hipStream_t s);
 

@@ -79,84 +79,41 @@ __global__ void __launch_bounds__(256) k_group_hist(int ncells, const unsigned l
   if (h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
 
-// Entries of a tile's targets flattened: each column's list start and the
-// exclusive prefix of the list lengths are staged in LDS, entry e ->
-// (column, index) by binary search, so all threads work on the tile at once.
-// M2L entries hold (source << 5 | image, key id).
-struct TileCols {
-  int off[kTN + 1];
-  int beg[kTN];
-};
-
-__device__ inline int tile_offsets(const int *cells, int ncol, const int *beg, const int *cnt, TileCols &tc) {
-  const int t = threadIdx.x;  // blockDim.x >= kTN
-  int v = 0;
-  if (t < ncol) {
-    const int c = cells[t];
-    v = cnt[c];
-    tc.beg[t] = beg[c];
-  }
-  if (t < kTN) tc.off[t + 1] = v;
-  if (t == 0) tc.off[0] = 0;
-  __syncthreads();
-  for (int o = 1; o < kTN; o <<= 1) {
-    int a = 0;
-    if (t < kTN && t >= o) a = tc.off[t + 1 - o];
-    __syncthreads();
-    if (t < kTN) tc.off[t + 1] += a;
-    __syncthreads();
-  }
-  return tc.off[ncol];
-}
-__device__ inline int col_of(const int *off, int ncol, int e) {
-  int lo = 0, hi = ncol;  // last t with off[t] <= e
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (off[mid] <= e)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
-constexpr int kTileUnroll = 4;  // entries in flight per thread
+// Tile kernels: one thread per target column walks that column's M2L list
+// (entries (source << 5 | image, key id)), eight entries in flight.
+constexpr int kTileUnroll = 8;
 
 // Distinct keys of one tile: bitmap over key ids in LDS.
-__device__ inline void tile_bitmap(int ncol, const TileCols &tc, int total, const int2 *list, unsigned *bm,
-                                   int nwords) {
+__device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, const int *cnt, const int2 *list,
+                                   unsigned *bm, int nwords) {
   for (int i = threadIdx.x; i < nwords; i += blockDim.x) bm[i] = 0;
   __syncthreads();
-  for (int e0 = threadIdx.x; e0 < total; e0 += blockDim.x * kTileUnroll) {
-    int id[kTileUnroll];
+  const int t = threadIdx.x;  // blockDim.x == kTN
+  if (t < ncol) {
+    const int c = cells[t];
+    const int b = beg[c], n = cnt[c];
+    for (int i0 = 0; i0 < n; i0 += kTileUnroll) {
+      int id[kTileUnroll];
 #pragma unroll
-    for (int k = 0; k < kTileUnroll; k++) {
-      const int e = e0 + k * blockDim.x;
-      id[k] = -1;
-      if (e < total) {
-        const int t = col_of(tc.off, ncol, e);
-        id[k] = list[tc.beg[t] + e - tc.off[t]].y;
-      }
+      for (int k = 0; k < kTileUnroll; k++) id[k] = i0 + k < n ? list[b + i0 + k].y : -1;
+#pragma unroll
+      for (int k = 0; k < kTileUnroll; k++)
+        if (id[k] >= 0) atomicOr(&bm[id[k] >> 5], 1u << (id[k] & 31));
     }
-#pragma unroll
-    for (int k = 0; k < kTileUnroll; k++)
-      if (id[k] >= 0) atomicOr(&bm[id[k] >> 5], 1u << (id[k] & 31));
   }
   __syncthreads();
 }
 
-__global__ void __launch_bounds__(256) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
+__global__ void __launch_bounds__(kTN) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                     const int *__restrict__ sorted, const int *__restrict__ beg,
                                                     const int *__restrict__ cnt, const int2 *__restrict__ list,
                                                     int nkeys, int *__restrict__ tU) {
   extern __shared__ unsigned bm[];
-  __shared__ TileCols tc;
   __shared__ int tot;
   const int nwords = (nkeys + 31) / 32;
   const int *cells = sorted + tfirst[blockIdx.x];
   const int ncol = tcount[blockIdx.x];
-  const int total = tile_offsets(cells, ncol, beg, cnt, tc);
-  tile_bitmap(ncol, tc, total, list, bm, nwords);
+  tile_bitmap(cells, ncol, beg, cnt, list, bm, nwords);
   if (threadIdx.x == 0) tot = 0;
   __syncthreads();
   int s = 0;
@@ -167,27 +124,42 @@ __global__ void __launch_bounds__(256) k_tile_count(const int *__restrict__ tfir
 }
 
 // Key list (ascending) and source matrix [U][kTN] (-1 = no pair) of a tile.
-__global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirst, const int *__restrict__ tcount,
+__global__ void __launch_bounds__(kTN) k_tile_fill(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                    const int *__restrict__ sorted, const int *__restrict__ beg,
                                                    const int *__restrict__ cnt, const int2 *__restrict__ list,
                                                    int nkeys, const int *__restrict__ tbase,
                                                    int *__restrict__ tkeys, int *__restrict__ tmat) {
   extern __shared__ unsigned bm[];
-  __shared__ TileCols tc;
   const int nwords = (nkeys + 31) / 32;
   unsigned *rank = bm + nwords;  // exclusive popcount prefix per word
   const int *cells = sorted + tfirst[blockIdx.x];
   const int ncol = tcount[blockIdx.x];
-  const int total = tile_offsets(cells, ncol, beg, cnt, tc);
-  tile_bitmap(ncol, tc, total, list, bm, nwords);
-  if (threadIdx.x == 0) {
-    unsigned r = 0;
-    for (int i = 0; i < nwords; i++) {
-      rank[i] = r;
-      r += __popc(bm[i]);
-    }
-  }
+  tile_bitmap(cells, ncol, beg, cnt, list, bm, nwords);
+  // exclusive prefix of the word popcounts (block scan in chunks of kTN words)
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
   __syncthreads();
+  for (int w0 = 0; w0 < nwords; w0 += kTN) {
+    const int i = w0 + threadIdx.x;
+    const int v = i < nwords ? __popc(bm[i]) : 0;
+    int incl = v;
+    // wave scan, then wave totals
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int u = __shfl_up(incl, d);
+      if (lane >= d) incl += u;
+    }
+    __shared__ int wsum[kTN / 64];
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    int before = carry;
+    for (int k = 0; k < wv; k++) before += wsum[k];
+    if (i < nwords) rank[i] = before + incl - v;
+    __syncthreads();
+    if (threadIdx.x == kTN - 1) carry = before + incl;
+    __syncthreads();
+  }
   const int base = tbase[blockIdx.x];
   for (int i = threadIdx.x; i < nwords; i += blockDim.x) {
     unsigned w = bm[i];
@@ -198,30 +170,25 @@ __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirs
       tkeys[base + r++] = i * 32 + b;
     }
   }
-  int U = 0;
-  if (nwords) U = rank[nwords - 1] + __popc(bm[nwords - 1]);
+  const int U = carry;
   for (int i = threadIdx.x; i < U * kTN; i += blockDim.x) tmat[int64_t(base) * kTN + i] = -1;
   __syncthreads();
-  for (int e0 = threadIdx.x; e0 < total; e0 += blockDim.x * kTileUnroll) {
-    int2 en[kTileUnroll];
-    int col[kTileUnroll];
+  const int t = threadIdx.x;
+  if (t < ncol) {
+    const int c = cells[t];
+    const int b = beg[c], n = cnt[c];
+    for (int i0 = 0; i0 < n; i0 += kTileUnroll) {
+      int2 en[kTileUnroll];
 #pragma unroll
-    for (int k = 0; k < kTileUnroll; k++) {
-      const int e = e0 + k * blockDim.x;
-      col[k] = -1;
-      if (e < total) {
-        const int t = col_of(tc.off, ncol, e);
-        col[k] = t;
-        en[k] = list[tc.beg[t] + e - tc.off[t]];
+      for (int k = 0; k < kTileUnroll; k++) en[k] = i0 + k < n ? list[b + i0 + k] : make_int2(0, -1);
+#pragma unroll
+      for (int k = 0; k < kTileUnroll; k++) {
+        const int id = en[k].y;
+        if (id < 0) continue;
+        const unsigned w = bm[id >> 5];
+        const int r = rank[id >> 5] + __popc(w & ((1u << (id & 31)) - 1u));
+        tmat[(int64_t(base) + r) * kTN + t] = en[k].x >> 5;  // source cell (the image is part of the key)
       }
-    }
-#pragma unroll
-    for (int k = 0; k < kTileUnroll; k++) {
-      if (col[k] < 0) continue;
-      const int id = en[k].y;
-      const unsigned w = bm[id >> 5];
-      const int r = rank[id >> 5] + __popc(w & ((1u << (id & 31)) - 1u));
-      tmat[(int64_t(base) + r) * kTN + col[k]] = en[k].x >> 5;  // source cell (the image is part of the key)
     }
   }
 }
@@ -647,10 +614,11 @@ static void ops_remember(Ctx &c, std::vector<unsigned long long> keys, int fillP
   c.ops_cycle = c.cfg.cycle;
 }
 
-void m2l_fast(Ctx &tc, const Ctx &sc) {
-  if (&tc != &sc) fail(EXAFMM_ERR_UNSUPPORTED, "keyed M2L needs source == target tree");
-  Ctx &c = tc;
-  hipStream_t s = c.s0;
+// Tiles, key lists and operators: depend only on the tree and the lists,
+// so they can be built on a side stream while the upward pass runs.
+void m2l_prepare(Ctx &c, hipStream_t s) {
+  c.m2l_ready = false;
+  c.m2l_ntiles = 0;
   const Geom g = geom(c.P);
   const bool bf16 = c.cfg.m2l_precision == EXAFMM_PREC_BF16X3;
   const int fillP = c.cfg.m2l_truncation == EXAFMM_M2L_FULL ? 2 * c.P - 1 : c.P;
@@ -668,14 +636,14 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   XF_LAUNCH();
   unsigned long long *k1 = c.t.key.p, *k2 = c.t.key2.p;
   int *v1 = c.t.val.p, *v2 = c.t.val2.p;
-  radix_sort_pairs(k1, v1, k2, v2, nc, 0, 8, c.t.rs_hist, s);
-  c.misc.ensure(512);
-  XF_HIP(hipMemsetAsync(c.misc.p, 0, 256 * sizeof(int), s));
-  hipLaunchKernelGGL(k_group_hist, dim3(std::min(ceil_div(nc, 256), 64)), dim3(256), 0, s, nc, k1, c.misc.p);
+  radix_sort_pairs(k1, v1, k2, v2, nc, 0, 8, c.m2l_rs_hist, s);
+  c.m2l_misc.ensure(512);
+  XF_HIP(hipMemsetAsync(c.m2l_misc.p, 0, 256 * sizeof(int), s));
+  hipLaunchKernelGGL(k_group_hist, dim3(std::min(ceil_div(nc, 256), 64)), dim3(256), 0, s, nc, k1, c.m2l_misc.p);
   XF_LAUNCH();
   std::vector<int> gc(256);
   std::vector<unsigned long long> keys(nkeys);
-  XF_HIP(hipMemcpyAsync(gc.data(), c.misc.p, 256 * sizeof(int), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipMemcpyAsync(gc.data(), c.m2l_misc.p, 256 * sizeof(int), hipMemcpyDeviceToHost, s));
   if (nkeys)
     XF_HIP(hipMemcpyAsync(keys.data(), c.lists.keys.p, sizeof(unsigned long long) * nkeys, hipMemcpyDeviceToHost,
                           s));
@@ -690,7 +658,10 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
     off += gc[gi];
   }
   const int ntiles = int(tfirst.size());
-  if (ntiles == 0) return;
+  if (ntiles == 0) {
+    c.m2l_ready = true;
+    return;
+  }
   c.tiles.ensure(size_t(4) * ntiles + nc);
   XF_HIP(hipMemcpyAsync(c.tiles.p, tfirst.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, s));
   XF_HIP(hipMemcpyAsync(c.tiles.p + ntiles, tcount.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, s));
@@ -698,30 +669,23 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   int *d_first = c.tiles.p, *d_count = c.tiles.p + ntiles, *d_base = c.tiles.p + 2 * ntiles,
       *d_U = c.tiles.p + 3 * ntiles, *d_sorted = c.tiles.p + 4 * ntiles;
   const int nwords = (nkeys + 31) / 32;
-  hipLaunchKernelGGL(k_tile_count, dim3(ntiles), dim3(256), sizeof(unsigned) * nwords, s, d_first, d_count,
+  hipLaunchKernelGGL(k_tile_count, dim3(ntiles), dim3(kTN), sizeof(unsigned) * nwords, s, d_first, d_count,
                      d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_U);
   XF_LAUNCH();
-  scan_exclusive(d_U, d_base, ntiles, c.misc.p + 256, s);
+  scan_exclusive(d_U, d_base, ntiles, c.m2l_misc.p + 256, c.m2l_scan, s);
   int totalU = 0;
-  XF_HIP(hipMemcpyAsync(&totalU, c.misc.p + 256, sizeof(int), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipMemcpyAsync(&totalU, c.m2l_misc.p + 256, sizeof(int), hipMemcpyDeviceToHost, s));
   XF_HIP(hipStreamSynchronize(s));
   c.tile_keys.ensure(std::max(totalU, 1));
   c.tile_mat.ensure(size_t(std::max(totalU, 1)) * kTN);
-  hipLaunchKernelGGL(k_tile_fill, dim3(ntiles), dim3(256), sizeof(unsigned) * 2 * nwords, s, d_first, d_count,
+  hipLaunchKernelGGL(k_tile_fill, dim3(ntiles), dim3(kTN), sizeof(unsigned) * 2 * nwords, s, d_first, d_count,
                      d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_base,
                      c.tile_keys.p, c.tile_mat.p);
   XF_LAUNCH();
 
-  // 2. operands: scaled multipoles, operators
-  const int KP = g.K4 * 4, KPb = g.K32 * 32;
+  // 2. operators (cached by key set)
   const double cyc = c.cfg.images > 0 ? c.cfg.cycle : 0.0;
   if (bf16) {
-    c.mt32.ensure((size_t(nc) * KPb * 2 + 1) / 2 + 1);  // two bf16 arrays
-    __bf16 *mhi = reinterpret_cast<__bf16 *>(c.mt32.p);
-    __bf16 *mlo = mhi + size_t(nc) * KPb;
-    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc) * KPb, 256)), dim3(256), 0, s, c.P * c.P, KPb, nc, c.M.p,
-                       (float *)nullptr, mhi, mlo);
-    XF_LAUNCH();
     const size_t per = size_t(g.K32) * g.R * 64 * 8;
     if (!ops_cached(c, keys, fillP, 1)) {
       c.ops32.ensure((per * nkeys * 2 + 1) / 2 + 1);
@@ -732,10 +696,6 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
       ops_remember(c, std::move(keys), fillP, 1);
     }
   } else {
-    c.mt32.ensure(size_t(nc) * KP);
-    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc) * KP, 256)), dim3(256), 0, s, c.P * c.P, KP, nc, c.M.p,
-                       c.mt32.p, (__bf16 *)nullptr, (__bf16 *)nullptr);
-    XF_LAUNCH();
     const size_t per = size_t(g.K4) * g.R * 64;
     if (!ops_cached(c, keys, fillP, 0)) {
       c.ops32.ensure(per * nkeys);
@@ -744,6 +704,36 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
       XF_LAUNCH();
       ops_remember(c, std::move(keys), fillP, 0);
     }
+  }
+  c.m2l_ntiles = ntiles;
+  c.m2l_ready = true;
+}
+
+// Cast the multipoles and run the keyed GEMM (after m2l_prepare, on s0).
+void m2l_fast(Ctx &tc, const Ctx &sc) {
+  if (&tc != &sc) fail(EXAFMM_ERR_UNSUPPORTED, "keyed M2L needs source == target tree");
+  Ctx &c = tc;
+  if (!c.m2l_ready) m2l_prepare(c, c.s0);
+  hipStream_t s = c.s0;
+  const Geom g = geom(c.P);
+  const bool bf16 = c.cfg.m2l_precision == EXAFMM_PREC_BF16X3;
+  const int nc = int(c.t.ncells);
+  const int ntiles = c.m2l_ntiles;
+  c.m2l_ready = false;  // tiles belong to this horizontal pass
+  if (ntiles == 0) return;
+  const int KP = g.K4 * 4, KPb = g.K32 * 32;
+  if (bf16) {
+    c.mt32.ensure((size_t(nc) * KPb * 2 + 1) / 2 + 1);  // two bf16 arrays
+    __bf16 *mhi = reinterpret_cast<__bf16 *>(c.mt32.p);
+    __bf16 *mlo = mhi + size_t(nc) * KPb;
+    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc) * KPb, 256)), dim3(256), 0, s, c.P * c.P, KPb, nc, c.M.p,
+                       (float *)nullptr, mhi, mlo);
+    XF_LAUNCH();
+  } else {
+    c.mt32.ensure(size_t(nc) * KP);
+    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc) * KP, 256)), dim3(256), 0, s, c.P * c.P, KP, nc, c.M.p,
+                       c.mt32.p, (__bf16 *)nullptr, (__bf16 *)nullptr);
+    XF_LAUNCH();
   }
 
   // 3. GEMM over tiles

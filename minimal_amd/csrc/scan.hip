@@ -1,46 +1,140 @@
 // scan.hip -- device-wide exclusive scan and stable LSD radix sort.
+#include <algorithm>
+
 #include "fmm_internal.h"
 
 namespace xf {
 
-// Single-workgroup exclusive scan: each of the 1024 threads scans a
-// contiguous chunk, chunk sums are scanned in LDS.  Used for the small
-// per-level arrays (<= a few 10^5 entries) where one launch beats a
-// multi-pass scan.  count comes from count_dev when non-null.
-__global__ void __launch_bounds__(1024) k_scan1(const int *__restrict__ in, int *__restrict__ out, int count_host,
-                                                const int *__restrict__ count_dev, int *__restrict__ total) {
-  __shared__ int part[1024];
-  const int count = count_dev ? *count_dev : count_host;
-  const int t = threadIdx.x;
-  const int chunk = (count + 1023) / 1024;
-  const int b = t * chunk, e = min(b + chunk, count);
-  int s = 0;
-  for (int i = b; i < e; i++) s += in[i];
-  part[t] = s;
+// Device-wide exclusive scan of NA int arrays of the same length in two
+// launches: each of G workgroups reduces a contiguous range (coalesced
+// 1024-element tiles), then each rescans its range starting from the sum of
+// the ranges before it.  The length may live on the device (n_dev); the
+// grid is sized from a host upper bound and idle workgroups exit.  base
+// (optional) is added to every output and receives the totals; cap/flag
+// (optional) raise bit 1 of *flag when a total exceeds cap[k].
+constexpr int kScanThreads = 1024;
+constexpr int kScanMaxBlocks = 64;
+
+template <int NA>
+struct ScanArrays {
+  const int *in[NA];
+  int *out[NA];
+};
+
+__device__ inline int block_sum(int v, int *red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d);
+  if (lane == 0) red[w] = v;
   __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v = t >= off ? part[t - off] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = t ? part[t - 1] : 0;
-  for (int i = b; i < e; i++) {
-    const int v = in[i];
-    out[i] = run;
-    run += v;
-  }
-  if (t == 1023 && total) *total = part[1023];
+  int s = 0;
+  for (int k = 0; k < kScanThreads / 64; k++) s += red[k];
+  __syncthreads();
+  return s;
 }
 
-void scan_exclusive(const int *in, int *out, int count, int *total, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, in, out, count, nullptr, total);
+__device__ inline void scan_range(int n, int G, int &b0, int &e) {
+  int chunk = (n + G - 1) / G;
+  chunk = (chunk + kScanThreads - 1) / kScanThreads * kScanThreads;
+  b0 = blockIdx.x * chunk;
+  e = min(b0 + chunk, n);
+}
+
+template <int NA>
+__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(ScanArrays<NA> a, int n_host, const int *n_dev,
+                                                              int *__restrict__ sums, const int *base) {
+  __shared__ int red[kScanThreads / 64];
+  const int n = n_dev ? *n_dev : n_host;
+  const int G = gridDim.x;
+  int b0, e;
+  scan_range(n, G, b0, e);
+#pragma unroll
+  for (int k = 0; k < NA; k++) {
+    int v = 0;
+    for (int i = b0 + threadIdx.x; i < e; i += kScanThreads) v += a.in[k][i];
+    v = block_sum(v, red);
+    if (threadIdx.x == 0) sums[blockIdx.x * NA + k] = v;
+  }
+  // keep the incoming running totals where the apply pass reads them
+  if (blockIdx.x == 0 && threadIdx.x < NA) sums[kScanMaxBlocks * NA + threadIdx.x] = base ? base[threadIdx.x] : 0;
+}
+
+template <int NA>
+__global__ void __launch_bounds__(kScanThreads) k_scan_apply(ScanArrays<NA> a, int n_host, const int *n_dev,
+                                                             const int *__restrict__ sums, int *total, int *base,
+                                                             const int64_t *cap, int *flag) {
+  __shared__ int red[kScanThreads / 64];
+  __shared__ int wtot[kScanThreads / 64];
+  const int n = n_dev ? *n_dev : n_host;
+  const int G = gridDim.x;
+  int b0, e;
+  scan_range(n, G, b0, e);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NA; k++) {
+    int carry = sums[kScanMaxBlocks * NA + k];
+    {
+      int v = 0;
+      for (int j = threadIdx.x; j < blockIdx.x; j += kScanThreads) v += sums[j * NA + k];
+      carry += block_sum(v, red);
+    }
+    for (int t0 = b0; t0 < e; t0 += kScanThreads) {
+      const int i = t0 + threadIdx.x;
+      const int v = i < e ? a.in[k][i] : 0;
+      int incl = v;
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int u = __shfl_up(incl, d);
+        if (lane >= d) incl += u;
+      }
+      if (lane == 63) wtot[w] = incl;
+      __syncthreads();
+      int before = carry;
+      int tile = 0;
+      for (int q = 0; q < kScanThreads / 64; q++) {
+        if (q < w) before += wtot[q];
+        tile += wtot[q];
+      }
+      if (i < e) a.out[k][i] = before + incl - v;
+      carry += tile;
+      __syncthreads();
+    }
+    if (blockIdx.x == G - 1 && threadIdx.x == 0) {
+      if (total) total[k] = carry;
+      if (base) base[k] = carry;
+      if (cap && flag && carry > cap[k]) atomicOr(flag, 1);
+    }
+  }
+}
+
+template <int NA>
+static void scan_launch(const ScanArrays<NA> &a, int n_host, const int *n_dev, int n_bound, int *total, int *base,
+                        const int64_t *cap, int *flag, Buf<int> &scratch, hipStream_t s) {
+  const int G = std::max(1, std::min(kScanMaxBlocks, ceil_div(std::max(n_bound, 1), 8 * kScanThreads)));
+  scratch.ensure(size_t(kScanMaxBlocks + 1) * NA);
+  hipLaunchKernelGGL(k_scan_reduce<NA>, dim3(G), dim3(kScanThreads), 0, s, a, n_host, n_dev, scratch.p,
+                     (const int *)base);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_scan_apply<NA>, dim3(G), dim3(kScanThreads), 0, s, a, n_host, n_dev, scratch.p, total, base,
+                     cap, flag);
   XF_LAUNCH();
 }
 
-void scan_exclusive_dev(const int *in, int *out, const int *count_dev, int *total, hipStream_t s) {
-  hipLaunchKernelGGL(k_scan1, dim3(1), dim3(1024), 0, s, in, out, 0, count_dev, total);
-  XF_LAUNCH();
+void scan_exclusive(const int *in, int *out, int count, int *total, Buf<int> &scratch, hipStream_t s) {
+  ScanArrays<1> a{{in}, {out}};
+  scan_launch<1>(a, count, nullptr, count, total, nullptr, nullptr, nullptr, scratch, s);
+}
+
+void scan_exclusive_dev(const int *in, int *out, const int *count_dev, int count_bound, int *total,
+                        Buf<int> &scratch, hipStream_t s) {
+  ScanArrays<1> a{{in}, {out}};
+  scan_launch<1>(a, 0, count_dev, count_bound, total, nullptr, nullptr, nullptr, scratch, s);
+}
+
+void scan_exclusive3(const int *in0, const int *in1, const int *in2, int *out0, int *out1, int *out2, int count,
+                     int *running, const int64_t *cap, int *flag, Buf<int> &scratch, hipStream_t s) {
+  ScanArrays<3> a{{in0, in1, in2}, {out0, out1, out2}};
+  scan_launch<3>(a, count, nullptr, count, nullptr, running, cap, flag, scratch, s);
 }
 
 // ---------------------------------------------------------------- radix sort

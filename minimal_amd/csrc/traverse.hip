@@ -234,80 +234,6 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
 }
 
 // per-level offsets: exclusive scans of the three counts + running totals
-__global__ void __launch_bounds__(1024) k_trav_scan(int lstart, int lcount, const int *__restrict__ m2l_cnt,
-                                                    const int *__restrict__ p2p_cnt,
-                                                    const int *__restrict__ out_cnt, int *__restrict__ m2l_beg,
-                                                    int *__restrict__ p2p_beg, int *__restrict__ out_beg,
-                                                    int *__restrict__ tot, int64_t cap_m2l, int64_t cap_p2p,
-                                                    int64_t cap_out) {
-  __shared__ int part[3][1024];
-  const int t = threadIdx.x;
-  const int chunk = (lcount + 1023) / 1024;
-  const int b = t * chunk, e = min(b + chunk, lcount);
-  int s0 = 0, s1 = 0, s2 = 0;
-  for (int i0 = b; i0 < e; i0 += 8) {  // eight loads in flight per counter
-    int v0[8], v1[8], v2[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const bool in = i0 + k < e;
-      v0[k] = in ? m2l_cnt[lstart + i0 + k] : 0;
-      v1[k] = in ? p2p_cnt[lstart + i0 + k] : 0;
-      v2[k] = in ? out_cnt[lstart + i0 + k] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      s0 += v0[k];
-      s1 += v1[k];
-      s2 += v2[k];
-    }
-  }
-  part[0][t] = s0;
-  part[1][t] = s1;
-  part[2][t] = s2;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    int v0 = t >= off ? part[0][t - off] : 0;
-    int v1 = t >= off ? part[1][t - off] : 0;
-    int v2 = t >= off ? part[2][t - off] : 0;
-    __syncthreads();
-    part[0][t] += v0;
-    part[1][t] += v1;
-    part[2][t] += v2;
-    __syncthreads();
-  }
-  int r0 = tot[0] + (t ? part[0][t - 1] : 0);
-  int r1 = tot[1] + (t ? part[1][t - 1] : 0);
-  int r2 = tot[2] + (t ? part[2][t - 1] : 0);
-  for (int i0 = b; i0 < e; i0 += 8) {
-    int v0[8], v1[8], v2[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const bool in = i0 + k < e;
-      v0[k] = in ? m2l_cnt[lstart + i0 + k] : 0;
-      v1[k] = in ? p2p_cnt[lstart + i0 + k] : 0;
-      v2[k] = in ? out_cnt[lstart + i0 + k] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      if (i0 + k < e) {
-        m2l_beg[lstart + i0 + k] = r0;
-        p2p_beg[lstart + i0 + k] = r1;
-        out_beg[lstart + i0 + k] = r2;
-      }
-      r0 += v0[k];
-      r1 += v1[k];
-      r2 += v2[k];
-    }
-  }
-  __syncthreads();
-  if (t == 0) {
-    tot[0] += part[0][1023];
-    tot[1] += part[1][1023];
-    tot[2] += part[2][1023];
-    if (tot[0] > cap_m2l || tot[1] > cap_p2p || tot[2] > cap_out) tot[3] |= 1;
-  }
-}
-
 __global__ void k_hash_clear(unsigned long long *slots, int n) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) slots[i] = kEmpty;
@@ -360,6 +286,9 @@ void traverse(Ctx &tc, const Ctx &sc) {
     Lr.p2p.ensure(Lr.cap_p2p);
     Lr.out.ensure(Lr.cap_out);
     XF_HIP(hipMemsetAsync(Lr.tot.p, 0, 8 * sizeof(int), s));
+    const int64_t caps[3] = {Lr.cap_m2l, Lr.cap_p2p, Lr.cap_out};
+    Lr.capd.ensure(3);
+    XF_HIP(hipMemcpyAsync(Lr.capd.p, caps, sizeof(caps), hipMemcpyHostToDevice, s));
     if (keys) {
       Lr.hslots.ensure(Lr.hcap);
       hipLaunchKernelGGL(k_hash_clear, dim3(ceil_div(Lr.hcap, 256)), dim3(256), 0, s, Lr.hslots.p, Lr.hcap);
@@ -402,10 +331,9 @@ void traverse(Ctx &tc, const Ctx &sc) {
       const int g = ceil_div(a.lcount, kTravWaves);
       hipLaunchKernelGGL(k_traverse<false>, dim3(g), dim3(kTravWaves * kWave), 0, s, a);
       XF_LAUNCH();
-      hipLaunchKernelGGL(k_trav_scan, dim3(1), dim3(1024), 0, s, a.lstart, a.lcount, Lr.m2l_cnt.p, Lr.p2p_cnt.p,
-                         Lr.out_cnt.p, Lr.m2l_beg.p, Lr.p2p_beg.p, Lr.out_beg.p, Lr.tot.p, Lr.cap_m2l, Lr.cap_p2p,
-                         Lr.cap_out);
-      XF_LAUNCH();
+      scan_exclusive3(Lr.m2l_cnt.p + a.lstart, Lr.p2p_cnt.p + a.lstart, Lr.out_cnt.p + a.lstart,
+                      Lr.m2l_beg.p + a.lstart, Lr.p2p_beg.p + a.lstart, Lr.out_beg.p + a.lstart, a.lcount, Lr.tot.p,
+                      Lr.capd.p, Lr.tot.p + 3, Lr.scan_tmp, s);
       hipLaunchKernelGGL(k_traverse<true>, dim3(g), dim3(kTravWaves * kWave), 0, s, a);
       XF_LAUNCH();
     }

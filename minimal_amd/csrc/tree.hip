@@ -438,7 +438,7 @@ void tree_build(Ctx &c) {
     hipLaunchKernelGGL(k_split, dim3(g), dim3(256), 0, s, l, ncrit, t.lcount.p, t.lstart_d.p, t.cbeg.p, t.ccnt.p, k1,
                        t.tmp_nch.p, t.tmp_bnd.p);
     XF_LAUNCH();
-    scan_exclusive_dev(t.tmp_nch.p, t.tmp_off.p, t.lcount.p + l, t.misc.p + 16, s);
+    scan_exclusive_dev(t.tmp_nch.p, t.tmp_off.p, t.lcount.p + l, int(ub[l]), t.misc.p + 16, t.scan_tmp, s);
     hipLaunchKernelGGL(k_level_next, dim3(1), dim3(1), 0, s, l, t.lcount.p, t.lstart_d.p, t.misc.p + 16);
     XF_LAUNCH();
     hipLaunchKernelGGL(k_emit, dim3(g), dim3(256), 0, s, l, t.lcount.p, t.lstart_d.p, t.tmp_nch.p, t.tmp_off.p,
@@ -473,7 +473,7 @@ void tree_build(Ctx &c) {
   t.leaves.ensure(nc);
   hipLaunchKernelGGL(k_leaf_flag, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, t.cnch.p, t.tmp_nch.p);
   XF_LAUNCH();
-  scan_exclusive(t.tmp_nch.p, t.tmp_off.p, nc, t.misc.p + 17, s);
+  scan_exclusive(t.tmp_nch.p, t.tmp_off.p, nc, t.misc.p + 17, t.scan_tmp, s);
   hipLaunchKernelGGL(k_leaf_compact, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, t.cnch.p, t.tmp_off.p,
                      t.leaves.p);
   XF_LAUNCH();
