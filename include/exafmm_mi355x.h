@@ -90,7 +90,8 @@ typedef struct exafmm_config {
   double cycle;           /* periodic cycle (images >= 1) */
   int32_t order_policy;   /* enum exafmm_order_policy */
   int32_t device;         /* HIP device ordinal */
-  int32_t reserved[6];
+  int32_t autotune;       /* 1: time the keyed-M2L GEMM shapes once per shape class and keep the fastest */
+  int32_t reserved[5];
 } exafmm_config;
 
 typedef struct exafmm_ctx exafmm_ctx;
@@ -185,6 +186,39 @@ int exafmm_l2l(exafmm_ctx *ctx, const double cparent[3], const double *Lparent, 
                double *Lchild);
 /* L2P(Ci) (3d/kernel.h:293-322): bodies p, F += local expansion. */
 int exafmm_l2p(exafmm_ctx *ctx, const double center[3], const double *L, exafmm_body *bodies, int64_t nb);
+
+/* ---- learned per-pair expansion order (no counterpart in the reference,
+ * which uses one order P for every M2L pair, 3d/kernel.h:233-260).
+ * Order model: MLP 4 -> 32 -> 32 -> 11 (ReLU), EXAFMM_MLP_PARAMS FP32
+ * master weights, evaluated with BF16-rounded weights and FP32
+ * accumulation.  Inputs per pair: log2(Rs/Rt), log2(|d|/(Rt+Rs)),
+ * level/8, log10 share of the source multipole norm in degrees >= 2.
+ * Outputs: predicted log10 truncation error at orders 4..14 (normalised
+ * by sum|q_source|/|d|).  With order_policy = EXAFMM_ORDER_LEARNED and a
+ * model set, each M2L pair runs at the smallest order whose prediction is
+ * <= log10_tol, at most the context's P (which is then the maximum order of
+ * the expansions); without a model the context's P is used for every pair. */
+#define EXAFMM_MLP_PARAMS 1579
+/* Trains the order model on the device from the context's current
+ * evaluation (P = 14, after build_tree, upward and horizontal): nsamples M2L
+ * pairs are drawn from the lists, their errors at orders 4..14 measured in
+ * FP64 against the direct sum, and the MLP fitted with Adam for `epochs`
+ * full-batch steps.  *loss = final mean squared error (log10 units). */
+int exafmm_order_model_train(exafmm_ctx *ctx, int nsamples, int epochs, double lr, uint64_t seed, double *loss);
+int exafmm_set_order_model(exafmm_ctx *ctx, const float *weights, int count, double log10_tol);
+int exafmm_get_order_model(exafmm_ctx *ctx, float *weights, int count);
+/* The last training set: X [n][4], T [n][11] (NaN = no data) and, when
+ * pairs is non-null, pairs [n][3] = (target cell, source cell, image) in
+ * the numbering of exafmm_get_cells (image 13 = no periodic shift). */
+int exafmm_order_training_set(exafmm_ctx *ctx, float *X, float *T, int32_t *pairs, int64_t capacity, int64_t *n);
+/* Device forward pass / loss gradient of the order MLP on host arrays
+ * (loss = sum over samples and non-NaN outputs of (y - t)^2 / n). */
+int exafmm_mlp_forward(exafmm_ctx *ctx, const float *weights, const float *X, int64_t n, float *Y);
+int exafmm_mlp_gradient(exafmm_ctx *ctx, const float *weights, const float *X, const float *T, int64_t n,
+                        float *grad, float *loss);
+/* Orders used by the last keyed M2L: hist[p] = number of (target, key)
+ * slots of the GEMM tiles run at order p (p < capacity). */
+int exafmm_order_stats(exafmm_ctx *ctx, int64_t *hist, int capacity);
 
 #ifdef __cplusplus
 }

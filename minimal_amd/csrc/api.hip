@@ -175,6 +175,7 @@ int exafmm_config_default(exafmm_config *cfg) {
   cfg->images = 0;
   cfg->cycle = 0;
   cfg->order_policy = EXAFMM_ORDER_FIXED;
+  cfg->autotune = 1;
   cfg->device = 0;
   return EXAFMM_OK;
 }
@@ -506,6 +507,97 @@ int exafmm_get_locals(exafmm_ctx *ctx, double *out) {
     if (!ctx || !out) fail(EXAFMM_ERR_ARG, "null argument");
     if (!ctx->c.have_L) fail(EXAFMM_ERR_STATE, "no locals");
     export_coef(ctx, ctx->c.L, true, out);
+  });
+}
+
+int exafmm_order_model_train(exafmm_ctx *ctx, int nsamples, int epochs, double lr, uint64_t seed, double *loss) {
+  return guard(ctx, [&] {
+    if (!ctx || nsamples < 1 || epochs < 0 || !(lr > 0)) fail(EXAFMM_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    XF_HIP(hipStreamSynchronize(c.s1));
+    const float l = order_train(c, nsamples, epochs, float(lr), seed);
+    if (loss) *loss = l;
+  });
+}
+
+int exafmm_set_order_model(exafmm_ctx *ctx, const float *weights, int count, double log10_tol) {
+  return guard(ctx, [&] {
+    if (!ctx || !weights || count != kMlpParams || !std::isfinite(log10_tol)) fail(EXAFMM_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    c.mlp_w.ensure(kMlpParams);
+    XF_HIP(hipMemcpyAsync(c.mlp_w.p, weights, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, c.s0));
+    XF_HIP(hipStreamSynchronize(c.s0));
+    c.order_log_tol = float(log10_tol);
+    c.have_model = true;
+  });
+}
+
+int exafmm_get_order_model(exafmm_ctx *ctx, float *weights, int count) {
+  return guard(ctx, [&] {
+    if (!ctx || !weights || count != kMlpParams) fail(EXAFMM_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    if (!c.have_model) fail(EXAFMM_ERR_STATE, "no order model");
+    XF_HIP(hipMemcpyAsync(weights, c.mlp_w.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, c.s0));
+    XF_HIP(hipStreamSynchronize(c.s0));
+  });
+}
+
+int exafmm_order_training_set(exafmm_ctx *ctx, float *X, float *T, int32_t *pairs, int64_t capacity, int64_t *n) {
+  return guard(ctx, [&] {
+    if (!ctx || !n) fail(EXAFMM_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    *n = c.train_n;
+    if (!X || !T) return;
+    if (capacity < c.train_n) fail(EXAFMM_ERR_CAPACITY, "training set larger than capacity");
+    if (pairs) {
+      std::vector<int64_t> b2r, r2b;
+      tree_ref_order(c, b2r, r2b);
+      for (int64_t i = 0; i < c.train_n; i++) {
+        pairs[3 * i] = int32_t(b2r[c.train_pairs[3 * i]]);
+        pairs[3 * i + 1] = int32_t(b2r[c.train_pairs[3 * i + 1]]);
+        pairs[3 * i + 2] = c.train_pairs[3 * i + 2];
+      }
+    }
+    XF_HIP(hipMemcpyAsync(X, c.train_X.p, sizeof(float) * c.train_n * kMlpIn, hipMemcpyDeviceToHost, c.s0));
+    XF_HIP(hipMemcpyAsync(T, c.train_T.p, sizeof(float) * c.train_n * kMlpOut, hipMemcpyDeviceToHost, c.s0));
+    XF_HIP(hipStreamSynchronize(c.s0));
+  });
+}
+
+int exafmm_mlp_forward(exafmm_ctx *ctx, const float *weights, const float *X, int64_t n, float *Y) {
+  return guard(ctx, [&] {
+    if (!ctx || !weights || !X || !Y || n < 0) fail(EXAFMM_ERR_ARG, "bad argument");
+    if (n) mlp_eval_host(ctx->c, weights, X, n, Y);
+  });
+}
+
+int exafmm_mlp_gradient(exafmm_ctx *ctx, const float *weights, const float *X, const float *T, int64_t n,
+                        float *grad, float *loss) {
+  return guard(ctx, [&] {
+    if (!ctx || !weights || !X || !T || !grad || !loss || n < 1) fail(EXAFMM_ERR_ARG, "bad argument");
+    mlp_grad_host(ctx->c, weights, X, T, n, grad, loss);
+  });
+}
+
+int exafmm_order_stats(exafmm_ctx *ctx, int64_t *hist, int capacity) {
+  return guard(ctx, [&] {
+    if (!ctx || !hist || capacity < 1) fail(EXAFMM_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    XF_HIP(hipStreamSynchronize(c.s1));
+    XF_HIP(hipStreamSynchronize(c.s0));
+    for (int i = 0; i < capacity; i++) hist[i] = 0;
+    const int nt = c.m2l_ntiles_last, U = c.m2l_totalU;
+    if (nt == 0 || U == 0) return;
+    std::vector<int> tU(nt);
+    XF_HIP(hipMemcpy(tU.data(), c.tiles.p + 3 * nt, sizeof(int) * nt, hipMemcpyDeviceToHost));
+    std::vector<unsigned char> tp(U, (unsigned char)c.P);
+    if (c.learned) XF_HIP(hipMemcpy(tp.data(), c.tile_p.p, U, hipMemcpyDeviceToHost));
+    std::vector<int> cnt(nt);
+    XF_HIP(hipMemcpy(cnt.data(), c.tiles.p + nt, sizeof(int) * nt, hipMemcpyDeviceToHost));
+    int64_t u = 0;
+    for (int t = 0; t < nt; t++)
+      for (int k = 0; k < tU[t]; k++, u++)
+        if (tp[u] < capacity) hist[tp[u]] += cnt[t];
   });
 }
 

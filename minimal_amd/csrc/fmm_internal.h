@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -154,7 +155,23 @@ struct Ctx {
   // keyed M2L tiles built by m2l_prepare for the current lists
   bool m2l_ready = false;
   int m2l_ntiles = 0;
-  Buf<int> m2l_misc, m2l_rs_hist, m2l_scan;  // scratch of m2l_prepare (may run on the side stream)
+  int m2l_ncell_tiled = 0;  // cells that receive M2L (the tiles' columns)
+  Buf<int> m2l_misc, m2l_rs_hist, m2l_scan;
+  Buf<float> m2l_part;      // FP32 partial locals of the key slices [slice][cell][P^2]
+  // variable-order M2L: order of each (tile, key), parallel to tile_keys
+  bool learned = false;
+  int m2l_totalU = 0;
+  int m2l_ntiles_last = 0;  // tiles of the last keyed M2L (statistics)
+  std::map<long long, int> gemm_tuned;  // autotuned GEMM shape (rows, keys per round, slices) per class
+  Buf<double> L_tune;                   // scratch locals of the autotuning runs
+  Buf<unsigned char> tile_p;
+  // learned order model (order.hip): FP32 master weights, tolerance
+  bool have_model = false;
+  float order_log_tol = -5.0f;
+  Buf<float> mlp_w, mlp_part, mshape;
+  int train_n = 0;
+  Buf<float> train_X, train_T;
+  std::vector<int> train_pairs;  // (target cell, source cell, image) per sample, internal numbering  // scratch of m2l_prepare (may run on the side stream)
   int ops_P = -1, ops_fill = -1, ops_bf16 = -1;
   double ops_R0 = 0, ops_cycle = 0;
   Buf<float> mt32;                 // scaled multipoles, real form [ncells][P^2]
@@ -176,7 +193,15 @@ void tree_ref_order(Ctx &c, std::vector<int64_t> &bfs2ref, std::vector<int64_t> 
 void traverse(Ctx &tgt, const Ctx &src);
 void upward(Ctx &c);
 void m2l_apply(Ctx &tgt, const Ctx &src);
-void m2l_prepare(Ctx &c, hipStream_t s);  // keyed-M2L tiles and operators (fast precisions only)
+void m2l_prepare(Ctx &c, hipStream_t s);
+// learned order (order.hip)
+constexpr int kMlpIn = 4, kMlpH = 32, kMlpOut = 11, kMlpPmin = 4;
+constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpOut * kMlpH + kMlpOut;
+void order_select(Ctx &c, const int *tfirst, const int *tcount, const int *tbase, const int *tU, const int *sorted,
+                  int ntiles, int totalU, int tn, hipStream_t s);
+float order_train(Ctx &c, int nsamples, int epochs, float lr, uint64_t seed);
+void mlp_eval_host(Ctx &c, const float *w, const float *X, int64_t n, float *Y);
+void mlp_grad_host(Ctx &c, const float *w, const float *X, const float *T, int64_t n, float *grad, float *loss);  // keyed-M2L tiles and operators (fast precisions only)
 void p2p_apply(Ctx &tgt, const Ctx &src, hipStream_t s);
 void shift_ops(Ctx &c);
 void l2l_pass(Ctx &c);

@@ -115,4 +115,65 @@ __device__ inline void singular_column(int m, int fillP, double dx, double dy, d
   }
 }
 
+// Geometry of an M2L translation key: offset d in units of the finer
+// radius rf (periodic image applied with the cycle), and the radius
+// exponents a, b with R_target = rf 2^a, R_source = rf 2^b.  Key layout:
+// (dx+32768)<<48 | (dy+32768)<<32 | (dz+32768)<<16 | image<<11 | level<<6 |
+// (level difference + 32).
+__device__ inline void key_at(const unsigned long long *keys, int id, double R0, double cycle, double &dx,
+                              double &dy, double &dz, int &a, int &b) {
+  const unsigned long long k = keys[id];
+  dx = double((long long)((k >> 48) & 0xFFFF) - 32768);
+  dy = double((long long)((k >> 32) & 0xFFFF) - 32768);
+  dz = double((long long)((k >> 16) & 0xFFFF) - 32768);
+  const int img = int((k >> 11) & 31), lf = int((k >> 6) & 31);
+  const int ld = int(k & 63) - 32;  // level(target) - level(source)
+  a = ld < 0 ? -ld : 0;
+  b = ld > 0 ? ld : 0;
+  if (img != 13) {
+    const double rf = R0 / (1 << lf);
+    dx -= ((img % 3) - 1) * cycle / rf;
+    dy -= (((img / 3) % 3) - 1) * cycle / rf;
+    dz -= ((img / 9) - 1) * cycle / rf;
+  }
+}
+
+// Operator entry T[(j,k,pr)][(n,m,pc)] of one key (harmonics I in LDS).
+__device__ inline double op_entry(int P, int fillP, int row, int col, int a, int b, const cpx<double> *I) {
+  const int PP = P * P;
+  if (row >= PP || col >= PP) return 0.0;
+  int j = 0;
+  while ((j + 1) * (j + 1) <= row) j++;
+  int rem = row - j * j;
+  const int k = (rem + 1) / 2;
+  const int pr = (rem > 0 && rem % 2 == 0) ? 1 : 0;
+  int n = 0;
+  while ((n + 1) * (n + 1) <= col) n++;
+  rem = col - n * n;
+  const int m = (rem + 1) / 2;
+  const int pc = (rem > 0 && rem % 2 == 0) ? 1 : 0;
+  if (j + n >= fillP) return 0.0;
+  const double s = ldexp(1.0, a * (j + 1) + b * n);
+  const double cj = sgn(j);
+  const double cp = cj * sgn(((k - m) * (k < m)) + m);
+  // Gp = I_{j+n}^{m-k} (conjugated for negative order)
+  cpx<double> gp = I[nms(j + n, abs(m - k))];
+  if (m - k < 0) gp.im = -gp.im;
+  gp.re *= s;
+  gp.im *= s;
+  if (m == 0) {
+    if (pc == 1) return 0.0;
+    return pr == 0 ? cp * gp.re : cp * gp.im;
+  }
+  // Gm = conj(I_{j+n}^{m+k})
+  cpx<double> gm = I[nms(j + n, m + k)];
+  gm.im = -gm.im;
+  gm.re *= s;
+  gm.im *= s;
+  if (pr == 0 && pc == 0) return cp * gp.re + cj * gm.re;
+  if (pr == 0 && pc == 1) return -cp * gp.im + cj * gm.im;
+  if (pr == 1 && pc == 0) return cp * gp.im + cj * gm.im;
+  return cp * gp.re - cj * gm.re;
+}
+
 }  // namespace xfd

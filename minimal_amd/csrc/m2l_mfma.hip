@@ -211,66 +211,6 @@ __global__ void __launch_bounds__(256) k_mt(int PP, int KP, int ncells, const do
   }
 }
 
-// Decodes a translation key (layout of m2l_key in traverse.hip) into the
-// offset d = dX / Rf in units of the finer radius and the radius ratios
-// Rc = Rf 2^a, Rs = Rf 2^b.  For a periodic image the shift iX * cycle is
-// subtracted in the same units (Rf = R0 / 2^level).
-__device__ inline void key_at(const unsigned long long *keys, int id, double R0, double cycle, double &dx,
-                              double &dy, double &dz, int &a, int &b) {
-  const unsigned long long k = keys[id];
-  dx = double((long long)((k >> 48) & 0xFFFF) - 32768);
-  dy = double((long long)((k >> 32) & 0xFFFF) - 32768);
-  dz = double((long long)((k >> 16) & 0xFFFF) - 32768);
-  const int img = int((k >> 11) & 31), lf = int((k >> 6) & 31);
-  const int ld = int(k & 63) - 32;  // level(target) - level(source)
-  a = ld < 0 ? -ld : 0;
-  b = ld > 0 ? ld : 0;
-  if (img != 13) {
-    const double rf = R0 / (1 << lf);
-    dx -= ((img % 3) - 1) * cycle / rf;
-    dy -= (((img / 3) % 3) - 1) * cycle / rf;
-    dz -= ((img / 9) - 1) * cycle / rf;
-  }
-}
-
-// Operator entry T[(j,k,pr)][(n,m,pc)] of one key (harmonics I in LDS).
-__device__ inline double op_entry(int P, int fillP, int row, int col, int a, int b, const cpx<double> *I) {
-  const int PP = P * P;
-  if (row >= PP || col >= PP) return 0.0;
-  int j = 0;
-  while ((j + 1) * (j + 1) <= row) j++;
-  int rem = row - j * j;
-  const int k = (rem + 1) / 2;
-  const int pr = (rem > 0 && rem % 2 == 0) ? 1 : 0;
-  int n = 0;
-  while ((n + 1) * (n + 1) <= col) n++;
-  rem = col - n * n;
-  const int m = (rem + 1) / 2;
-  const int pc = (rem > 0 && rem % 2 == 0) ? 1 : 0;
-  if (j + n >= fillP) return 0.0;
-  const double s = ldexp(1.0, a * (j + 1) + b * n);
-  const double cj = sgn(j);
-  const double cp = cj * sgn(((k - m) * (k < m)) + m);
-  // Gp = I_{j+n}^{m-k} (conjugated for negative order)
-  cpx<double> gp = I[nms(j + n, abs(m - k))];
-  if (m - k < 0) gp.im = -gp.im;
-  gp.re *= s;
-  gp.im *= s;
-  if (m == 0) {
-    if (pc == 1) return 0.0;
-    return pr == 0 ? cp * gp.re : cp * gp.im;
-  }
-  // Gm = conj(I_{j+n}^{m+k})
-  cpx<double> gm = I[nms(j + n, m + k)];
-  gm.im = -gm.im;
-  gm.re *= s;
-  gm.im *= s;
-  if (pr == 0 && pc == 0) return cp * gp.re + cj * gm.re;
-  if (pr == 0 && pc == 1) return -cp * gp.im + cj * gm.im;
-  if (pr == 1 && pc == 0) return cp * gp.im + cj * gm.im;
-  return cp * gp.re - cj * gm.re;
-}
-
 // One workgroup per key: singular harmonics of d in FP64, then the operator
 // in MFMA fragment order.
 //  FP32 (16x16x4):  op32[key][kk][r][lane], element (row, kcol):
@@ -343,6 +283,39 @@ __device__ inline void write_locals(int PP, int rbase, int nr, const int *cells,
   }
 }
 
+// Split-key variant: this workgroup's share of the locals, stored (each
+// (cell, row) of a tile belongs to one workgroup per key slice).
+template <int RH>
+__device__ inline void write_partial(int PP, int rbase, int nr, const int *cells, int ncol, int cg,
+                                     const v4f (&acc)[RH][4], int lane, float *Lp) {
+#pragma unroll
+  for (int ct = 0; ct < 4; ct++) {
+    const int col = cg * 64 + ct * 16 + (lane & 15);
+    if (col >= ncol) continue;
+    float *Lc = Lp + int64_t(cells[col]) * PP;
+#pragma unroll
+    for (int r = 0; r < RH; r++) {
+      if (r >= nr) break;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = (rbase + r) * 16 + 4 * (lane >> 4) + i;
+        if (row < PP) Lc[row] = acc[r][ct][i];
+      }
+    }
+  }
+}
+
+// Adds the key slices of the split-key GEMM into the locals (fixed order).
+__global__ void k_combine_slices(int PP, int64_t ncells, int nslice, const int *__restrict__ sorted, int ncell_tiled,
+                                 const float *__restrict__ Lpart, double *__restrict__ L) {
+  const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (g >= int64_t(ncell_tiled) * PP) return;
+  const int c = sorted[g / PP], r = int(g % PP);
+  double s = 0.0;
+  for (int z = 0; z < nslice; z++) s += double(Lpart[(int64_t(z) * ncells + c) * PP + r]);
+  L[int64_t(c) * PP + r] += s;
+}
+
 // Parity 1 hands its accumulators to parity 0 through LDS (the operator
 // buffers are free by then); parity 0 adds them and writes the locals.
 template <int RH>
@@ -380,12 +353,19 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
                                                           const int *__restrict__ tkeys,
                                                           const int *__restrict__ tmat,
                                                           const float *__restrict__ op32,
-                                                          const float *__restrict__ mt, double *__restrict__ L) {
+                                                          const float *__restrict__ mt,
+                                                          const unsigned char *__restrict__ tkp, int kr,
+                                                          int64_t ncells, float *__restrict__ Lpart,
+                                                          double *__restrict__ L) {
   extern __shared__ float As[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int cg = w & 3, kp = w >> 2;
   const int tile = blockIdx.x;
-  const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
+  const int ncol = tcount[tile];
+  // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
+  const int Ut = tU[tile], ub = int(int64_t(Ut) * blockIdx.z / gridDim.z),
+            ue = int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
+  const int U = ue - ub, base = tbase[tile] + ub;
   const int rt0 = blockIdx.y * RTB;
   const int nrt = min(RTB, RT - rt0);
   const int kq = lane >> 4;
@@ -397,55 +377,80 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
   const int64_t per = int64_t(K4) * RT * 64;
   const int NC = K4 * nrt;  // runs of 64 floats per staged block
   const int BUF = NC * 64;  // floats per LDS buffer
+  // order of key u (tkp: per-key orders of a variable-order evaluation):
+  // k-steps and row tiles of this block that it needs, and its K bound
+  auto kcols = [&](int u) { const int p = tkp ? tkp[base + u] : (int)__builtin_sqrtf(float(PP) + 0.5f); return p * p; };
+  auto ksteps = [&](int kc) { return (kc + 3) / 4; };
+  auto rtiles = [&](int kc) { return max(0, min(nrt, (kc + 15) / 16 - rt0)); };
   if (U > 0) {
     auto issue = [&](int u, float *dst) {
+      const int kc = kcols(u), ks = ksteps(kc), nr = rtiles(kc);
       const float *Ak = op32 + int64_t(tkeys[base + u]) * per;
-      for (int c = w; c < NC; c += kGemmThreads / 64)
-        __builtin_amdgcn_global_load_lds(Ak + int64_t(op_run(c, nrt, RT, rt0)) * 64 + lane, dst + c * 64, 4, 0, 0);
+      for (int c = w; c < NC; c += kGemmThreads / 64) {
+        const int kk = c / nrt, rl = c % nrt;
+        if (kk < ks && rl < nr)
+          __builtin_amdgcn_global_load_lds(Ak + int64_t(op_run(c, nrt, RT, rt0)) * 64 + lane, dst + c * 64, 4, 0, 0);
+      }
     };
-    auto load_b = [&](const int (&sc)[4], int kk, float (&bv)[4]) {
+    auto load_b = [&](const int (&sc)[4], int kk, int kc, float (&bv)[4]) {
+      const int k = kk * 4 + kq;
 #pragma unroll
-      for (int ct = 0; ct < 4; ct++) bv[ct] = sc[ct] < 0 ? 0.0f : mt[int64_t(sc[ct]) * KP + kk * 4 + kq];
+      for (int ct = 0; ct < 4; ct++) bv[ct] = (sc[ct] < 0 || k >= kc) ? 0.0f : mt[int64_t(sc[ct]) * KP + k];
     };
+    auto slot = [&](int u) { return As + size_t(((u / kr) & 1) * kr + (u % kr)) * BUF; };
     int src[4], nsrc[4];
     float bcur[4], bnext[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) src[ct] = tmat[int64_t(base) * kTN + cg * 64 + ct * 16 + (lane & 15)];
-    if (kp < K4) load_b(src, kp, bnext);
-    issue(0, As);
+    int kc = kcols(0);
+    if (kp < ksteps(kc)) load_b(src, kp, kc, bnext);
+    for (int u = 0; u < min(kr, U); u++) issue(u, slot(u));
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (int u = 0; u < U; u++) {
-      const float *Acur = As + (u & 1) * BUF;
-      if (u + 1 < U) {
-        issue(u + 1, As + ((u + 1) & 1) * BUF);
-#pragma unroll
-        for (int ct = 0; ct < 4; ct++) nsrc[ct] = tmat[(int64_t(base) + u + 1) * kTN + cg * 64 + ct * 16 + (lane & 15)];
-      }
-      for (int kk = kp; kk < K4; kk += 2) {
-#pragma unroll
-        for (int ct = 0; ct < 4; ct++) bcur[ct] = bnext[ct];
-        if (kk + 2 < K4)
-          load_b(src, kk + 2, bnext);
-        else if (u + 1 < U && kp < K4)
-          load_b(nsrc, kp, bnext);
-#pragma unroll
-        for (int r = 0; r < RH; r++) {
-          if (r >= nrt) break;
-          const float a = Acur[(kk * nrt + r) * 64 + lane];
+    for (int u0 = 0; u0 < U; u0 += kr) {
+      for (int u = u0 + kr; u < min(u0 + 2 * kr, U); u++) issue(u, slot(u));
+      for (int u = u0; u < min(u0 + kr, U); u++) {
+        const float *Acur = slot(u);
+        const int ks = ksteps(kc), nr = rtiles(kc);
+        const int kc_next = u + 1 < U ? kcols(u + 1) : 0;
+        if (u + 1 < U) {
 #pragma unroll
           for (int ct = 0; ct < 4; ct++)
-            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bcur[ct], acc[r][ct], 0, 0, 0);
+            nsrc[ct] = tmat[(int64_t(base) + u + 1) * kTN + cg * 64 + ct * 16 + (lane & 15)];
         }
-      }
+        for (int kk = kp; kk < ks; kk += 2) {
 #pragma unroll
-      for (int ct = 0; ct < 4; ct++) src[ct] = nsrc[ct];
-      __builtin_amdgcn_s_waitcnt(0);  // next operator block landed in LDS
-      __syncthreads();               // and everyone is done with this one
+          for (int ct = 0; ct < 4; ct++) bcur[ct] = bnext[ct];
+          if (kk + 2 < ks)
+            load_b(src, kk + 2, kc, bnext);
+          else if (u + 1 < U && kp < ksteps(kc_next))
+            load_b(nsrc, kp, kc_next, bnext);
+#pragma unroll
+          for (int r = 0; r < RH; r++) {
+            if (r >= nr) break;
+            const float a = Acur[(kk * nrt + r) * 64 + lane];
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++)
+              acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, bcur[ct], acc[r][ct], 0, 0, 0);
+          }
+        }
+        if (ks <= kp && u + 1 < U && kp < ksteps(kc_next)) load_b(nsrc, kp, kc_next, bnext);
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) src[ct] = nsrc[ct];
+        kc = kc_next;
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // next round's operator blocks landed in LDS
+      __syncthreads();               // and everyone is done with this round's
     }
   }
   reduce_parities<RH>(acc, kp, cg, lane, As);
-  if (kp == 0) write_locals<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane, L);
+  if (kp == 0) {
+    if (Lpart)
+      write_partial<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane,
+                        Lpart + int64_t(blockIdx.z) * ncells * PP);
+    else
+      write_locals<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane, L);
+  }
 }
 
 template <int RH>
@@ -461,12 +466,18 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
                                                              const v8bf *__restrict__ oplo,
                                                              const __bf16 *__restrict__ mhi,
                                                              const __bf16 *__restrict__ mlo,
+                                                             const unsigned char *__restrict__ tkp, int kr,
+                                                             int64_t ncells, float *__restrict__ Lpart,
                                                              double *__restrict__ L) {
   extern __shared__ v8bf Ab[];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int cg = w & 3, kp = w >> 2;
   const int tile = blockIdx.x;
-  const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
+  const int ncol = tcount[tile];
+  // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
+  const int Ut = tU[tile], ub = int(int64_t(Ut) * blockIdx.z / gridDim.z),
+            ue = int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
+  const int U = ue - ub, base = tbase[tile] + ub;
   const int rt0 = blockIdx.y * RTB;
   const int nrt = min(RTB, RT - rt0);
   const int kq = lane >> 4;
@@ -479,104 +490,142 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
   const int NC = K32 * nrt;                    // runs of 64 v8bf per staged part
   const int E = NC * 64;                       // v8bf per staged part
   const v8bf zero = {};
+  // order of key u (tkp: per-key orders of a variable-order evaluation):
+  // k-steps and row tiles of this block that it needs, and its K bound
+  auto kcols = [&](int u) { const int p = tkp ? tkp[base + u] : (int)__builtin_sqrtf(float(PP) + 0.5f); return p * p; };
+  auto ksteps = [&](int kc) { return (kc + 31) / 32; };
+  auto rtiles = [&](int kc) { return max(0, min(nrt, (kc + 15) / 16 - rt0)); };
   if (U > 0) {
     // buffer b: hi part at Ab + 2 b E, lo part at Ab + (2 b + 1) E
     auto issue = [&](int u, v8bf *dst) {
+      const int kc = kcols(u), ks = ksteps(kc), nr = rtiles(kc);
       const int64_t k = tkeys[base + u];
       for (int c = w; c < 2 * NC; c += kGemmThreads / 64) {
         const int part = c / NC, cc = c % NC;
+        const int kk = cc / nrt, rl = cc % nrt;
+        if (kk >= ks || rl >= nr) continue;
         const v8bf *A = (part ? oplo : ophi) + k * per;
         __builtin_amdgcn_global_load_lds(A + int64_t(op_run(cc, nrt, RT, rt0)) * 64 + lane,
                                          dst + part * E + cc * 64, 16, 0, 0);
       }
     };
-    // columns past P^2 are zero in the cast multipoles: not fetched
-    auto load_b = [&](const int (&sc)[4], int kk, v8bf (&bh)[4], v8bf (&bl)[4]) {
+    // columns at or past the key's K bound are zero (not fetched; the
+    // boundary vector is masked element by element)
+    auto load_b = [&](const int (&sc)[4], int kk, int kc, v8bf (&bh)[4], v8bf (&bl)[4]) {
       const int k0 = kk * 32 + kq * 8;
 #pragma unroll
       for (int ct = 0; ct < 4; ct++) {
-        const bool have = sc[ct] >= 0 && k0 < PP;
+        const bool have = sc[ct] >= 0 && k0 < kc;
         const int64_t o = int64_t(have ? sc[ct] : 0) * KP + k0;
         bh[ct] = have ? *reinterpret_cast<const v8bf *>(mhi + o) : zero;
         bl[ct] = have ? *reinterpret_cast<const v8bf *>(mlo + o) : zero;
+        if (have && k0 + 8 > kc) {
+#pragma unroll
+          for (int i = 0; i < 8; i++)
+            if (k0 + i >= kc) {
+              bh[ct][i] = __bf16(0.0f);
+              bl[ct][i] = __bf16(0.0f);
+            }
+        }
       }
     };
+    // rounds of kr keys: a round's operator blocks are staged together
+    // (slot i of buffer b at Ab + (b kr + i) 2E), one wait + barrier per round
+    auto slot = [&](int u) { return Ab + size_t(((u / kr) & 1) * kr + (u % kr)) * 2 * E; };
     int src[4], nsrc[4];
     v8bf bh[4], bl[4], nbh[4], nbl[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) src[ct] = tmat[int64_t(base) * kTN + cg * 64 + ct * 16 + (lane & 15)];
-    if (kp < K32) load_b(src, kp, nbh, nbl);
-    issue(0, Ab);
+    int kc = kcols(0);
+    if (kp < ksteps(kc)) load_b(src, kp, kc, nbh, nbl);
+    for (int u = 0; u < min(kr, U); u++) issue(u, slot(u));
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (int u = 0; u < U; u++) {
-      const v8bf *Ahs = Ab + 2 * (u & 1) * E, *Als = Ahs + E;
-      if (u + 1 < U) {
-        issue(u + 1, Ab + 2 * ((u + 1) & 1) * E);
+    for (int u0 = 0; u0 < U; u0 += kr) {
+      for (int u = u0 + kr; u < min(u0 + 2 * kr, U); u++) issue(u, slot(u));
+      for (int u = u0; u < min(u0 + kr, U); u++) {
+        const v8bf *Ahs = slot(u), *Als = Ahs + E;
+        const int ks = ksteps(kc), nr = rtiles(kc);
+        const int kc_next = u + 1 < U ? kcols(u + 1) : 0;
+        if (u + 1 < U) {
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++) nsrc[ct] = tmat[(int64_t(base) + u + 1) * kTN + cg * 64 + ct * 16 + (lane & 15)];
-      }
-      for (int kk = kp; kk < K32; kk += 2) {
-#pragma unroll
-        for (int ct = 0; ct < 4; ct++) {
-          bh[ct] = nbh[ct];
-          bl[ct] = nbl[ct];
+          for (int ct = 0; ct < 4; ct++)
+            nsrc[ct] = tmat[(int64_t(base) + u + 1) * kTN + cg * 64 + ct * 16 + (lane & 15)];
         }
-        if (kk + 2 < K32)
-          load_b(src, kk + 2, nbh, nbl);
-        else if (u + 1 < U && kp < K32)
-          load_b(nsrc, kp, nbh, nbl);
-#pragma unroll
-        for (int r = 0; r < RH; r++) {
-          if (r >= nrt) break;
-          const int ai = (kk * nrt + r) * 64 + lane;
-          const v8bf a_hi = Ahs[ai], a_lo = Als[ai];
+        for (int kk = kp; kk < ks; kk += 2) {
 #pragma unroll
           for (int ct = 0; ct < 4; ct++) {
-            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, bh[ct], acc[r][ct], 0, 0, 0);
-            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, bl[ct], acc[r][ct], 0, 0, 0);
-            acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, bh[ct], acc[r][ct], 0, 0, 0);
+            bh[ct] = nbh[ct];
+            bl[ct] = nbl[ct];
+          }
+          if (kk + 2 < ks)
+            load_b(src, kk + 2, kc, nbh, nbl);
+          else if (u + 1 < U && kp < ksteps(kc_next))
+            load_b(nsrc, kp, kc_next, nbh, nbl);
+#pragma unroll
+          for (int r = 0; r < RH; r++) {
+            if (r >= nr) break;
+            const int ai = (kk * nrt + r) * 64 + lane;
+            const v8bf a_hi = Ahs[ai], a_lo = Als[ai];
+#pragma unroll
+            for (int ct = 0; ct < 4; ct++) {
+              acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, bh[ct], acc[r][ct], 0, 0, 0);
+              acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, bl[ct], acc[r][ct], 0, 0, 0);
+              acc[r][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, bh[ct], acc[r][ct], 0, 0, 0);
+            }
           }
         }
-      }
+        if (ks <= kp && u + 1 < U && kp < ksteps(kc_next)) load_b(nsrc, kp, kc_next, nbh, nbl);
 #pragma unroll
-      for (int ct = 0; ct < 4; ct++) src[ct] = nsrc[ct];
-      __builtin_amdgcn_s_waitcnt(0);  // next operator block landed in LDS
-      __syncthreads();               // and everyone is done with this one
+        for (int ct = 0; ct < 4; ct++) src[ct] = nsrc[ct];
+        kc = kc_next;
+      }
+      __builtin_amdgcn_s_waitcnt(0);  // next round's operator blocks landed in LDS
+      __syncthreads();               // and everyone is done with this round's
     }
   }
   reduce_parities<RH>(acc, kp, cg, lane, Ab);
-  if (kp == 0) write_locals<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane, L);
+  if (kp == 0) {
+    if (Lpart)
+      write_partial<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane,
+                        Lpart + int64_t(blockIdx.z) * ncells * PP);
+    else
+      write_locals<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane, L);
+  }
 }
 
 // Row tiles per workgroup: two operator buffers within the LDS budget, at
 // most kGemmMaxRT accumulator row tiles per wave, and room for the parity
 // reduction (4 x RTB x 4 KB).
-static int rows_per_block(const Geom &g, bool bf16, int device) {
+static int rows_per_block(const Geom &g, bool bf16, int device, int kr) {
   int max_lds = 0;
   XF_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
-  const int per_rt = 2 * (bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4);  // both buffers
+  const int per_rt = 2 * kr * (bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4);  // both buffers of kr keys
   const int need = std::max(per_rt, 4 * 4 * 64 * 16);                   // or the reduction
-  if (need > max_lds) fail(EXAFMM_ERR_UNSUPPORTED, "expansion order too large for the LDS operator stage");
+  if (need > max_lds) {
+    if (kr == 1) fail(EXAFMM_ERR_UNSUPPORTED, "expansion order too large for the LDS operator stage");
+    return 0;
+  }
   int rtb = std::min({g.R, kGemmMaxRT, max_lds / need});
   // balance the row blocks
   const int nblk = ceil_div(g.R, rtb);
   return ceil_div(g.R, nblk);
 }
 
-static size_t gemm_lds(const Geom &g, bool bf16, int RTB) {
-  const size_t ops = size_t(2) * RTB * (bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4);
+static size_t gemm_lds(const Geom &g, bool bf16, int RTB, int kr) {
+  const size_t ops = size_t(2) * kr * RTB * (bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4);
   const size_t red = size_t(4) * RTB * 4 * 64 * 16;
   return std::max(ops, red);
 }
 
 template <int RH>
-static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, int RTB) {
+static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, int RTB,
+                        int kr, int ksplit, float *Lpart, double *Lout) {
   Ctx &c = tc;
   const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ntiles, *tbase = c.tiles.p + 2 * ntiles,
             *tU = c.tiles.p + 3 * ntiles;
   const int *sorted = c.tiles.p + 4 * ntiles;
-  const dim3 grid(ntiles, ceil_div(g.R, RTB));
+  const dim3 grid(ntiles, ceil_div(g.R, RTB), ksplit);
   if (bf16) {
     const v8bf *ophi = reinterpret_cast<const v8bf *>(c.ops32.p);
     const size_t per = size_t(g.K32) * g.R * 64 * 8;
@@ -584,19 +633,121 @@ static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool 
                                                       per * c.lists.nkeys);
     const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
     const __bf16 *mlo = mhi + size_t(sc.t.ncells) * KPb;
-    const size_t lds = gemm_lds(g, true, RTB);
+    const size_t lds = gemm_lds(g, true, RTB, kr);
     XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3<RH>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL((k_m2l_bf16x3<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, tfirst,
-                       tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi, mlo, c.L.p);
+                       tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi, mlo,
+                       c.learned ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), ksplit > 1 ? Lpart : nullptr,
+                       Lout);
   } else {
-    const size_t lds = gemm_lds(g, false, RTB);
+    const size_t lds = gemm_lds(g, false, RTB, kr);
     XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_f32<RH>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL((k_m2l_f32<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K4, KP, tfirst,
-                       tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, c.ops32.p, sc.mt32.p, c.L.p);
+                       tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, c.ops32.p, sc.mt32.p,
+                       c.learned ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), ksplit > 1 ? Lpart : nullptr,
+                       Lout);
   }
   XF_LAUNCH();
+}
+
+struct GemmShape {
+  int rtb, kr, ksplit;
+};
+
+static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, GemmShape sh,
+                     double *Lout) {
+  float *part = nullptr;
+  if (sh.ksplit > 1) {
+    c.m2l_part.ensure(size_t(sh.ksplit) * c.t.ncells * g.PP);
+    part = c.m2l_part.p;
+  }
+  switch (sh.rtb) {
+#define XF_GEMM_CASE(n) \
+  case n: launch_gemm<n>(c, sc, g, ntiles, bf16, KP, KPb, sh.rtb, sh.kr, sh.ksplit, part, Lout); break;
+    XF_GEMM_CASE(1)
+    XF_GEMM_CASE(2)
+    XF_GEMM_CASE(3)
+    XF_GEMM_CASE(4)
+    XF_GEMM_CASE(5)
+    XF_GEMM_CASE(6)
+    XF_GEMM_CASE(7)
+    XF_GEMM_CASE(8)
+#undef XF_GEMM_CASE
+    default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
+  }
+  if (sh.ksplit > 1) {
+    const int64_t n = int64_t(c.m2l_ncell_tiled) * g.PP;
+    hipLaunchKernelGGL(k_combine_slices, dim3(ceil_div(n, 256)), dim3(256), 0, c.s0, g.PP, int64_t(c.t.ncells),
+                       sh.ksplit, c.tiles.p + 4 * ntiles, c.m2l_ncell_tiled, part, Lout);
+    XF_LAUNCH();
+  }
+}
+
+// Autotuner of the GEMM shape: rows of the operator per workgroup (the
+// rows of a tile may be split over several workgroups: more workgroups
+// when tiles are few, more re-reads of the source multipoles), keys per
+// LDS round (more keys per barrier hide the operator-copy latency of short,
+// low-order keys) and key slices per tile (a tile's keys split over
+// several workgroups whose FP32 partial locals a second kernel adds in a
+// fixed order: more parallelism when tiles are few and keys many).  The first time a (order, precision, order policy, tile
+// count class) is seen, every candidate runs once into a scratch copy of
+// the locals, alone on the device, timed with HIP events; the fastest is
+// kept for the context's lifetime.
+static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb) {
+  int tclass = 0;
+  while ((1 << tclass) < ntiles) tclass++;
+  const long long key = ((long long)g.P << 24) | ((long long)bf16 << 20) | ((long long)c.learned << 16) | tclass;
+  auto it = c.gemm_tuned.find(key);
+  if (it != c.gemm_tuned.end())
+    return GemmShape{int(it->second & 255), int((it->second >> 8) & 255), int(it->second >> 16)};
+  std::vector<GemmShape> cand;
+  for (int kr = 1; kr <= 3; kr++) {
+    const int rmax = rows_per_block(g, bf16, c.cfg.device, kr);
+    if (rmax < 1) continue;
+    int last = -1;
+    for (int nb = ceil_div(g.R, rmax); nb <= ceil_div(g.R, rmax) + 2; nb++) {
+      const int r = ceil_div(g.R, nb);
+      if (r != last && r <= rmax) {
+        cand.push_back(GemmShape{r, kr, 1});
+        // key slices only where the workgroups would not fill the device
+        const int wg = ntiles * nb;
+        for (int ks = 2; ks <= 4; ks *= 2)
+          if (wg * ks <= 4096 && c.m2l_totalU >= 8 * ks * ntiles) cand.push_back(GemmShape{r, kr, ks});
+      }
+      last = r;
+    }
+  }
+  GemmShape best = cand[0];
+  if (cand.size() > 1 && c.cfg.autotune) {
+    hipStream_t s = c.s0;
+    XF_HIP(hipStreamSynchronize(c.s1));  // tune alone on the device
+    c.L_tune.ensure(size_t(c.t.ncells) * c.P * c.P);
+    int kmax = 1;
+    for (const GemmShape &sh : cand) kmax = std::max(kmax, sh.ksplit);
+    if (kmax > 1) c.m2l_part.ensure(size_t(kmax) * c.t.ncells * g.PP);  // no allocation inside the timings
+    hipEvent_t e0, e1;
+    XF_HIP(hipEventCreate(&e0));
+    XF_HIP(hipEventCreate(&e1));
+    float bt = 1e30f;
+    for (const GemmShape &sh : cand) {
+      XF_HIP(hipEventRecord(e0, s));
+      run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L_tune.p);
+      XF_HIP(hipEventRecord(e1, s));
+      XF_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      XF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < bt) {
+        bt = ms;
+        best = sh;
+      }
+    }
+    XF_HIP(hipEventDestroy(e0));
+    XF_HIP(hipEventDestroy(e1));
+  }
+  c.gemm_tuned[key] = best.rtb | (best.kr << 8) | (best.ksplit << 16);
+  return best;
 }
 
 static bool ops_cached(const Ctx &c, const std::vector<unsigned long long> &keys, int fillP, int bf16) {
@@ -706,6 +857,8 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
     }
   }
   c.m2l_ntiles = ntiles;
+  c.m2l_ncell_tiled = off;
+  c.m2l_totalU = totalU;
   c.m2l_ready = true;
 }
 
@@ -719,6 +872,7 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   const bool bf16 = c.cfg.m2l_precision == EXAFMM_PREC_BF16X3;
   const int nc = int(c.t.ncells);
   const int ntiles = c.m2l_ntiles;
+  c.m2l_ntiles_last = ntiles;
   c.m2l_ready = false;  // tiles belong to this horizontal pass
   if (ntiles == 0) return;
   const int KP = g.K4 * 4, KPb = g.K32 * 32;
@@ -736,20 +890,16 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
     XF_LAUNCH();
   }
 
-  // 3. GEMM over tiles
+  // per-key orders of a learned-order evaluation (need the multipoles)
+  c.learned = c.cfg.order_policy == EXAFMM_ORDER_LEARNED && c.have_model;
+  if (c.learned)
+    order_select(c, c.tiles.p, c.tiles.p + ntiles, c.tiles.p + 2 * ntiles, c.tiles.p + 3 * ntiles,
+                 c.tiles.p + 4 * ntiles, ntiles, c.m2l_totalU, kTN, s);
+
+  // 3. GEMM over tiles, shape from the autotuner
+  const GemmShape sh = gemm_shape(c, sc, g, ntiles, bf16, KP, KPb);
   XF_HIP(hipEventRecord(c.ev[13], s));
-  const int RTB = rows_per_block(g, bf16, c.cfg.device);
-  switch (RTB) {
-    case 1: launch_gemm<1>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
-    case 2: launch_gemm<2>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
-    case 3: launch_gemm<3>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
-    case 4: launch_gemm<4>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
-    case 5: launch_gemm<5>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
-    case 6: launch_gemm<6>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
-    case 7: launch_gemm<7>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
-    case 8: launch_gemm<8>(c, sc, g, ntiles, bf16, KP, KPb, RTB); break;
-    default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
-  }
+  run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L.p);
 }
 
 }  // namespace xf

@@ -38,15 +38,21 @@ EXPORTS = [
     "exafmm_horizontal", "exafmm_downward", "exafmm_evaluate", "exafmm_direct", "exafmm_list_sizes",
     "exafmm_get_lists", "exafmm_get_multipoles", "exafmm_get_locals", "exafmm_stage_times", "exafmm_p2p",
     "exafmm_p2m", "exafmm_m2m", "exafmm_m2l", "exafmm_l2l", "exafmm_l2p", "exafmm_work_counts",
-    "exafmm_dipole_correction", "exafmm_ewald",
+    "exafmm_dipole_correction", "exafmm_ewald", "exafmm_order_model_train", "exafmm_set_order_model",
+    "exafmm_get_order_model", "exafmm_order_training_set", "exafmm_mlp_forward", "exafmm_mlp_gradient",
+    "exafmm_order_stats",
 ]
+
+MLP_PARAMS = 1579  # EXAFMM_MLP_PARAMS
+MLP_IN, MLP_OUT, ORDER_MIN = 4, 11, 4
+ORDER_FIXED, ORDER_LEARNED = 0, 1
 
 
 class Config(C.Structure):
     _fields_ = [("P", C.c_int32), ("ncrit", C.c_int32), ("theta", C.c_double), ("m2l_truncation", C.c_int32),
                 ("m2l_precision", C.c_int32), ("p2p_precision", C.c_int32), ("images", C.c_int32),
                 ("cycle", C.c_double), ("order_policy", C.c_int32), ("device", C.c_int32),
-                ("reserved", C.c_int32 * 6)]
+                ("autotune", C.c_int32), ("reserved", C.c_int32 * 5)]
 
 
 class ExaFMMError(RuntimeError):
@@ -94,6 +100,13 @@ class Library:
         L.exafmm_m2l.argtypes = [_P, _P, _P, _P, _P]
         L.exafmm_l2l.argtypes = [_P, _P, _P, _P, _P]
         L.exafmm_l2p.argtypes = [_P, _P, _P, _P, _L]
+        L.exafmm_order_model_train.argtypes = [_P, _I, _I, _D, C.c_uint64, _P]
+        L.exafmm_set_order_model.argtypes = [_P, _P, _I, _D]
+        L.exafmm_get_order_model.argtypes = [_P, _P, _I]
+        L.exafmm_order_training_set.argtypes = [_P, _P, _P, _P, _L, C.POINTER(C.c_int64)]
+        L.exafmm_mlp_forward.argtypes = [_P, _P, _P, _L, _P]
+        L.exafmm_mlp_gradient.argtypes = [_P, _P, _P, _P, _L, _P, _P]
+        L.exafmm_order_stats.argtypes = [_P, _P, _I]
 
     def device_count(self) -> int:
         n = C.c_int(0)
@@ -241,6 +254,53 @@ class FMM:
         keys = ["m2l_pairs", "p2p_pairs", "p2p_interactions", "p2p_source_bodies", "cells", "leaves", "m2l_keys",
                 "depth"]
         return {k: int(x) for k, x in zip(keys, v)}
+
+    # ---- learned order
+    def train_order_model(self, nsamples=16384, epochs=400, lr=3e-3, seed=1) -> float:
+        loss = np.zeros(1)
+        self._chk(self.L.lib.exafmm_order_model_train(self.h, nsamples, epochs, lr, seed, _ptr(loss)))
+        return float(loss[0])
+
+    def set_order_model(self, weights, log10_tol):
+        w = np.ascontiguousarray(weights, np.float32)
+        self._chk(self.L.lib.exafmm_set_order_model(self.h, _ptr(w), w.size, float(log10_tol)))
+
+    def order_model(self) -> np.ndarray:
+        w = np.zeros(MLP_PARAMS, np.float32)
+        self._chk(self.L.lib.exafmm_get_order_model(self.h, _ptr(w), MLP_PARAMS))
+        return w
+
+    def order_training_set(self):
+        """(X [n,4], T [n,11], pairs [n,3] = target cell, source cell, image)."""
+        n = C.c_int64()
+        self._chk(self.L.lib.exafmm_order_training_set(self.h, None, None, None, 0, C.byref(n)))
+        X = np.zeros((n.value, MLP_IN), np.float32)
+        T = np.zeros((n.value, MLP_OUT), np.float32)
+        pairs = np.zeros((n.value, 3), np.int32)
+        self._chk(self.L.lib.exafmm_order_training_set(self.h, _ptr(X), _ptr(T), _ptr(pairs), n.value,
+                                                       C.byref(n)))
+        return X, T, pairs
+
+    def mlp_forward(self, weights, X) -> np.ndarray:
+        w = np.ascontiguousarray(weights, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        Y = np.zeros((len(X), MLP_OUT), np.float32)
+        self._chk(self.L.lib.exafmm_mlp_forward(self.h, _ptr(w), _ptr(X), len(X), _ptr(Y)))
+        return Y
+
+    def mlp_gradient(self, weights, X, T):
+        w = np.ascontiguousarray(weights, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(MLP_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(self.L.lib.exafmm_mlp_gradient(self.h, _ptr(w), _ptr(X), _ptr(T), len(X), _ptr(g), _ptr(loss)))
+        return g, float(loss[0])
+
+    def order_stats(self) -> np.ndarray:
+        h = np.zeros(16, np.int64)
+        self._chk(self.L.lib.exafmm_order_stats(self.h, _ptr(h), 16))
+        return h
 
     # ---- kernel.h operator surface (host arrays)
     def p2p(self, targets, sources):
