@@ -8,10 +8,24 @@ p and F resident in HBM.  Prints one JSON line.
     python bench.py [--gpus 1] [--steps 10] [--warmup 3] [--workload plain]
 
 Other workloads (one JSON line each):
+  --workload learned    N = 2^22: 90% Plummer (a = 0.05) + 10% uniform,
+                        q ~ U[-1,1); the learned per-pair M2L order (MLP
+                        trained on the device from a separate 2^20-body set
+                        of the same distribution) against the fixed order
+                        12 it falls back to, both timed here
   --workload periodic   N = 2^20 in the unit periodic cube, q = +1/-1
                         alternating, P = 10, images = 3, dipole correction;
                         checked against the FP64 Ewald sum (alpha = 10,
                         ksize = 11, cutoff 0.5) on sampled targets
+  --workload copy       a 256 MiB device copy: the reference kernel that
+                        scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters
+                        to bytes
+
+The roofline of the dominant kernel divides its algorithmic work by its
+average duration over the timed steps, read from the HIP events the library
+records around the launch on its own stream.  HBM traffic per launch, when
+profiles/traffic.json holds it for the workload, comes from rocprofv3 PMC
+runs scaled by the copy calibration.
 """
 from __future__ import annotations
 
@@ -29,6 +43,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "3D Laplace FMM particles/s, N=2^20, P=10, L2 err≤1e-5; learned vs fixed order"
 PEAK = {"fp32": 157.3, "bf16": 2500.0, "hbm": 8000.0}
+P2P_FLOP = 20  # the conventional count per pair interaction (potential + force, rsqrt as one)
 
 
 def log(*a):
@@ -62,10 +77,217 @@ print(json.dumps({{"kind": kind, "times": [float(x) for x in t]}}))
     d = json.loads(r.stdout.strip().splitlines()[-1])
     t_eval = sum(d["times"])
     return {"value": n / t_eval, "unit": "particles/s", "cores": threads, "kind": d["kind"],
-            "sample": f"full config: N={n} uniform [0,1)^3, q~U[-1,1), P={P}, theta={theta}, ncrit={ncrit}; "
-                      f"tree+upward+lists+M2L/P2P+downward in {t_eval:.3f} s "
+            "sample": f"N={n} uniform [0,1)^3, q~U[-1,1), P={P}, theta={theta}, ncrit={ncrit}: one evaluation "
+                      f"(tree+upward+lists+M2L/P2P+downward) in {t_eval:.3f} s "
                       f"({'reference 3d/ lazy traversal, FP64, as written' if d['kind'] == 'reference' else 'FP64 oracle port'})",
             "stage_s": d["times"]}
+
+
+def timed(f, step, steps: int, warmup: int, torch) -> tuple[float, dict]:
+    """Warm up, then time `steps` calls of step() between device
+    synchronisations.  Returns ms per step and the per-stage durations
+    (HIP events recorded by the library on its streams) averaged over the
+    timed steps."""
+    for _ in range(warmup):
+        step()
+    f.synchronize()
+    torch.cuda.synchronize()
+    acc = {}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+        st = f.stage_times()  # waits for both streams of the context
+        for k, v in st.items():
+            acc[k] = acc.get(k, 0.0) + v
+    f.synchronize()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    return ms, {k: v / steps for k, v in acc.items()}
+
+
+def accuracy(f, dev, host, n, ntargets, torch, fmm_mod):
+    """One evaluation of a fresh copy, then the FP64 direct sum on the
+    stride-sampled targets (3d/fmm.cxx:57-61).  Returns (err_p, err_F)."""
+    from minimal_amd.inputs import BODY_DTYPE, rel_l2, sample_targets
+    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+    ptr = dev.data_ptr()
+    f.evaluate(ptr, n=n, on_device=True)
+    f.synchronize()
+    res = dev.cpu().numpy().view(BODY_DTYPE)
+    idx = sample_targets(n, ntargets)
+    tg = res[idx].copy()
+    tg["p"] = 0
+    tg["F"] = 0
+    tdev = torch.from_numpy(tg.view(np.uint8).copy()).cuda()
+    f.direct(tdev.data_ptr(), ptr, nt=len(tg), ns=n, on_device=True)
+    ref = tdev.cpu().numpy().view(BODY_DTYPE)
+    return rel_l2(res["p"][idx], ref["p"]), rel_l2(res["F"][idx], ref["F"]), len(idx)
+
+
+def traffic_for(workload: str, kernel: str):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            t = json.load(fh)
+        e = t["workloads"][workload][kernel]
+        return float(e["fetch_bytes"]) + float(e["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def roofline(workload: str, kern_ms: dict, work: dict, P: int, m2l_prec: str, order_hist=None) -> dict:
+    """Roofline of the longer of the two hot kernels.  P2P: 20 flop per pair
+    interaction against the FP32 vector peak (it is VALU-bound: no MFMA
+    form, no HBM bound at ~250 MB per launch).  M2L: the keyed GEMM's MFMA
+    work, 2 (P^2)^2 flop per M2L pair (3x for the three BF16 products of the
+    split-precision form) against the MFMA peak of that precision."""
+    p2p = {"kernel": "p2p", "bound": "valu", "ms": kern_ms["p2p"],
+           "algorithmic": f"{P2P_FLOP} flop x {work['p2p_interactions']} pair interactions per launch",
+           "achieved": P2P_FLOP * work["p2p_interactions"] / (kern_ms["p2p"] * 1e-3) / 1e12,
+           "peak": PEAK["fp32"], "unit": "TFLOP/s"}
+    p2p["frac"] = p2p["achieved"] / p2p["peak"]
+    p2p["traffic"] = traffic_for(workload, "p2p")
+    flop_pair = 2.0 * P ** 4
+    if order_hist is not None:  # per-pair orders of a learned evaluation: slot-weighted mean of p^4
+        tot = sum(int(c) for c in order_hist)
+        flop_pair = 2.0 * sum(int(c) * p ** 4 for p, c in enumerate(order_hist)) / max(tot, 1)
+    mult = 3.0 if m2l_prec == "bf16x3" else 1.0
+    m2l = {"kernel": "m2l_gemm", "bound": "mfma", "ms": kern_ms["m2l_kernel"],
+           "algorithmic": f"{mult:g} x {flop_pair:.4g} flop x {work['m2l_pairs']} M2L pairs per launch",
+           "achieved": mult * flop_pair * work["m2l_pairs"] / (kern_ms["m2l_kernel"] * 1e-3) / 1e12,
+           "peak": PEAK["bf16"] if m2l_prec == "bf16x3" else PEAK["fp32"], "unit": "TFLOP/s"}
+    m2l["frac"] = m2l["achieved"] / m2l["peak"]
+    m2l["traffic"] = traffic_for(workload, "m2l_gemm")
+    dom, other = (p2p, m2l) if p2p["ms"] >= m2l["ms"] else (m2l, p2p)
+    out = dict(dom)
+    out["other"] = other
+    return out
+
+
+def copy_workload(args, torch):
+    """256 MiB device-to-device copy, three times: the known-size kernel
+    whose FETCH_SIZE / WRITE_SIZE readings scale the counters to bytes."""
+    x = torch.ones(64 << 20, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "copy", "bytes_read_per_launch": x.numel() * 4,
+                      "bytes_written_per_launch": y.numel() * 4, "launches": 3}), flush=True)
+
+
+def plain_workload(args, fmm, torch):
+    from minimal_amd.inputs import uniform_cube
+    prec = {"bf16x3": fmm.PREC_BF16X3, "fp32": fmm.PREC_FP32, "fp64": fmm.PREC_FP64}[args.m2l]
+    trunc = fmm.M2L_FULL if args.truncation == "full" else fmm.M2L_REFERENCE
+    n = args.n or (1 << 20)
+    host = uniform_cube(n, 0, "sym")
+    dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
+    ptr = dev.data_ptr()
+    f = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=trunc, m2l_precision=prec,
+                p2p_precision=fmm.PREC_FP32)
+    log(f"bench: N={n} P={args.P} m2l={args.m2l} truncation={args.truncation}")
+    ms, kern = timed(f, lambda: f.evaluate(ptr, n=n, on_device=True), args.steps, args.warmup, torch)
+    work = f.work_counts()
+    log("stages ms (mean of timed steps):", json.dumps(kern), "work:", json.dumps(work))
+    err_p, err_F, nt = accuracy(f, dev, host, n, args.targets, torch, fmm)
+    log(f"rel L2 vs FP64 direct ({nt} targets): p {err_p:.3e} F {err_F:.3e}")
+    wl = "plain_n2p20" if n == 1 << 20 and args.P == 10 else "plain"
+    roof = roofline(wl, kern, work, args.P, args.m2l)
+    # The timed steps run P2P and the keyed GEMM on two streams at once, so
+    # each kernel's event span includes time the other holds the CUs.  The
+    # same steps with both on one stream give each kernel's own duration.
+    os.environ["EXAFMM_SERIAL_STREAMS"] = "1"
+    try:
+        fi = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=trunc, m2l_precision=prec,
+                     p2p_precision=fmm.PREC_FP32)
+    finally:
+        del os.environ["EXAFMM_SERIAL_STREAMS"]
+    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+    ms_i, kern_i = timed(fi, lambda: fi.evaluate(ptr, n=n, on_device=True), args.steps, args.warmup, torch)
+    ri = roofline(wl, kern_i, work, args.P, args.m2l)
+    roof["one_stream"] = {"ms_per_step": ms_i, "stages_ms": kern_i,
+                          ri["kernel"]: {k: ri[k] for k in ("ms", "achieved", "frac")},
+                          ri["other"]["kernel"]: {k: ri["other"][k] for k in ("ms", "achieved", "frac")}}
+    log(f"one stream: {ms_i:.3f} ms/step, stages {json.dumps(kern_i)}")
+    cpu = None
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        cpu = cpu_baseline(n, args.P, args.ncrit, args.theta, threads)
+    out = {
+        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"plain 3D Laplace FMM, N={n} uniform [0,1)^3, q~U[-1,1), P={args.P}, "
+                               f"theta={args.theta}, ncrit={args.ncrit}",
+                   "m2l_truncation": args.truncation, "m2l_precision": args.m2l, "p2p_precision": "fp32",
+                   "order_policy": "fixed"},
+        "error": {"p": err_p, "F": err_F, "targets": int(nt), "vs": "FP64 direct sum"},
+        "stages_ms": kern, "work": work, "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def learned_workload(args, fmm, torch):
+    """Learned per-pair order against fixed order 12 on the same bodies.
+    The model is trained on the device (exafmm_order_model_train: pair
+    errors from the FP64 truncated expansions vs the exact pair sum, then
+    Adam on the MLP) from a separate body set of the same distribution; the
+    training time is reported, not timed in the steps."""
+    from minimal_amd.inputs import plummer_background
+    n = args.n or (1 << 22)
+    prec = {"bf16x3": fmm.PREC_BF16X3, "fp32": fmm.PREC_FP32}[args.m2l]
+    t0 = time.perf_counter()
+    tr = fmm.FMM(P=14, ncrit=args.ncrit, theta=args.theta, m2l_precision=prec)
+    tr.build_tree(plummer_background(1 << 20, 5))
+    tr.upward()
+    tr.horizontal()
+    loss = tr.train_order_model(nsamples=16384, epochs=400, lr=3e-3)
+    weights = tr.order_model()
+    train_s = time.perf_counter() - t0
+    del tr
+    log(f"trained order model in {train_s:.2f} s, loss {loss:.3f}")
+    host = plummer_background(n, 6)
+    dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
+    ptr = dev.data_ptr()
+    res = {}
+    for name in ("fixed", "learned"):
+        if name == "fixed":
+            f = fmm.FMM(P=args.P_fixed, ncrit=args.ncrit, theta=args.theta, m2l_precision=prec)
+        else:
+            f = fmm.FMM(P=args.P_fixed, ncrit=args.ncrit, theta=args.theta, m2l_precision=prec,
+                        order_policy=fmm.ORDER_LEARNED)
+            f.set_order_model(weights, args.tol)
+        dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+        ms, kern = timed(f, lambda: f.evaluate(ptr, n=n, on_device=True), args.steps, args.warmup, torch)
+        hist = f.order_stats()
+        work = f.work_counts()
+        err_p, err_F, nt = accuracy(f, dev, host, n, args.targets, torch, fmm)
+        log(f"{name}: {ms:.3f} ms/step, err p {err_p:.3e} F {err_F:.3e}, stages {json.dumps(kern)}")
+        res[name] = {"value": n / (ms * 1e-3), "ms_per_step": ms, "error": {"p": err_p, "F": err_F},
+                     "stages_ms": kern, "work": work,
+                     "gemm_slot_orders": {str(p): int(c) for p, c in enumerate(hist) if c},
+                     "roofline": roofline("learned", kern, work, args.P_fixed, args.m2l,
+                                          hist if name == "learned" else None)}
+        del f
+    L, F = res["learned"], res["fixed"]
+    out = {
+        "metric": METRIC, "value": L["value"], "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": L["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
+        "config": {"workload": f"learned per-pair M2L order, N={n}: 90% Plummer (a=0.05, r<=0.5) + 10% uniform, "
+                               f"q~U[-1,1), orders 4..{args.P_fixed}, fallback P={args.P_fixed}, theta={args.theta}, "
+                               f"ncrit={args.ncrit}",
+                   "m2l_precision": args.m2l, "p2p_precision": "fp32", "order_policy": "learned",
+                   "model": "MLP 4-32-32-11, ReLU, BF16 weights, FP32 accumulate", "log10_tolerance": args.tol},
+        "error": {"p": L["error"]["p"], "F": L["error"]["F"], "targets": args.targets, "vs": "FP64 direct sum"},
+        "fixed_order": F, "learned": L,
+        "speedup_vs_fixed": L["value"] / F["value"],
+        "training": {"seconds": train_s, "loss": loss, "pairs": 16384, "epochs": 400,
+                     "set": "2^20 bodies of the same distribution, another seed, P=14"},
+        "roofline": L["roofline"], "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def periodic_workload(args, fmm, torch):
@@ -79,7 +301,7 @@ def periodic_workload(args, fmm, torch):
     per-body correction) and the one global constant by which the two
     summation conventions differ is removed."""
     from minimal_amd.inputs import BODY_DTYPE, rel_l2, sample_targets, uniform_cube
-    n = args.n
+    n = args.n or (1 << 20)
     host = uniform_cube(n, 0, "sym")
     host["q"] = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     cycle = 1.0
@@ -89,19 +311,12 @@ def periodic_workload(args, fmm, torch):
     f = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=fmm.M2L_FULL, m2l_precision=prec,
                 p2p_precision=fmm.PREC_FP32, images=args.images, cycle=cycle)
     log(f"bench periodic: N={n} P={args.P} images={args.images} m2l={args.m2l}")
-    for _ in range(args.warmup):
+
+    def step():
         f.evaluate(ptr, n=n, on_device=True)
         f.dipole_correction()
-    f.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f.evaluate(ptr, n=n, on_device=True)
-        f.dipole_correction()
-    f.synchronize()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    stages = f.stage_times()
+
+    ms, kern = timed(f, step, args.steps, args.warmup, torch)
     work = f.work_counts()
     # accuracy: fresh evaluation without the per-body correction
     dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
@@ -135,7 +350,8 @@ def periodic_workload(args, fmm, torch):
                    "m2l_truncation": "full", "m2l_precision": args.m2l, "p2p_precision": "fp32"},
         "error": {"p": err_p, "F": err_F, "targets": int(len(idx)),
                   "vs": f"FP64 Ewald sum, alpha={alpha}, ksize={ksize}, cutoff={cutoff}"},
-        "stages_ms": stages, "work": work, "roofline": None, "cpu_baseline": None,
+        "stages_ms": kern, "work": work, "roofline": roofline("periodic", kern, work, args.P, args.m2l),
+        "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
 
@@ -145,8 +361,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--n", type=int, default=0, help="bodies (default: the workload's size)")
     ap.add_argument("--P", type=int, default=10)
+    ap.add_argument("--P-fixed", dest="P_fixed", type=int, default=12, help="learned workload: fallback order")
+    ap.add_argument("--tol", type=float, default=-7.5, help="learned workload: log10 pair-error tolerance")
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--m2l", choices=["bf16x3", "fp32", "fp64"], default="bf16x3")
@@ -154,7 +372,7 @@ def main():
     ap.add_argument("--targets", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
-    ap.add_argument("--workload", choices=["plain", "periodic"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "learned", "periodic", "copy"], default="plain")
     ap.add_argument("--images", type=int, default=3)
     args = ap.parse_args()
 
@@ -164,89 +382,12 @@ def main():
         sys.exit(2)
 
     import torch
-    from minimal_amd import fmm
-    from minimal_amd.inputs import BODY_DTYPE, rel_l2, sample_targets, uniform_cube
-
-    if args.workload == "periodic":
-        periodic_workload(args, fmm, torch)
+    if args.workload == "copy":
+        copy_workload(args, torch)
         return
-
-    prec = {"bf16x3": fmm.PREC_BF16X3, "fp32": fmm.PREC_FP32, "fp64": fmm.PREC_FP64}[args.m2l]
-    trunc = fmm.M2L_FULL if args.truncation == "full" else fmm.M2L_REFERENCE
-    n = args.n
-    host = uniform_cube(n, 0, "sym")
-    dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
-    ptr = dev.data_ptr()
-    f = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=trunc, m2l_precision=prec,
-                p2p_precision=fmm.PREC_FP32)
-    log(f"bench: N={n} P={args.P} m2l={args.m2l} truncation={args.truncation}")
-    for _ in range(args.warmup):
-        f.evaluate(ptr, n=n, on_device=True)
-    f.synchronize()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f.evaluate(ptr, n=n, on_device=True)
-    f.synchronize()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms = (t1 - t0) * 1e3 / args.steps
-    stages = f.stage_times()
-    work = f.work_counts()
-    log("stages ms:", json.dumps(stages), "work:", json.dumps(work))
-
-    # accuracy: fresh copy, one evaluation, FP64 direct sum on sampled targets
-    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
-    f.evaluate(ptr, n=n, on_device=True)
-    f.synchronize()
-    res = dev.cpu().numpy().view(BODY_DTYPE)
-    idx = sample_targets(n, args.targets)
-    tg = res[idx].copy()
-    tg["p"] = 0
-    tg["F"] = 0
-    tdev = torch.from_numpy(tg.view(np.uint8).copy()).cuda()
-    f.direct(tdev.data_ptr(), ptr, nt=len(tg), ns=n, on_device=True)
-    ref = tdev.cpu().numpy().view(BODY_DTYPE)
-    err_p = rel_l2(res["p"][idx], ref["p"])
-    err_F = rel_l2(res["F"][idx], ref["F"])
-    log(f"rel L2 vs FP64 direct ({len(idx)} targets): p {err_p:.3e} F {err_F:.3e}")
-
-    # roofline of the dominant kernel
-    p2p_flop = 20.0 * work["p2p_interactions"]
-    m2l_flop = 2.0 * args.P ** 4 * work["m2l_pairs"]
-    m2l_ms = stages["m2l_kernel"] if args.m2l != "fp64" else stages["m2l"]
-    if stages["p2p"] >= m2l_ms:
-        ach = p2p_flop / (stages["p2p"] * 1e-3) / 1e12
-        roof = {"kernel": "p2p", "bound": "valu", "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                "frac": ach / PEAK["fp32"], "traffic": None,
-                "algorithmic": f"20 flop x {work['p2p_interactions']} interactions per launch"}
-    else:
-        if args.m2l == "bf16x3":
-            ach = 3 * m2l_flop / (m2l_ms * 1e-3) / 1e12
-            pk = PEAK["bf16"]
-        else:
-            ach = m2l_flop / (m2l_ms * 1e-3) / 1e12
-            pk = PEAK["fp32"]
-        roof = {"kernel": "m2l", "bound": "mfma", "achieved": ach, "peak": pk, "unit": "TFLOP/s", "frac": ach / pk,
-                "traffic": None, "algorithmic": f"2 P^4 flop x {work['m2l_pairs']} pairs per launch"}
-
-    cpu = None
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(n, args.P, args.ncrit, args.theta, threads)
-
-    out = {
-        "metric": METRIC, "value": n / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32", "data": "synthetic",
-        "config": {"workload": f"plain 3D Laplace FMM, N={n} uniform [0,1)^3, q~U[-1,1), P={args.P}, "
-                               f"theta={args.theta}, ncrit={args.ncrit}",
-                   "m2l_truncation": args.truncation, "m2l_precision": args.m2l, "p2p_precision": "fp32",
-                   "order_policy": "fixed"},
-        "error": {"p": err_p, "F": err_F, "targets": int(len(idx)), "vs": "FP64 direct sum"},
-        "stages_ms": stages, "work": work, "roofline": roof, "cpu_baseline": cpu,
-    }
-    print(json.dumps(out), flush=True)
+    from minimal_amd import fmm
+    {"plain": plain_workload, "learned": learned_workload, "periodic": periodic_workload}[args.workload](
+        args, fmm, torch)
 
 
 if __name__ == "__main__":

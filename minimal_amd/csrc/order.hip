@@ -173,18 +173,29 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ grad, fl
 
 // ---------------------------------------------------------------- features
 // Per cell: log10 of the share of the scaled multipole norm in degrees >= 2.
-__global__ void k_mshape(int PP, int ncells, const double *__restrict__ M, float *__restrict__ shape) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per cell: the shares of the squared real-form coefficients,
+// read coalesced across the lanes and reduced over the wave.
+__global__ void __launch_bounds__(256) k_mshape(int PP, int ncells, const double *__restrict__ M,
+                                                float *__restrict__ shape) {
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ncells) return;
   const double *Mc = M + int64_t(c) * PP;
   double tot = 0.0, high = 0.0;
-  for (int i = 0; i < PP; i++) {
+  for (int i = lane; i < PP; i += 64) {
     const double v = Mc[i] * Mc[i];
     tot += v;
     if (i >= 4) high += v;  // real-form index >= 4 <=> degree >= 2
   }
-  const double r = tot > 0.0 ? sqrt(high / tot) : 0.0;
-  shape[c] = float(fmin(0.0, fmax(-6.0, log10(fmax(r, 1e-30)))));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    tot += __shfl_xor(tot, o, 64);
+    high += __shfl_xor(high, o, 64);
+  }
+  if (lane == 0) {
+    const double r = tot > 0.0 ? sqrt(high / tot) : 0.0;
+    shape[c] = float(fmin(0.0, fmax(-6.0, log10(fmax(r, 1e-30)))));
+  }
 }
 
 __device__ inline void pair_features(double dx, double dy, double dz, int a, int b, int level_t, float shape,
@@ -206,9 +217,10 @@ __device__ inline int order_of(const float (&y)[kMlpOut], float log_tol) {
 // slot share the key geometry and the target level (tiles hold cells of one
 // level); they differ only in the source multipole shape, so the slot is
 // evaluated once with the largest shape among its sources (the pair that
-// needs the highest order).  One thread per slot: the max over the 256
-// columns, one MLP evaluation, the smallest order under the tolerance,
-// capped at the context's order pmax.
+// needs the highest order).  Per chunk of 256 slots of a tile: each wave
+// takes slots in turn and reduces the shapes of the slot's 256 columns
+// (coalesced row reads); then one thread per slot runs the MLP, takes the
+// smallest order under the tolerance and caps it at the context's order.
 __global__ void __launch_bounds__(256) k_order_select(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                       const int *__restrict__ tbase, const int *__restrict__ tU,
                                                       const int *__restrict__ sorted, const int *__restrict__ tkeys,
@@ -218,32 +230,42 @@ __global__ void __launch_bounds__(256) k_order_select(const int *__restrict__ tf
                                                       float log_tol, double R0, double cycle, int tn, int pmax,
                                                       unsigned char *__restrict__ tile_p) {
   __shared__ float W[kMlpParams];
+  __shared__ float smax_s[256];
   load_weights(w, W);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int tile = blockIdx.x;
   const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
   const int lt = clevel[sorted[tfirst[tile]]];
-  for (int u = threadIdx.x; u < U; u += blockDim.x) {
-    const int *row = tmat + (int64_t(base) + u) * tn;
-    float smax = -1e30f;
-    for (int c0 = 0; c0 < ncol; c0 += 8) {
-      int sc[8];
+  for (int u0 = 0; u0 < U; u0 += 256) {
+    const int nu = min(256, U - u0);
+    for (int j = wv; j < nu; j += 4) {
+      const int *row = tmat + (int64_t(base) + u0 + j) * tn;
+      float m = -1e30f;
+      for (int c = lane; c < ncol; c += 64) {
+        const int sc = row[c];
+        if (sc >= 0) m = fmaxf(m, shape[sc]);
+      }
 #pragma unroll
-      for (int k = 0; k < 8; k++) sc[k] = c0 + k < ncol ? row[c0 + k] : -1;
-#pragma unroll
-      for (int k = 0; k < 8; k++)
-        if (sc[k] >= 0) smax = fmaxf(smax, shape[sc[k]]);
+      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+      if (lane == 0) smax_s[j] = m;
     }
-    int p = kMlpPmin;
-    if (smax > -1e29f) {
-      double dx, dy, dz;
-      int a, b;
-      key_at(keys, tkeys[base + u], R0, cycle, dx, dy, dz, a, b);
-      float x[kMlpIn], h1[kMlpH], h2[kMlpH], y[kMlpOut];
-      pair_features(dx, dy, dz, a, b, lt, smax, x);
-      mlp_forward(W, x, h1, h2, y);
-      p = order_of(y, log_tol);
+    __syncthreads();
+    if (int(threadIdx.x) < nu) {
+      const int u = u0 + threadIdx.x;
+      const float smax = smax_s[threadIdx.x];
+      int p = kMlpPmin;
+      if (smax > -1e29f) {
+        double dx, dy, dz;
+        int a, b;
+        key_at(keys, tkeys[base + u], R0, cycle, dx, dy, dz, a, b);
+        float x[kMlpIn], h1[kMlpH], h2[kMlpH], y[kMlpOut];
+        pair_features(dx, dy, dz, a, b, lt, smax, x);
+        mlp_forward(W, x, h1, h2, y);
+        p = order_of(y, log_tol);
+      }
+      tile_p[base + u] = (unsigned char)min(p, pmax);
     }
-    tile_p[base + u] = (unsigned char)min(p, pmax);
+    __syncthreads();
   }
 }
 
@@ -386,7 +408,7 @@ void order_select(Ctx &c, const int *tfirst, const int *tcount, const int *tbase
   c.tile_p.ensure(std::max(totalU, 1));
   const int nc = int(c.t.ncells);
   c.mshape.ensure(nc);
-  hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 256)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, c.mshape.p);
+  hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 4)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, c.mshape.p);
   XF_LAUNCH();
   const double cyc = c.cfg.images > 0 ? c.cfg.cycle : 0.0;
   hipLaunchKernelGGL(k_order_select, dim3(ntiles), dim3(256), 0, s, tfirst, tcount, tbase, tU, sorted,
@@ -436,7 +458,7 @@ float order_train(Ctx &c, int nsamples, int epochs, float lr, uint64_t seed) {
   }
   XF_HIP(hipMemcpyAsync(dpe.p, pe.data(), sizeof(int2) * n, hipMemcpyHostToDevice, s));
   c.mshape.ensure(nc);
-  hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 256)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, c.mshape.p);
+  hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 4)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, c.mshape.p);
   XF_LAUNCH();
   Buf<float> X, T;
   X.ensure(size_t(n) * kMlpIn);

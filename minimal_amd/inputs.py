@@ -124,3 +124,36 @@ def rel_l2(test: np.ndarray, ref: np.ndarray) -> float:
     """sqrt(sum |test-ref|^2 / sum |ref|^2), normalised by the FP64 direct values."""
     d = np.asarray(test, dtype=np.float64) - np.asarray(ref, dtype=np.float64)
     return float(np.sqrt(np.sum(d * d) / np.sum(np.asarray(ref, dtype=np.float64) ** 2)))
+
+
+def plummer_background(n: int, seed: int) -> np.ndarray:
+    """The learned-order workload: 90% of the bodies in a Plummer sphere of
+    scale radius 0.05 centred in the unit cube and truncated at r = 0.5, 10%
+    uniform in [0,1)^3, charges uniform in [-1,1)."""
+    rng = np.random.default_rng(seed)
+    b = np.zeros(n, dtype=BODY_DTYPE)
+    nb = n // 10
+    b["X"][: n - nb] = plummer(n - nb, 0.05, 0.5, rng)
+    b["X"][n - nb:] = rng.random((nb, 3))
+    b["q"] = rng.uniform(-1, 1, n)
+    return b
+
+
+def batched_problems(count: int, seed: int) -> list:
+    """The batched workload: ``count`` independent problems, N log-uniform in
+    [10^3, 10^5], the first half uniform in [0,1)^3 and the second half
+    Plummer spheres (scale radius 0.1, truncated at 10 scale radii), charges
+    U[0,1)/N, expansion order drawn from {6, 8, 12}.  Returns a list of
+    (bodies, P)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(round(10.0 ** rng.uniform(3.0, 5.0)))
+        b = np.zeros(n, dtype=BODY_DTYPE)
+        if i < count // 2:
+            b["X"] = rng.random((n, 3))
+        else:
+            b["X"] = plummer(n, 0.1, 1.0, rng, center=(0.0, 0.0, 0.0))
+        b["q"] = rng.random(n) / n
+        out.append((b, int(rng.choice([6, 8, 12]))))
+    return out
