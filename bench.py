@@ -13,6 +13,9 @@ Other workloads (one JSON line each):
                         trained on the device from a separate 2^20-body set
                         of the same distribution) against the fixed order
                         12 it falls back to, both timed here
+  --workload batched    256 independent problems (N log-uniform in 1e3..1e5,
+                        uniform / Plummer, P in {6, 8, 12}) re-evaluated by
+                        one hipGraph launch per step
   --workload periodic   N = 2^20 in the unit periodic cube, q = +1/-1
                         alternating, P = 10, images = 3, dipole correction;
                         checked against the FP64 Ewald sum (alpha = 10,
@@ -290,6 +293,92 @@ def learned_workload(args, fmm, torch):
     print(json.dumps(out), flush=True)
 
 
+def batched_workload(args, fmm, torch):
+    """256 independent problems (N log-uniform in [10^3, 10^5], half uniform
+    cubes and half Plummer spheres, q = U[0,1)/N, P drawn from {6, 8, 12},
+    theta = 0.4, ncrit = 64), bodies resident in one device buffer.  Each
+    problem is planned once by an ordinary evaluation (tree, lists, keyed-M2L
+    tiles and operators, autotuned GEMM shape); one step is then one
+    hipGraph launch that re-evaluates all 256 problems (every charge-dependent
+    pass, p and F overwritten).  The planning time and one eager pass over
+    all problems (tree build included) are reported beside it."""
+    from minimal_amd.inputs import BODY_DTYPE, batched_problems, rel_l2, sample_targets
+    probs = batched_problems(256, 3)
+    ns = [len(b) for b, _ in probs]
+    offs = np.concatenate([[0], np.cumsum(ns)])
+    N = int(offs[-1])
+    host = np.concatenate([b for b, _ in probs])
+    dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
+    base = dev.data_ptr()
+    log(f"bench batched: {len(probs)} problems, N total {N}, P counts "
+        f"{ {P: sum(1 for _, q in probs if q == P) for P in (6, 8, 12)} }")
+    t0 = time.perf_counter()
+    ctxs = []
+    for (b, P), o in zip(probs, offs[:-1]):
+        f = fmm.FMM(P=P, ncrit=args.ncrit, theta=args.theta)
+        f.evaluate(base + int(o) * 64, n=len(b), on_device=True)
+        ctxs.append(f)
+    for f in ctxs:
+        f.synchronize()
+    plan_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for f, nb, o in zip(ctxs, ns, offs[:-1]):
+        f.evaluate(base + int(o) * 64, n=nb, on_device=True)
+    for f in ctxs:
+        f.synchronize()
+    eager_ms = (time.perf_counter() - t0) * 1e3
+    t0 = time.perf_counter()
+    g = fmm.Graph(ctxs, streams=args.streams)
+    create_s = time.perf_counter() - t0
+    nodes = g.nodes()
+    for _ in range(args.warmup):
+        g.launch()
+    g.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.launch()
+    g.synchronize()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"plan {plan_s:.2f} s, eager pass {eager_ms:.2f} ms, graph create {create_s:.2f} s ({nodes} nodes), "
+        f"graph step {ms:.3f} ms")
+    # accuracy on 24 problems (both kinds, every P), FP64 direct sum on sampled targets
+    res = dev.cpu().numpy().view(BODY_DTYPE)
+    errs = {}
+    for i in list(range(0, 128, 11)) + list(range(128, 256, 11)):
+        b = res[offs[i]:offs[i + 1]]
+        idx = sample_targets(len(b), 100)
+        ref = b[idx].copy()
+        ref["p"] = 0
+        ref["F"] = 0
+        ctxs[i].direct(ref, b.copy())
+        P = probs[i][1]
+        e = errs.setdefault(P, [0.0, 0.0, 0])
+        e[0] = max(e[0], rel_l2(b["p"][idx], ref["p"]))
+        e[1] = max(e[1], rel_l2(b["F"][idx], ref["F"]))
+        e[2] += 1
+    log("max rel L2 per order:", errs)
+    out = {
+        "metric": "batched 3D Laplace FMM particles/s, 256 problems in one hipGraph",
+        "value": N / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"batched FMM: 256 problems, N log-uniform in [1e3,1e5] (total {N}), half uniform "
+                               f"cube, half Plummer (a=0.1, r<=1), q=U[0,1)/N, P in {{6,8,12}}, theta={args.theta}, "
+                               f"ncrit={args.ncrit}; one step = one hipGraph launch re-evaluating all problems",
+                   "m2l_precision": "bf16x3", "p2p_precision": "fp32", "graph_streams": args.streams,
+                   "graph_nodes": nodes},
+        "plan_s": plan_s, "graph_create_s": create_s, "eager_pass_ms": eager_ms,
+        "eager_value": N / (eager_ms * 1e-3),
+        "error": {"max_rel_l2_by_P": {str(k): {"p": v[0], "F": v[1], "problems": v[2]} for k, v in errs.items()},
+                  "targets_per_problem": 100, "vs": "FP64 direct sum"},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    g.close()
+
+
 def periodic_workload(args, fmm, torch):
     """N bodies uniform in [0,1)^3 with cycle 1, charges +1/-1 alternating
     (net neutral), P = 10, theta = 0.4, images = 3 (3dp/fmm.cxx with the
@@ -372,7 +461,8 @@ def main():
     ap.add_argument("--targets", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "0") or 0))
-    ap.add_argument("--workload", choices=["plain", "learned", "periodic", "copy"], default="plain")
+    ap.add_argument("--workload", choices=["plain", "learned", "batched", "periodic", "copy"], default="plain")
+    ap.add_argument("--streams", type=int, default=8, help="batched workload: graph capture streams")
     ap.add_argument("--images", type=int, default=3)
     args = ap.parse_args()
 
@@ -386,7 +476,8 @@ def main():
         copy_workload(args, torch)
         return
     from minimal_amd import fmm
-    {"plain": plain_workload, "learned": learned_workload, "periodic": periodic_workload}[args.workload](
+    {"plain": plain_workload, "learned": learned_workload, "batched": batched_workload,
+     "periodic": periodic_workload}[args.workload](
         args, fmm, torch)
 
 

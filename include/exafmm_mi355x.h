@@ -220,6 +220,27 @@ int exafmm_mlp_gradient(exafmm_ctx *ctx, const float *weights, const float *X, c
  * slots of the GEMM tiles run at order p (p < capacity). */
 int exafmm_order_stats(exafmm_ctx *ctx, int64_t *hist, int capacity);
 
+/* ---- batched evaluation in a HIP graph (no counterpart in the reference,
+ * whose drivers evaluate one problem per process).  A graph re-evaluates a
+ * set of independent problems with one hipGraphLaunch: every context must
+ * have been evaluated once with device-resident bodies (exafmm_evaluate,
+ * on_device = 1; not periodic).  Creating the graph keeps each context's
+ * tree, lists, keyed-M2L tiles, operators and autotuned GEMM shape, and
+ * records per problem the passes that depend on the charges (P2M, M2M,
+ * P2P, M2L, L2L, L2P, after zeroing p, F and the locals) on one of
+ * `nstreams` (1..16) capture streams, so independent problems run as
+ * parallel branches.  A launch reads the current charges q of the
+ * contexts' (tree-ordered) body arrays, positions as at creation, and
+ * overwrites p and F.  While a graph holds a context, building a tree on
+ * it or destroying it fails with EXAFMM_ERR_STATE.  Errors of these calls
+ * are reported by exafmm_last_error(NULL). */
+typedef struct exafmm_graph exafmm_graph;
+int exafmm_graph_create(exafmm_ctx *const *ctxs, int count, int nstreams, exafmm_graph **out);
+int exafmm_graph_launch(exafmm_graph *graph);      /* asynchronous */
+int exafmm_graph_synchronize(exafmm_graph *graph);
+int exafmm_graph_nodes(exafmm_graph *graph, int64_t *nodes);
+int exafmm_graph_destroy(exafmm_graph *graph);
+
 #ifdef __cplusplus
 }
 #endif

@@ -67,6 +67,7 @@ float elapsed(Ctx &c, int a, int b) {
 }
 
 void set_bodies(Ctx &c, exafmm_body *b, int64_t n, int on_device) {
+  if (c.graph_locks) fail(EXAFMM_ERR_STATE, "the context is held by a graph (exafmm_graph_destroy it first)");
   if (!b || n < 1) fail(EXAFMM_ERR_ARG, "need at least one body");
   c.t.n = n;
   if (on_device) {
@@ -91,6 +92,7 @@ void bodies_to_host(Ctx &c) {
 // event slots: 0/1 build, 2/3 upward, 11/12 traversal, 9/4 M2L (s0),
 // 5/6 P2P (s1), 10/8 downward
 void do_build(Ctx &c) {
+  if (c.graph_locks) fail(EXAFMM_ERR_STATE, "the context is held by a graph (exafmm_graph_destroy it first)");
   rec(c, 0, c.s0);
   tree_build(c);
   rec(c, 1, c.s0);
@@ -108,6 +110,7 @@ void do_upward(Ctx &c) {
 // Interaction lists (geometry only: no multipoles needed).
 void do_lists(Ctx &tc, const Ctx &sc) {
   if (!tc.t.built || !sc.t.built) fail(EXAFMM_ERR_STATE, "horizontal pass before build_tree");
+  if (tc.graph_locks) fail(EXAFMM_ERR_STATE, "the context is held by a graph (exafmm_graph_destroy it first)");
   if (tc.P != sc.P) fail(EXAFMM_ERR_ARG, "target and source contexts differ in P");
   if (tc.cfg.images != sc.cfg.images || (tc.cfg.images > 0 && &tc != &sc))
     fail(EXAFMM_ERR_UNSUPPORTED, "a periodic horizontal pass needs source == target");
@@ -231,6 +234,10 @@ int exafmm_create(const exafmm_config *cfg, exafmm_ctx **out) {
 
 int exafmm_destroy(exafmm_ctx *ctx) {
   if (!ctx) return EXAFMM_ERR_ARG;
+  if (ctx->c.graph_locks) {
+    ctx->c.last_error = "the context is held by a graph (exafmm_graph_destroy it first)";
+    return EXAFMM_ERR_STATE;
+  }
   hipSetDevice(ctx->c.cfg.device);
   if (ctx->c.s0) hipStreamSynchronize(ctx->c.s0);
   if (ctx->c.s1) hipStreamSynchronize(ctx->c.s1);
@@ -636,6 +643,208 @@ int exafmm_l2p(exafmm_ctx *ctx, const double center[3], const double *L, exafmm_
     if (!ctx || !center || !L || !b || nb < 0) fail(EXAFMM_ERR_ARG, "bad argument");
     op_l2p(ctx->c, center, L, b, nb);
   });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- graphs
+// Many independent problems re-evaluated by one hipGraph launch.  Each
+// context was evaluated once (tree, lists, keyed-M2L tiles and operators,
+// autotuned GEMM shape); the graph records, per problem, the
+// charge-dependent passes on one of several capture streams (independent
+// problems form parallel branches of the graph):
+//   charges of the tree-ordered bodies -> P2M inputs, p = F = 0, L = 0
+//   P2M, M2M (per level), P2P, keyed M2L (cast, GEMM, slice sum),
+//   L2L (per level), L2P
+// No memset nodes (fill kernels instead), no events, no host waits.
+namespace xf {
+namespace {
+
+__global__ void __launch_bounds__(256) k_replay_bodies(int64_t n, exafmm_body *__restrict__ b,
+                                                       double4 *__restrict__ xs, float4 *__restrict__ xq) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double q = b[i].q;
+  xs[i].w = q;
+  xq[i].w = float(q);
+  b[i].p = 0.0;
+  b[i].F[0] = 0.0;
+  b[i].F[1] = 0.0;
+  b[i].F[2] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_zero(int64_t n, double *__restrict__ p) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+// The charge-dependent passes of one problem, all on stream s.
+void replay(Ctx &c, hipStream_t s) {
+  const hipStream_t a = c.s0, b = c.s1;
+  c.s0 = c.s1 = s;
+  try {
+    const int64_t n = c.t.n, nl = int64_t(c.t.ncells) * c.P * c.P;
+    hipLaunchKernelGGL(k_replay_bodies, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c.bodies, c.t.xs.p, c.t.xq.p);
+    XF_LAUNCH();
+    hipLaunchKernelGGL(k_zero, dim3(ceil_div(nl, 256)), dim3(256), 0, s, nl, c.L.p);
+    XF_LAUNCH();
+    upward(c);
+    p2p_apply(c, c, s);
+    m2l_apply(c, c);
+    l2l_pass(c);
+    l2p_pass(c);
+  } catch (...) {
+    c.s0 = a;
+    c.s1 = b;
+    c.capturing = false;
+    throw;
+  }
+  c.s0 = a;
+  c.s1 = b;
+}
+
+}  // namespace
+}  // namespace xf
+
+struct exafmm_graph {
+  std::vector<exafmm_ctx *> ctxs;
+  int device = 0;
+  hipStream_t main = nullptr;
+  std::vector<hipStream_t> streams;
+  std::vector<hipEvent_t> events;  // [0] fork, [1..] joins
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void graph_release(exafmm_graph *g) {
+  if (!g) return;
+  hipSetDevice(g->device);
+  if (g->main) hipStreamSynchronize(g->main);
+  for (hipStream_t s : g->streams) hipStreamSynchronize(s);
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  for (hipEvent_t e : g->events)
+    if (e) hipEventDestroy(e);
+  for (hipStream_t s : g->streams)
+    if (s) hipStreamDestroy(s);
+  if (g->main) hipStreamDestroy(g->main);
+  for (exafmm_ctx *x : g->ctxs) {
+    x->c.graph_locks--;
+    x->c.capturing = false;
+    if (!x->c.graph_locks) x->c.m2l_ready = false;
+  }
+  delete g;
+}
+
+}  // namespace
+
+extern "C" {
+
+int exafmm_graph_create(exafmm_ctx *const *ctxs, int count, int nstreams, exafmm_graph **out) {
+  exafmm_graph *g = nullptr;
+  bool capturing = false;
+  hipStream_t cap = nullptr;
+  const int rc = guard(nullptr, [&] {
+    if (!ctxs || count < 1 || !out) fail(EXAFMM_ERR_ARG, "bad argument");
+    *out = nullptr;
+    const int dev = ctxs[0] ? ctxs[0]->c.cfg.device : 0;
+    for (int i = 0; i < count; i++) {
+      if (!ctxs[i]) fail(EXAFMM_ERR_ARG, "null context");
+      const Ctx &c = ctxs[i]->c;
+      if (c.cfg.device != dev) fail(EXAFMM_ERR_ARG, "contexts on different devices");
+      if (!c.have_L || c.src != &c) fail(EXAFMM_ERR_STATE, "each context must have been evaluated once");
+      if (c.host_mode) fail(EXAFMM_ERR_UNSUPPORTED, "graphs need device-resident bodies (on_device = 1)");
+      if (c.cfg.images > 0) fail(EXAFMM_ERR_UNSUPPORTED, "periodic contexts cannot be recorded in a graph");
+      if (c.graph_locks) fail(EXAFMM_ERR_STATE, "a context is already held by a graph");
+      for (int j = 0; j < i; j++)
+        if (ctxs[j] == ctxs[i]) fail(EXAFMM_ERR_ARG, "a context appears twice");
+    }
+    XF_HIP(hipSetDevice(dev));
+    g = new exafmm_graph;
+    g->device = dev;
+    const int ns = std::max(1, std::min(nstreams, 16));
+    XF_HIP(hipStreamCreateWithFlags(&g->main, hipStreamNonBlocking));
+    g->streams.assign(ns, nullptr);
+    g->events.assign(ns + 1, nullptr);
+    for (int k = 0; k < ns; k++) XF_HIP(hipStreamCreateWithFlags(&g->streams[k], hipStreamNonBlocking));
+    for (auto &e : g->events) XF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (int i = 0; i < count; i++) {
+      Ctx &c = ctxs[i]->c;
+      XF_HIP(hipStreamSynchronize(c.s1));
+      XF_HIP(hipStreamSynchronize(c.s0));
+      g->ctxs.push_back(ctxs[i]);
+      c.graph_locks++;
+      if (c.cfg.m2l_precision != EXAFMM_PREC_FP64 && c.lists.nkeys > 0) {
+        m2l_prepare(c, c.s0);  // tiles kept for the graph's lifetime
+        XF_HIP(hipStreamSynchronize(c.s0));
+      }
+    }
+    // one unrecorded pass: sizes every buffer, tunes any GEMM shape not yet
+    // tuned, and leaves the results of the current charges
+    for (int i = 0; i < count; i++) replay(ctxs[i]->c, g->streams[i % ns]);
+    for (hipStream_t s : g->streams) XF_HIP(hipStreamSynchronize(s));
+    cap = g->main;
+    XF_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
+    capturing = true;
+    XF_HIP(hipEventRecord(g->events[0], cap));
+    for (int k = 0; k < ns; k++) XF_HIP(hipStreamWaitEvent(g->streams[k], g->events[0], 0));
+    for (int i = 0; i < count; i++) {
+      Ctx &c = ctxs[i]->c;
+      c.capturing = true;
+      replay(c, g->streams[i % ns]);
+      c.capturing = false;
+    }
+    for (int k = 0; k < ns; k++) {
+      XF_HIP(hipEventRecord(g->events[k + 1], g->streams[k]));
+      XF_HIP(hipStreamWaitEvent(cap, g->events[k + 1], 0));
+    }
+    capturing = false;
+    XF_HIP(hipStreamEndCapture(cap, &g->graph));
+    XF_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g;
+  });
+  if (rc != EXAFMM_OK) {
+    if (capturing) {
+      hipGraph_t junk = nullptr;
+      if (hipStreamEndCapture(cap, &junk) == hipSuccess && junk) hipGraphDestroy(junk);
+      (void)hipGetLastError();
+    }
+    graph_release(g);
+  }
+  return rc;
+}
+
+int exafmm_graph_launch(exafmm_graph *g) {
+  return guard(nullptr, [&] {
+    if (!g || !g->exec) fail(EXAFMM_ERR_ARG, "null graph");
+    XF_HIP(hipSetDevice(g->device));
+    XF_HIP(hipGraphLaunch(g->exec, g->main));
+  });
+}
+
+int exafmm_graph_synchronize(exafmm_graph *g) {
+  return guard(nullptr, [&] {
+    if (!g) fail(EXAFMM_ERR_ARG, "null graph");
+    XF_HIP(hipSetDevice(g->device));
+    XF_HIP(hipStreamSynchronize(g->main));
+  });
+}
+
+int exafmm_graph_nodes(exafmm_graph *g, int64_t *nodes) {
+  return guard(nullptr, [&] {
+    if (!g || !nodes || !g->graph) fail(EXAFMM_ERR_ARG, "bad argument");
+    size_t n = 0;
+    XF_HIP(hipGraphGetNodes(g->graph, nullptr, &n));
+    *nodes = int64_t(n);
+  });
+}
+
+int exafmm_graph_destroy(exafmm_graph *g) {
+  if (!g) return EXAFMM_ERR_ARG;
+  graph_release(g);
+  return EXAFMM_OK;
 }
 
 }  // extern "C"

@@ -143,6 +143,11 @@ struct Ctx {
   //   M~ = (Re, Im) M_n^m / R^n,  L~ = (Re, Im) L_j^k R^(j+1)
   Buf<double> M, L;
   bool have_M = false, have_L = false;
+  // graph recording (api.hip, exafmm_graph_*): while `capturing`, launches
+  // are recorded into a graph (no events, no host waits, no allocation);
+  // while a graph holds the context its tree, lists and tiles stay fixed
+  bool capturing = false;
+  int graph_locks = 0;
   Buf<double> shift_T;             // scaled M2M / L2L operators per octant
   int shift_P = -1;
 

@@ -837,6 +837,7 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
     }
   }
   GemmShape best = cand[0];
+  if (c.capturing) fail(EXAFMM_ERR_STATE, "GEMM shape not tuned before recording a graph");
   if (cand.size() > 1 && c.cfg.autotune) {
     hipStream_t s = c.s0;
     XF_HIP(hipStreamSynchronize(c.s1));  // tune alone on the device
@@ -989,14 +990,17 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
 void m2l_fast(Ctx &tc, const Ctx &sc) {
   if (&tc != &sc) fail(EXAFMM_ERR_UNSUPPORTED, "keyed M2L needs source == target tree");
   Ctx &c = tc;
-  if (!c.m2l_ready) m2l_prepare(c, c.s0);
+  if (!c.m2l_ready) {
+    if (c.capturing) fail(EXAFMM_ERR_STATE, "keyed M2L tiles missing while recording a graph");
+    m2l_prepare(c, c.s0);
+  }
   hipStream_t s = c.s0;
   const Geom g = geom(c.P);
   const bool bf16 = c.cfg.m2l_precision == EXAFMM_PREC_BF16X3;
   const int nc = int(c.t.ncells);
   const int ntiles = c.m2l_ntiles;
   c.m2l_ntiles_last = ntiles;
-  c.m2l_ready = false;  // tiles belong to this horizontal pass
+  if (!c.graph_locks) c.m2l_ready = false;  // tiles belong to this horizontal pass (or to the graph)
   if (ntiles == 0) return;
   const int KP = g.K4 * 4, KPb = g.K32 * 32;
   if (bf16) {
@@ -1021,7 +1025,7 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
 
   // 3. GEMM over tiles, shape from the autotuner
   const GemmShape sh = gemm_shape(c, sc, g, ntiles, bf16, KP, KPb);
-  XF_HIP(hipEventRecord(c.ev[13], s));
+  if (!c.capturing) XF_HIP(hipEventRecord(c.ev[13], s));
   run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L.p);
 }
 

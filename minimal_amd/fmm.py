@@ -40,7 +40,8 @@ EXPORTS = [
     "exafmm_p2m", "exafmm_m2m", "exafmm_m2l", "exafmm_l2l", "exafmm_l2p", "exafmm_work_counts",
     "exafmm_dipole_correction", "exafmm_ewald", "exafmm_order_model_train", "exafmm_set_order_model",
     "exafmm_get_order_model", "exafmm_order_training_set", "exafmm_mlp_forward", "exafmm_mlp_gradient",
-    "exafmm_order_stats",
+    "exafmm_order_stats", "exafmm_graph_create", "exafmm_graph_launch", "exafmm_graph_synchronize",
+    "exafmm_graph_nodes", "exafmm_graph_destroy",
 ]
 
 MLP_PARAMS = 1579  # EXAFMM_MLP_PARAMS
@@ -107,6 +108,11 @@ class Library:
         L.exafmm_mlp_forward.argtypes = [_P, _P, _P, _L, _P]
         L.exafmm_mlp_gradient.argtypes = [_P, _P, _P, _P, _L, _P, _P]
         L.exafmm_order_stats.argtypes = [_P, _P, _I]
+        L.exafmm_graph_create.argtypes = [_P, _I, _I, C.POINTER(_P)]
+        L.exafmm_graph_launch.argtypes = [_P]
+        L.exafmm_graph_synchronize.argtypes = [_P]
+        L.exafmm_graph_nodes.argtypes = [_P, C.POINTER(C.c_int64)]
+        L.exafmm_graph_destroy.argtypes = [_P]
 
     def device_count(self) -> int:
         n = C.c_int(0)
@@ -337,3 +343,36 @@ class FMM:
         c = np.ascontiguousarray(center, np.float64)
         Lc = np.ascontiguousarray(L, np.complex128)
         self._chk(self.L.lib.exafmm_l2p(self.h, _ptr(c), _ptr(Lc), _ptr(bodies), len(bodies)))
+
+
+class Graph:
+    """A hipGraph that re-evaluates many evaluated contexts (device-resident
+    bodies) with one launch; see exafmm_graph_create.  The contexts must
+    outlive the graph."""
+
+    def __init__(self, contexts, streams: int = 8):
+        self.L = load()
+        self.contexts = list(contexts)
+        arr = (_P * len(self.contexts))(*[c.h for c in self.contexts])
+        h = _P()
+        self.L.check(self.L.lib.exafmm_graph_create(arr, len(self.contexts), streams, C.byref(h)))
+        self.h = h
+
+    def launch(self):
+        self.L.check(self.L.lib.exafmm_graph_launch(self.h))
+
+    def synchronize(self):
+        self.L.check(self.L.lib.exafmm_graph_synchronize(self.h))
+
+    def nodes(self) -> int:
+        n = C.c_int64(0)
+        self.L.check(self.L.lib.exafmm_graph_nodes(self.h, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.lib.exafmm_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
