@@ -136,7 +136,7 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
         const double4 Cs = scx[sc];
         off[pos] = make_float4(float(Cs.x + ((img % 3) - 1) * cycle - Cc.x),
                                float(Cs.y + (((img / 3) % 3) - 1) * cycle - Cc.y),
-                               float(Cs.z + ((img / 9) - 1) * cycle - Cc.z), 0.0f);
+                               float(Cs.z + ((img / 9) - 1) * cycle - Cc.z), 1.0f);
       }
     }
     wave_sync();
@@ -165,33 +165,49 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
     wave_sync();
     const int total = pre[ne];
     const int nchecked = self_cell == -2 ? total : (sidx >= 0 ? pre[1] : 0);
-    // round k: lane takes source k*64 + lane; its entry cursor only moves forward
-    const float4 far = make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+    // Round k: lane takes source k*64 + lane; its entry cursor only moves
+    // forward.  Sources are fetched two rounds ahead into two register
+    // buffers used alternately (no copies, so a fetch stays in flight while
+    // the round before it computes).  A lane past the end fetches a valid
+    // body and gets the offset (1e30, 1e30, 1e30) and charge factor 0: r^2
+    // overflows to inf, rsqrt gives 0 and the pair adds exact zeros (body 0
+    // of the source tree always exists).
+    const float4 far_off = make_float4(1e30f, 1e30f, 1e30f, 0.0f);
     int e = 0;
-    float4 cur = far, o = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (lane < total) {
-      while (pre[e + 1] <= lane) e++;
-      cur = sxq[sbg[e] + lane - pre[e]];
-      o = off[e];
-    }
-    // rounds that may hold zero-distance pairs first, then the rest (two
-    // loops, one interaction variant each, keep the register allocation of
-    // one variant)
-    auto round = [&](int k0, auto check) {
-      const float4 sj = make_float4(cur.x + o.x, cur.y + o.y, cur.z + o.z, cur.w);
-      const int g = k0 + kWave + lane;
-      cur = far;
-      o = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (g < total) {  // prefetch the next round's source
-        while (pre[e + 1] <= g) e++;
-        cur = sxq[sbg[e] + g - pre[e]];
-        o = off[e];
-      }
-      interact_block<decltype(check)::value, TC>(xt, sj, ap, ax, ay, az);
+    struct Src {
+      float4 c, o;
     };
+    auto fetch = [&](int g) {
+      const bool in = g < total;
+      if (in)
+        while (pre[e + 1] <= g) e++;
+      const int idx = in ? sbg[e] + g - pre[e] : 0;
+      const float4 t = off[e];  // e < ne always
+      const float4 ov = make_float4(in ? t.x : far_off.x, in ? t.y : far_off.y, in ? t.z : far_off.z,
+                                    in ? t.w : far_off.w);
+      return Src{sxq[idx], ov};
+    };
+    auto step = [&](int k0, Src b, auto check) {
+      const float4 sj = make_float4(b.c.x + b.o.x, b.c.y + b.o.y, b.c.z + b.o.z, b.c.w * b.o.w);
+      const Src nb = fetch(k0 + 2 * kWave + lane);
+      interact_block<decltype(check)::value, TC>(xt, sj, ap, ax, ay, az);
+      return nb;
+    };
+    Src b0 = fetch(lane), b1 = fetch(kWave + lane);
+    // Rounds go in pairs (an odd count ends with one round of far lanes, so
+    // every loop iteration issues the same loads and waits for the older
+    // buffer only).  Rounds that may hold zero-distance pairs first (rounded
+    // up to a pair), then the rest: two loops, one interaction variant each.
+    const int nck = ((nchecked + 2 * kWave - 1) / (2 * kWave)) * (2 * kWave);
     int k0 = 0;
-    for (; k0 < total && k0 < nchecked; k0 += kWave) round(k0, std::true_type());
-    for (; k0 < total; k0 += kWave) round(k0, std::false_type());
+    for (; k0 < total && k0 < nck; k0 += 2 * kWave) {
+      b0 = step(k0, b0, std::true_type());
+      b1 = step(k0 + kWave, b1, std::true_type());
+    }
+    for (; k0 < total; k0 += 2 * kWave) {
+      b0 = step(k0, b0, std::false_type());
+      b1 = step(k0 + kWave, b1, std::false_type());
+    }
   }
   // 64-lane sums: halving over the targets, then a butterfly over the lanes
   // that share a target; lane l writes component l % 4 of target l / 4
