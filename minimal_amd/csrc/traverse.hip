@@ -256,6 +256,26 @@ __global__ void k_entry_ids(int2 *__restrict__ m2l, int64_t n, const int *__rest
   if (i < n) m2l[i].y = slot2id[m2l[i].y];
 }
 
+// Key sort for small key sets (the usual case: hundreds to a few thousand
+// distinct translations): one workgroup, keys in LDS, each key's rank is the
+// number of smaller keys (keys are distinct).  Writes the sorted keys in
+// place and the slot -> rank map.
+constexpr int kRankMax = 6144;
+__global__ void __launch_bounds__(1024) k_key_rank(unsigned long long *__restrict__ keys, const int *__restrict__ sl,
+                                                   const int *__restrict__ count, int *__restrict__ slot2id) {
+  __shared__ unsigned long long K[kRankMax];
+  const int n = *count;
+  for (int i = threadIdx.x; i < n; i += 1024) K[i] = keys[i];
+  __syncthreads();
+  for (int i = threadIdx.x; i < n; i += 1024) {
+    const unsigned long long k = K[i];
+    int r = 0;
+    for (int j = 0; j < n; j++) r += K[j] < k;
+    keys[r] = k;
+    slot2id[sl[i]] = r;
+  }
+}
+
 __global__ void k_slot_ids(const int *__restrict__ sl, int n, int *__restrict__ slot2id) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) slot2id[sl[i]] = i;
@@ -373,17 +393,21 @@ void traverse(Ctx &tc, const Ctx &sc) {
     int nk = 0;
     XF_HIP(hipMemcpyAsync(&nk, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
     XF_HIP(hipStreamSynchronize(s));
-    radix_sort_pairs(k1, v1, k2, v2, nk, 0, 64, T.rs_hist, s);
-    hipLaunchKernelGGL(k_slot_ids, dim3(ceil_div(std::max(nk, 1), 256)), dim3(256), 0, s, v1, nk, Lr.slot2id.p);
-    XF_LAUNCH();
+    if (nk <= kRankMax) {
+      hipLaunchKernelGGL(k_key_rank, dim3(1), dim3(1024), 0, s, k1, v1, cnt, Lr.slot2id.p);
+      XF_LAUNCH();
+    } else {
+      radix_sort_pairs(k1, v1, k2, v2, nk, 0, 64, T.rs_hist, s);
+      hipLaunchKernelGGL(k_slot_ids, dim3(ceil_div(std::max(nk, 1), 256)), dim3(256), 0, s, v1, nk, Lr.slot2id.p);
+      XF_LAUNCH();
+      if (k1 != Lr.keys.p)
+        XF_HIP(hipMemcpyAsync(Lr.keys.p, k1, sizeof(unsigned long long) * nk, hipMemcpyDeviceToDevice, s));
+    }
     if (Lr.nm2l > 0) {
       hipLaunchKernelGGL(k_entry_ids, dim3(ceil_div(Lr.nm2l, 256)), dim3(256), 0, s, Lr.m2l.p, Lr.nm2l,
                          Lr.slot2id.p);
       XF_LAUNCH();
     }
-    if (k1 != Lr.keys.p)
-      XF_HIP(hipMemcpyAsync(Lr.keys.p, k1, sizeof(unsigned long long) * nk, hipMemcpyDeviceToDevice, s));
-    XF_HIP(hipStreamSynchronize(s));
     Lr.nkeys = nk;
   }
 }
