@@ -236,6 +236,12 @@ int exafmm_order_stats(exafmm_ctx *ctx, int64_t *hist, int capacity);
  * are reported by exafmm_last_error(NULL). */
 typedef struct exafmm_graph exafmm_graph;
 int exafmm_graph_create(exafmm_ctx *const *ctxs, int count, int nstreams, exafmm_graph **out);
+/* The same with one launch per stage over all problems instead of per-problem
+ * launches: P2M, P2P, L2P and the charge prologue once each, M2M and L2L once
+ * per tree level, the keyed-M2L cast and GEMM once per expansion order (one
+ * GEMM shape per order: the largest row block, one key per round, no key
+ * slices).  Needs BF16x3 M2L, FP32 P2P and fixed orders. */
+int exafmm_graph_create_grouped(exafmm_ctx *const *ctxs, int count, exafmm_graph **out);
 int exafmm_graph_launch(exafmm_graph *graph);      /* asynchronous */
 int exafmm_graph_synchronize(exafmm_graph *graph);
 int exafmm_graph_nodes(exafmm_graph *graph, int64_t *nodes);

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <exception>
+#include <memory>
 
 #include "fmm_internal.h"
 
@@ -678,6 +679,30 @@ __global__ void __launch_bounds__(256) k_zero(int64_t n, double *__restrict__ p)
   if (i < n) p[i] = 0.0;
 }
 
+// Grouped prologue: per problem, its bodies (charges into the P2M / P2P
+// inputs, p = F = 0) and then its locals (= 0).
+__global__ void __launch_bounds__(256) k_replay_grouped(const GroupTree *__restrict__ g, const int *__restrict__ start,
+                                                        int nprob) {
+  const int b = blockIdx.x, i = group_of(start, nprob, b), lb = b - start[i];
+  const GroupTree &t = g[i];
+  const int nbb = int((t.n + 255) / 256);
+  if (lb < nbb) {
+    const int64_t k = int64_t(lb) * 256 + threadIdx.x;
+    if (k >= t.n) return;
+    exafmm_body *bo = t.bodies + k;
+    const double q = bo->q;
+    t.xs_w[k].w = q;
+    t.xq_w[k].w = float(q);
+    bo->p = 0.0;
+    bo->F[0] = 0.0;
+    bo->F[1] = 0.0;
+    bo->F[2] = 0.0;
+  } else {
+    const int64_t k = int64_t(lb - nbb) * 256 + threadIdx.x;
+    if (k < t.ncells * t.P * t.P) t.L[k] = 0.0;
+  }
+}
+
 // The charge-dependent passes of one problem, all on stream s.
 void replay(Ctx &c, hipStream_t s) {
   const hipStream_t a = c.s0, b = c.s1;
@@ -707,6 +732,19 @@ void replay(Ctx &c, hipStream_t s) {
 }  // namespace xf
 
 struct exafmm_graph {
+  // grouped mode: one launch per stage over all problems
+  struct Stage {
+    int off = 0, n = 0, blocks = 0;  // descriptor / start offsets, entries, workgroups
+  };
+  bool grouped = false;
+  xf::Buf<xf::GroupTree> gt;
+  xf::Buf<xf::GroupLevel> lv;
+  xf::Buf<int> starts;
+  Stage pro, p2m, p2p, l2p;
+  std::vector<Stage> m2m_levels, l2l_levels;  // M2M bottom-up, L2L top-down
+  std::vector<std::unique_ptr<xf::GemmGroupPlan>> gemm;  // one per order
+  std::vector<int> level_soff_m2m, level_soff_l2l;       // start-array offsets of the level stages
+
   std::vector<exafmm_ctx *> ctxs;
   int device = 0;
   hipStream_t main = nullptr;
@@ -736,6 +774,139 @@ void graph_release(exafmm_graph *g) {
     if (!x->c.graph_locks) x->c.m2l_ready = false;
   }
   delete g;
+}
+
+// Descriptor tables of a grouped graph (host-built, copied once).
+void grouped_tables(exafmm_graph *g) {
+  const int n = int(g->ctxs.size());
+  std::vector<GroupTree> gt(n);
+  std::vector<int> st;
+  std::vector<GroupLevel> lv;
+  int maxd = 0;
+  for (int i = 0; i < n; i++) {
+    const Ctx &c = g->ctxs[i]->c;
+    GroupTree &t = gt[i];
+    t.leaves = c.t.leaves.p;
+    t.nleaves = c.t.misc.p + 17;
+    t.cbeg = c.t.cbeg.p;
+    t.ccnt = c.t.ccnt.p;
+    t.cchild = c.t.cchild.p;
+    t.cnch = c.t.cnch.p;
+    t.cx = c.t.cx.p;
+    t.xs = c.t.xs.p;
+    t.xq = c.t.xq.p;
+    t.xs_w = c.t.xs.p;
+    t.xq_w = c.t.xq.p;
+    t.lbeg = c.lists.p2p_beg.p;
+    t.lcnt = c.lists.p2p_cnt.p;
+    t.list = c.lists.p2p.p;
+    t.shift_T = c.shift_T.p;
+    t.M = c.M.p;
+    t.L = c.L.p;
+    t.bodies = c.bodies;
+    t.n = c.t.n;
+    t.ncells = c.t.ncells;
+    t.P = c.P;
+    maxd = std::max(maxd, c.t.depth);
+  }
+  // per-problem stages: starts = prefix of each problem's workgroups
+  auto per_problem = [&](exafmm_graph::Stage &sg, auto blocks_of) {
+    sg.off = int(st.size());
+    sg.n = n;
+    int64_t acc = 0;
+    for (int i = 0; i < n; i++) {
+      st.push_back(int(acc));
+      acc += blocks_of(g->ctxs[i]->c);
+    }
+    st.push_back(int(acc));
+    if (acc > (int64_t(1) << 31) - 1) fail(EXAFMM_ERR_CAPACITY, "batch too large");
+    sg.blocks = int(acc);
+  };
+  per_problem(g->pro, [](const Ctx &c) {
+    return int64_t(ceil_div(c.t.n, 256)) + ceil_div(c.t.ncells * c.P * c.P, 256);
+  });
+  per_problem(g->p2m, [](const Ctx &c) { return int64_t(p2m_group_blocks(c.t.nleaves)); });
+  per_problem(g->p2p, [](const Ctx &c) { return int64_t(p2p_group_blocks(c.t.nleaves)); });
+  per_problem(g->l2p, [](const Ctx &c) { return int64_t(c.t.nleaves); });
+  // per-level stages: the problems that have parents at level l
+  auto per_level = [&](int l, int mult) {
+    exafmm_graph::Stage sg;
+    sg.off = int(lv.size());
+    const int soff = int(st.size());
+    int64_t acc = 0;
+    for (int i = 0; i < n; i++) {
+      const Ctx &c = g->ctxs[i]->c;
+      if (l >= c.t.depth) continue;
+      const int cnt = c.t.lstart[l + 1] - c.t.lstart[l];
+      if (cnt <= 0) continue;
+      GroupLevel e{i, c.t.lstart[l], cnt, ceil_div(cnt, 16), shift_group_rows(c.P)};
+      lv.push_back(e);
+      st.push_back(int(acc));
+      acc += int64_t(e.nbx) * e.nby * mult;
+    }
+    st.push_back(int(acc));
+    sg.n = int(lv.size()) - sg.off;
+    sg.blocks = int(acc);
+    // the level's start array sits at soff; keep it beside the descriptors
+    return std::make_pair(sg, soff);
+  };
+  std::vector<std::pair<exafmm_graph::Stage, int>> m2m, l2l;
+  for (int l = maxd - 1; l >= 0; l--) m2m.push_back(per_level(l, 1));
+  for (int l = 0; l < maxd; l++) l2l.push_back(per_level(l, 8));
+  g->gt.ensure(n);
+  g->lv.ensure(std::max<size_t>(lv.size(), 1));
+  g->starts.ensure(st.size());
+  XF_HIP(hipMemcpy(g->gt.p, gt.data(), sizeof(GroupTree) * n, hipMemcpyHostToDevice));
+  if (!lv.empty()) XF_HIP(hipMemcpy(g->lv.p, lv.data(), sizeof(GroupLevel) * lv.size(), hipMemcpyHostToDevice));
+  XF_HIP(hipMemcpy(g->starts.p, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
+  // level stages: (descriptor offset, entries, workgroups), start-array
+  // offsets in parallel lists
+  g->m2m_levels.clear();
+  g->l2l_levels.clear();
+  g->level_soff_m2m.clear();
+  g->level_soff_l2l.clear();
+  for (auto &pr : m2m) {
+    g->m2m_levels.push_back(pr.first);
+    g->level_soff_m2m.push_back(pr.second);
+  }
+  for (auto &pr : l2l) {
+    g->l2l_levels.push_back(pr.first);
+    g->level_soff_l2l.push_back(pr.second);
+  }
+  // keyed GEMM: one plan per order
+  std::map<int, std::vector<Ctx *>> byP;
+  for (int i = 0; i < n; i++) byP[g->ctxs[i]->c.P].push_back(&g->ctxs[i]->c);
+  g->gemm.clear();
+  for (auto &kv : byP) {
+    g->gemm.emplace_back(new GemmGroupPlan);
+    m2l_group_setup(kv.second, *g->gemm.back());
+  }
+}
+
+// All charge-dependent passes of every problem: one launch per stage (per
+// level for M2M and L2L, per order for the keyed GEMM); P2P on the second
+// stream beside the upward pass and M2L, joined before L2P.
+void grouped_record(exafmm_graph *g, hipStream_t sa, hipStream_t sb, hipEvent_t fork, hipEvent_t join) {
+  const int n = int(g->ctxs.size());
+  const int *st = g->starts.p;
+  hipLaunchKernelGGL(k_replay_grouped, dim3(g->pro.blocks), dim3(256), 0, sa, g->gt.p, st + g->pro.off, n);
+  XF_LAUNCH();
+  XF_HIP(hipEventRecord(fork, sa));
+  XF_HIP(hipStreamWaitEvent(sb, fork, 0));
+  if (g->p2p.blocks) p2p_grouped(g->gt.p, st + g->p2p.off, n, g->p2p.blocks, sb);
+  XF_HIP(hipEventRecord(join, sb));
+  if (g->p2m.blocks) p2m_grouped(g->gt.p, st + g->p2m.off, n, g->p2m.blocks, sa);
+  for (size_t k = 0; k < g->m2m_levels.size(); k++) {
+    const auto &sg = g->m2m_levels[k];
+    if (sg.blocks) m2m_grouped(g->gt.p, g->lv.p + sg.off, st + g->level_soff_m2m[k], sg.n, sg.blocks, sa);
+  }
+  for (auto &plan : g->gemm) m2l_group_launch(*plan, sa);
+  for (size_t k = 0; k < g->l2l_levels.size(); k++) {
+    const auto &sg = g->l2l_levels[k];
+    if (sg.blocks) l2l_grouped(g->gt.p, g->lv.p + sg.off, st + g->level_soff_l2l[k], sg.n, sg.blocks, sa);
+  }
+  XF_HIP(hipStreamWaitEvent(sa, join, 0));
+  if (g->l2p.blocks) l2p_grouped(g->gt.p, st + g->l2p.off, n, g->l2p.blocks, sa);
 }
 
 }  // namespace
@@ -800,6 +971,74 @@ int exafmm_graph_create(exafmm_ctx *const *ctxs, int count, int nstreams, exafmm
       XF_HIP(hipEventRecord(g->events[k + 1], g->streams[k]));
       XF_HIP(hipStreamWaitEvent(cap, g->events[k + 1], 0));
     }
+    capturing = false;
+    XF_HIP(hipStreamEndCapture(cap, &g->graph));
+    XF_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    *out = g;
+  });
+  if (rc != EXAFMM_OK) {
+    if (capturing) {
+      hipGraph_t junk = nullptr;
+      if (hipStreamEndCapture(cap, &junk) == hipSuccess && junk) hipGraphDestroy(junk);
+      (void)hipGetLastError();
+    }
+    graph_release(g);
+  }
+  return rc;
+}
+
+int exafmm_graph_create_grouped(exafmm_ctx *const *ctxs, int count, exafmm_graph **out) {
+  exafmm_graph *g = nullptr;
+  bool capturing = false;
+  hipStream_t cap = nullptr;
+  const int rc = guard(nullptr, [&] {
+    if (!ctxs || count < 1 || !out) fail(EXAFMM_ERR_ARG, "bad argument");
+    *out = nullptr;
+    const int dev = ctxs[0] ? ctxs[0]->c.cfg.device : 0;
+    for (int i = 0; i < count; i++) {
+      if (!ctxs[i]) fail(EXAFMM_ERR_ARG, "null context");
+      const Ctx &c = ctxs[i]->c;
+      if (c.cfg.device != dev) fail(EXAFMM_ERR_ARG, "contexts on different devices");
+      if (!c.have_L || c.src != &c) fail(EXAFMM_ERR_STATE, "each context must have been evaluated once");
+      if (c.host_mode) fail(EXAFMM_ERR_UNSUPPORTED, "graphs need device-resident bodies (on_device = 1)");
+      if (c.cfg.images > 0) fail(EXAFMM_ERR_UNSUPPORTED, "periodic contexts cannot be recorded in a graph");
+      if (c.cfg.m2l_precision != EXAFMM_PREC_BF16X3 || c.cfg.p2p_precision != EXAFMM_PREC_FP32)
+        fail(EXAFMM_ERR_UNSUPPORTED, "grouped graphs need BF16x3 M2L and FP32 P2P");
+      if (c.cfg.order_policy == EXAFMM_ORDER_LEARNED && c.have_model)
+        fail(EXAFMM_ERR_UNSUPPORTED, "grouped graphs run fixed orders");
+      if (c.lists.nkeys < 1) fail(EXAFMM_ERR_UNSUPPORTED, "a problem without M2L work");
+      if (c.graph_locks) fail(EXAFMM_ERR_STATE, "a context is already held by a graph");
+      for (int j = 0; j < i; j++)
+        if (ctxs[j] == ctxs[i]) fail(EXAFMM_ERR_ARG, "a context appears twice");
+    }
+    XF_HIP(hipSetDevice(dev));
+    g = new exafmm_graph;
+    g->device = dev;
+    g->grouped = true;
+    XF_HIP(hipStreamCreateWithFlags(&g->main, hipStreamNonBlocking));
+    g->streams.assign(1, nullptr);
+    g->events.assign(2, nullptr);
+    XF_HIP(hipStreamCreateWithFlags(&g->streams[0], hipStreamNonBlocking));
+    for (auto &e : g->events) XF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    for (int i = 0; i < count; i++) {
+      Ctx &c = ctxs[i]->c;
+      XF_HIP(hipStreamSynchronize(c.s1));
+      XF_HIP(hipStreamSynchronize(c.s0));
+      g->ctxs.push_back(ctxs[i]);
+      c.graph_locks++;
+      m2l_prepare(c, c.s0);  // tiles kept for the graph's lifetime
+      XF_HIP(hipStreamSynchronize(c.s0));
+      replay(c, c.s0);  // sizes every buffer (cast multipoles, shift operators)
+      XF_HIP(hipStreamSynchronize(c.s0));
+    }
+    grouped_tables(g);
+    // one unrecorded grouped pass, then the recording
+    grouped_record(g, g->main, g->streams[0], g->events[0], g->events[1]);
+    XF_HIP(hipStreamSynchronize(g->main));
+    cap = g->main;
+    XF_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
+    capturing = true;
+    grouped_record(g, g->main, g->streams[0], g->events[0], g->events[1]);
     capturing = false;
     XF_HIP(hipStreamEndCapture(cap, &g->graph));
     XF_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));

@@ -44,15 +44,15 @@ static __constant__ InvTable kInv = InvTable();
 constexpr int kP2MWaves = 4;
 
 template <int PMAX>
-__global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m(int P, const int *__restrict__ leaves,
-                                                         const int *__restrict__ nleaves,
-                                                         const int *__restrict__ cbeg, const int *__restrict__ ccnt,
-                                                         const double4 *__restrict__ cx,
-                                                         const double4 *__restrict__ xs, double *__restrict__ M) {
+__device__ __attribute__((always_inline)) inline void p2m_body(int bx, int P, const int *__restrict__ leaves,
+                                const int *__restrict__ nleaves,
+                                const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                const double4 *__restrict__ cx,
+                                const double4 *__restrict__ xs, double *__restrict__ M) {
   static_assert(PMAX <= 16, "16 orders per body group");
   __shared__ double4 sb[kP2MWaves][kWave];
   const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  const int li = blockIdx.x * kP2MWaves + w;
+  const int li = bx * kP2MWaves + w;
   if (li >= *nleaves) return;
   const int c = leaves[li];
   const int beg = cbeg[c], cnt = ccnt[c];
@@ -128,6 +128,24 @@ __global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m(int P, const int *__re
     if (m > 0) Mc[rf(n, m, 1)] = ai[n];
   }
 }
+
+template <int PMAX>
+__global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m(int P, const int *__restrict__ leaves,
+                                                          const int *__restrict__ nleaves,
+                                                          const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                                          const double4 *__restrict__ cx,
+                                                          const double4 *__restrict__ xs, double *__restrict__ M) {
+  p2m_body<PMAX>(blockIdx.x, P, leaves, nleaves, cbeg, ccnt, cx, xs, M);
+}
+
+// All problems of a batch in one launch (block -> problem through start[]).
+__global__ void __launch_bounds__(kP2MWaves *kWave) k_p2m_grouped(const GroupTree *__restrict__ g,
+                                                                  const int *__restrict__ start, int nprob) {
+  const int b = blockIdx.x, i = group_of(start, nprob, b);
+  const GroupTree &t = g[i];
+  p2m_body<kMaxP>(b - start[i], t.P, t.leaves, t.nleaves, t.cbeg, t.ccnt, t.cx, t.xs, t.M);
+}
+
 
 // ---------------------------------------------------------------- shift operators
 // Column `col` of the scaled M2M (kind 0) or L2L (kind 1) operator of octant
@@ -218,22 +236,22 @@ __device__ inline int child_in(int p, int o, const int *__restrict__ cchild, con
 
 // M~_parent = sum over children of T_M2M[octant] M~_child (3d/kernel.h:206).
 // grid: (parent tiles of 16, row blocks of 64); K runs over (octant, column).
-__global__ void __launch_bounds__(kShiftWaves *kWave) k_m2m(int P, int lstart, int lcount,
-                                                            const int *__restrict__ cchild,
-                                                            const int *__restrict__ cnch,
-                                                            const double4 *__restrict__ cx,
-                                                            const double *__restrict__ T, double *__restrict__ M) {
+__device__ __attribute__((always_inline)) inline void m2m_body(int bx, int by, int P, int lstart, int lcount,
+                                const int *__restrict__ cchild,
+                                const int *__restrict__ cnch,
+                                const double4 *__restrict__ cx,
+                                const double *__restrict__ T, double *__restrict__ M) {
   __shared__ int chl[8][16];
   const int PP = P * P;
   const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
   const int j16 = lane & 15, kq = lane >> 4;
-  const int p0 = blockIdx.x * 16;
+  const int p0 = bx * 16;
   if (threadIdx.x < 128) {
     const int o = threadIdx.x / 16, q = threadIdx.x % 16;
     chl[o][q] = p0 + q < lcount ? child_in(lstart + p0 + q, o, cchild, cnch, cx) : -1;
   }
   __syncthreads();
-  const int r0 = (blockIdx.y * kShiftWaves + w) * 16;
+  const int r0 = (by * kShiftWaves + w) * 16;
   if (r0 >= PP) return;
   const int jmax = degree_of(min(r0 + 16, PP) - 1);
   const int ncol = (jmax + 1) * (jmax + 1);
@@ -268,23 +286,43 @@ __global__ void __launch_bounds__(kShiftWaves *kWave) k_m2m(int P, int lstart, i
   }
 }
 
-// L~_child += T_L2L[octant] L~_parent for the children of one level of
-// parents (3d/kernel.h:265-280).  grid: (parent tiles of 16, row blocks of
-// 64, octant).
-__global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l(int P, int lstart, int lcount,
+__global__ void __launch_bounds__(kShiftWaves *kWave) k_m2m(int P, int lstart, int lcount,
                                                             const int *__restrict__ cchild,
                                                             const int *__restrict__ cnch,
                                                             const double4 *__restrict__ cx,
-                                                            const double *__restrict__ T, double *__restrict__ L) {
+                                                            const double *__restrict__ T, double *__restrict__ M) {
+  m2m_body(blockIdx.x, blockIdx.y, P, lstart, lcount, cchild, cnch, cx, T, M);
+}
+
+// One level of every problem of a batch in one launch: lv[i] describes
+// problem lv[i].prob's cells of this level as nbx x nby blocks.
+__global__ void __launch_bounds__(kShiftWaves *kWave) k_m2m_grouped(const GroupTree *__restrict__ g,
+                                                                    const GroupLevel *__restrict__ lv,
+                                                                    const int *__restrict__ start, int n) {
+  const int b = blockIdx.x, i = group_of(start, n, b), lb = b - start[i];
+  const GroupLevel L = lv[i];
+  const GroupTree &t = g[L.prob];
+  m2m_body(lb % L.nbx, lb / L.nbx, t.P, L.lstart, L.lcount, t.cchild, t.cnch, t.cx, t.shift_T, t.M);
+}
+
+
+// L~_child += T_L2L[octant] L~_parent for the children of one level of
+// parents (3d/kernel.h:265-280).  grid: (parent tiles of 16, row blocks of
+// 64, octant).
+__device__ __attribute__((always_inline)) inline void l2l_body(int bx, int by, int bz, int P, int lstart, int lcount,
+                                const int *__restrict__ cchild,
+                                const int *__restrict__ cnch,
+                                const double4 *__restrict__ cx,
+                                const double *__restrict__ T, double *__restrict__ L) {
   __shared__ int chl[16];
   const int PP = P * P;
   const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
   const int j16 = lane & 15, kq = lane >> 4;
-  const int p0 = blockIdx.x * 16, o = blockIdx.z;
+  const int p0 = bx * 16, o = bz;
   if (threadIdx.x < 16)
     chl[threadIdx.x] = p0 + threadIdx.x < lcount ? child_in(lstart + p0 + threadIdx.x, o, cchild, cnch, cx) : -1;
   __syncthreads();
-  const int r0 = (blockIdx.y * kShiftWaves + w) * 16;
+  const int r0 = (by * kShiftWaves + w) * 16;
   if (r0 >= PP) return;
   const int jmin = degree_of(r0);
   const int c0 = jmin * jmin;
@@ -315,6 +353,27 @@ __global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l(int P, int lstart, i
   }
 }
 
+__global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l(int P, int lstart, int lcount,
+                                                            const int *__restrict__ cchild,
+                                                            const int *__restrict__ cnch,
+                                                            const double4 *__restrict__ cx,
+                                                            const double *__restrict__ T, double *__restrict__ L) {
+  l2l_body(blockIdx.x, blockIdx.y, blockIdx.z, P, lstart, lcount, cchild, cnch, cx, T, L);
+}
+
+// One level of parents of every problem of a batch in one launch (blocks
+// nbx x nby x 8 octants per problem).
+__global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l_grouped(const GroupTree *__restrict__ g,
+                                                                    const GroupLevel *__restrict__ lv,
+                                                                    const int *__restrict__ start, int n) {
+  const int b = blockIdx.x, i = group_of(start, n, b), lb = b - start[i];
+  const GroupLevel L = lv[i];
+  const GroupTree &t = g[L.prob];
+  const int bx = lb % L.nbx, r = lb / L.nbx;
+  l2l_body(bx, r % L.nby, r / L.nby, t.P, L.lstart, L.lcount, t.cchild, t.cnch, t.cx, t.shift_T, t.L);
+}
+
+
 // ---------------------------------------------------------------- L2P
 // One workgroup per leaf, one thread per body.  With u = (x - Xc)/R and
 // Lh_n^m = L_n^m R^n = L~/R, phi = Re sum_{n, m>=0} w_m Lh_n^m Y_n^m(u)
@@ -323,13 +382,13 @@ __global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l(int P, int lstart, i
 //   (dx - i dy) Y_n^m = Y_{n-1}^{m-1} (m >= 1), = -conj(Y_{n-1}^1) (m = 0),
 // collected per harmonic Y_n^m so each column m is generated once.
 template <int MAXP, class T>
-__global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leaves, const int *__restrict__ nleaves,
-                                            const int *__restrict__ cbeg, const int *__restrict__ ccnt,
-                                            const double4 *__restrict__ cx, const double *__restrict__ L,
-                                            const double4 *__restrict__ xs, exafmm_body *__restrict__ bodies) {
+__device__ __attribute__((always_inline)) inline void l2p_body(int bx, int P, const int *__restrict__ leaves, const int *__restrict__ nleaves,
+                                const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                const double4 *__restrict__ cx, const double *__restrict__ L,
+                                const double4 *__restrict__ xs, exafmm_body *__restrict__ bodies) {
   __shared__ cpx<T> Lh[MAXP * (MAXP + 1) / 2];
   __shared__ T rec[MAXP * (MAXP + 1) / 2];  // 1/(2m) at (m, m), -1/((n+m)(n-m)) at (n, m), n >= m + 2
-  const int li = blockIdx.x;
+  const int li = bx;
   if (li >= *nleaves) return;
   const int c = leaves[li];
   const double4 C = cx[c];
@@ -403,6 +462,43 @@ __global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leave
     o->F[1] += 0.5 * double(Ai - Bi) * invR;
     o->F[2] += double(Dz) * invR;
   }
+}
+
+template <int MAXP, class T>
+__global__ void __launch_bounds__(64) k_l2p(int P, const int *__restrict__ leaves, const int *__restrict__ nleaves,
+                                            const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                            const double4 *__restrict__ cx, const double *__restrict__ L,
+                                            const double4 *__restrict__ xs, exafmm_body *__restrict__ bodies) {
+  l2p_body<MAXP, T>(blockIdx.x, P, leaves, nleaves, cbeg, ccnt, cx, L, xs, bodies);
+}
+
+__global__ void __launch_bounds__(64) k_l2p_grouped(const GroupTree *__restrict__ g, const int *__restrict__ start,
+                                                    int nprob) {
+  const int b = blockIdx.x, i = group_of(start, nprob, b);
+  const GroupTree &t = g[i];
+  l2p_body<kMaxP, float>(b - start[i], t.P, t.leaves, t.nleaves, t.cbeg, t.ccnt, t.cx, t.L, t.xs, t.bodies);
+}
+
+
+// ---------------------------------------------------------------- grouped launches
+int p2m_group_blocks(int64_t nleaves) { return ceil_div(nleaves, kP2MWaves); }
+int shift_group_rows(int P) { return ceil_div(P * P, 16 * kShiftWaves); }
+
+void p2m_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_p2m_grouped, dim3(nblocks), dim3(kP2MWaves * kWave), 0, s, g, start, nprob);
+  XF_LAUNCH();
+}
+void m2m_grouped(const GroupTree *g, const GroupLevel *lv, const int *start, int n, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_m2m_grouped, dim3(nblocks), dim3(kShiftWaves * kWave), 0, s, g, lv, start, n);
+  XF_LAUNCH();
+}
+void l2l_grouped(const GroupTree *g, const GroupLevel *lv, const int *start, int n, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2l_grouped, dim3(nblocks), dim3(kShiftWaves * kWave), 0, s, g, lv, start, n);
+  XF_LAUNCH();
+}
+void l2p_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_l2p_grouped, dim3(nblocks), dim3(64), 0, s, g, start, nprob);
+  XF_LAUNCH();
 }
 
 void upward(Ctx &c) {

@@ -70,6 +70,44 @@ struct Buf {
 
 inline int ceil_div(int64_t a, int64_t b) { return int((a + b - 1) / b); }
 
+// ------------------------------------------------------------------ grouped launches
+// One launch per stage over many independent problems (a batch): block b
+// of a grouped launch belongs to the problem i with start[i] <= b <
+// start[i + 1] and runs that problem's block b - start[i].
+__device__ inline int group_of(const int *__restrict__ start, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+// The device arrays of one problem that the charge-dependent passes read
+// and write (its own tree is source and target).
+struct GroupTree {
+  const int *leaves, *nleaves, *cbeg, *ccnt, *cchild, *cnch;
+  const double4 *cx, *xs;
+  const float4 *xq;
+  double4 *xs_w;
+  float4 *xq_w;
+  const int *lbeg, *lcnt, *list;  // P2P lists
+  const double *shift_T;
+  double *M, *L;
+  exafmm_body *bodies;
+  int64_t n, ncells;
+  int P;
+};
+
+// One problem's share of a per-level grouped launch (M2M or L2L): the
+// cells of level l of problem `prob` form nbx x nby blocks.
+struct GroupLevel {
+  int prob, lstart, lcount, nbx, nby;
+};
+
 // ------------------------------------------------------------------ context
 // Device-side tree in breadth-first (level-major) order; children of a
 // cell are contiguous and in octant order, so each level is a Morton-
@@ -208,6 +246,25 @@ float order_train(Ctx &c, int nsamples, int epochs, float lr, uint64_t seed);
 void mlp_eval_host(Ctx &c, const float *w, const float *X, int64_t n, float *Y);
 void mlp_grad_host(Ctx &c, const float *w, const float *X, const float *T, int64_t n, float *grad, float *loss);  // keyed-M2L tiles and operators (fast precisions only)
 void p2p_apply(Ctx &tgt, const Ctx &src, hipStream_t s);
+// grouped launches over the problems of a batch (exafmm_graph_create)
+int p2m_group_blocks(int64_t nleaves);
+int shift_group_rows(int P);
+int p2p_group_blocks(int64_t nleaves);
+void p2m_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, hipStream_t s);
+void m2m_grouped(const GroupTree *g, const GroupLevel *lv, const int *start, int n, int nblocks, hipStream_t s);
+void l2l_grouped(const GroupTree *g, const GroupLevel *lv, const int *start, int n, int nblocks, hipStream_t s);
+void l2p_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, hipStream_t s);
+void p2p_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, hipStream_t s);
+// keyed M2L of every problem of one order (BF16x3, fixed order): cast +
+// GEMM, one launch each; the plan holds the device descriptor tables
+struct GemmGroupPlan {
+  int P = 0, rtb = 0, nprob = 0, nblocks = 0, cast_blocks = 0;
+  size_t lds = 0;
+  Buf<char> desc, cast_desc;
+  Buf<int> start, cast_start;
+};
+void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan);
+void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s);
 void shift_ops(Ctx &c);
 void l2l_pass(Ctx &c);
 void l2p_pass(Ctx &c);

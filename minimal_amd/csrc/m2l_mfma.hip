@@ -22,6 +22,7 @@
 // v_mfma_f32_16x16x32_bf16 (FP32 accumulate), ~16 significant bits per
 // product.
 #include <algorithm>
+#include <type_traits>
 
 #include "fmm_internal.h"
 #include "fmm_ops.h"
@@ -520,34 +521,38 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
   }
 }
 
-template <int RH>
-__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int RTB, int K32, int KP,
-                                                             const int *__restrict__ tfirst,
-                                                             const int *__restrict__ tcount,
-                                                             const int *__restrict__ tbase,
-                                                             const int *__restrict__ tU,
-                                                             const int *__restrict__ sorted,
-                                                             const int *__restrict__ tkeys,
-                                                             const int *__restrict__ tmat,
-                                                             const v8bf *__restrict__ ophi,
-                                                             const v8bf *__restrict__ oplo,
-                                                             const __bf16 *__restrict__ mhi,
-                                                             const __bf16 *__restrict__ mlo,
-                                                             const unsigned char *__restrict__ tkp, int kr,
-                                                             int64_t ncells, float *__restrict__ Lpart,
-                                                             double *__restrict__ L) {
+template <int RH, bool SKIP>
+__device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, int by, int bz, int nz, int PP, int RT, int RTB, int K32, int KP,
+                                       const int *__restrict__ tfirst,
+                                       const int *__restrict__ tcount,
+                                       const int *__restrict__ tbase,
+                                       const int *__restrict__ tU,
+                                       const int *__restrict__ sorted,
+                                       const int *__restrict__ tkeys,
+                                       const int *__restrict__ tmat,
+                                       const v8bf *__restrict__ ophi,
+                                       const v8bf *__restrict__ oplo,
+                                       const __bf16 *__restrict__ mhi,
+                                       const __bf16 *__restrict__ mlo,
+                                       const unsigned char *__restrict__ tkp, int kr,
+                                       int64_t ncells, float *__restrict__ Lpart,
+                                       double *__restrict__ L) {
   extern __shared__ v8bf Ab[];
   // wave index read through a scalar register: the k-step and key loops
   // below then branch uniformly instead of under execution masks
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int cg = w & 3, kp = w >> 2;
-  const int tile = blockIdx.x;
+  const int tile = bx;
   const int ncol = tcount[tile];
-  // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
-  const int Ut = tU[tile], ub = int(int64_t(Ut) * blockIdx.z / gridDim.z),
-            ue = int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
+  // column subtiles of 16 that hold target cells (with SKIP, a partly
+  // filled tile skips the MFMAs of its empty subtiles, whose B operands are
+  // the zero row)
+  const int nct = max(0, min(4, (ncol - cg * 64 + 15) / 16));
+  // keys of this workgroup: slice bz of nz of the tile's keys
+  const int Ut = tU[tile], ub = int(int64_t(Ut) * bz / nz),
+            ue = int(int64_t(Ut) * (bz + 1) / nz);
   const int U = ue - ub, base = tbase[tile] + ub;
-  const int rt0 = blockIdx.y * RTB;
+  const int rt0 = by * RTB;
   const int nrt = min(RTB, RT - rt0);
   const int kq = lane >> 4;
   v4f acc[RH][4];
@@ -672,6 +677,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
             const v8bf a_hi = Ahs[ai], a_lo = Als[ai];
 #pragma unroll
             for (int ct = 0; ct < 4; ct++) {
+              if (SKIP && ct >= nct) break;
               acc[r2][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, bh[ct], acc[r2][ct], 0, 0, 0);
               acc[r2][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, bl[ct], acc[r2][ct], 0, 0, 0);
               acc[r2][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, bh[ct], acc[r2][ct], 0, 0, 0);
@@ -703,11 +709,55 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
   if (kp == 0) {
     if (Lpart)
       write_partial<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane,
-                        Lpart + int64_t(blockIdx.z) * ncells * PP);
+                        Lpart + int64_t(bz) * ncells * PP);
     else
       write_locals<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane, L);
   }
 }
+
+template <int RH>
+__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int RTB, int K32, int KP,
+                                                             const int *__restrict__ tfirst,
+                                                             const int *__restrict__ tcount,
+                                                             const int *__restrict__ tbase,
+                                                             const int *__restrict__ tU,
+                                                             const int *__restrict__ sorted,
+                                                             const int *__restrict__ tkeys,
+                                                             const int *__restrict__ tmat,
+                                                             const v8bf *__restrict__ ophi,
+                                                             const v8bf *__restrict__ oplo,
+                                                             const __bf16 *__restrict__ mhi,
+                                                             const __bf16 *__restrict__ mlo,
+                                                             const unsigned char *__restrict__ tkp, int kr,
+                                                             int64_t ncells, float *__restrict__ Lpart,
+                                                             double *__restrict__ L) {
+  m2l_bf16x3_body<RH, false>(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, PP, RT, RTB, K32, KP, tfirst, tcount, tbase, tU,
+                      sorted, tkeys, tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells, Lpart, L);
+}
+
+// One problem of a grouped keyed-GEMM launch (all problems of one order in
+// a batch): its tiles, operators, cast multipoles and locals.
+struct GroupGemm {
+  const int *tfirst, *tcount, *tbase, *tU, *sorted, *tkeys, *tmat;
+  const v8bf *ophi, *oplo;
+  const __bf16 *mhi, *mlo;
+  double *L;
+  int64_t ncells;
+  int ntiles;
+};
+
+// All problems of one order in one launch: each problem has ntiles x
+// (row blocks) workgroups, one key slice, kr = 1.
+template <int RH>
+__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_grouped(const GroupGemm *__restrict__ g,
+                                                                     const int *__restrict__ start, int nprob,
+                                                                     int PP, int RT, int RTB, int K32, int KP) {
+  const int b = blockIdx.x, i = group_of(start, nprob, b), lb = b - start[i];
+  const GroupGemm &q = g[i];
+  m2l_bf16x3_body<RH, true>(lb % q.ntiles, lb / q.ntiles, 0, 1, PP, RT, RTB, K32, KP, q.tfirst, q.tcount, q.tbase, q.tU,
+                      q.sorted, q.tkeys, q.tmat, q.ophi, q.oplo, q.mhi, q.mlo, nullptr, 1, q.ncells, nullptr, q.L);
+}
+
 
 // Row tiles per workgroup: two operator buffers within the LDS budget, at
 // most kGemmMaxRT accumulator row tiles per wave, and room for the parity
@@ -1027,6 +1077,145 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   const GemmShape sh = gemm_shape(c, sc, g, ntiles, bf16, KP, KPb);
   if (!c.capturing) XF_HIP(hipEventRecord(c.ev[13], s));
   run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L.p);
+}
+
+
+// ---------------------------------------------------------------- grouped (batch)
+struct GroupCast {
+  const double *M;
+  __bf16 *mhi, *mlo;
+  int64_t ncells;
+  int PP, KP;
+};
+
+// BF16 hi/lo cast of the multipoles of every problem of one order (row
+// ncells of each problem is the zero row).
+__global__ void __launch_bounds__(256) k_mt_grouped(const GroupCast *__restrict__ g, const int *__restrict__ start,
+                                                    int nprob) {
+  const int b = blockIdx.x, i = group_of(start, nprob, b);
+  const GroupCast q = g[i];
+  const int64_t e = int64_t(b - start[i]) * 256 + threadIdx.x;
+  const int64_t c = e / q.KP;
+  const int r = int(e % q.KP);
+  if (c > q.ncells) return;
+  const float v = (c < q.ncells && r < q.PP) ? float(q.M[c * q.PP + r]) : 0.0f;
+  const __bf16 h = __bf16(v);
+  q.mhi[e] = h;
+  q.mlo[e] = __bf16(v - float(h));
+}
+
+template <int RH>
+static void group_attr(size_t lds) {
+  XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3_grouped<RH>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+}
+
+// Descriptor tables of one order's problems (their tiles and operators from
+// m2l_prepare, their cast buffers sized by an earlier evaluation).  The
+// shape is the largest row block that fits (one key per LDS round, one key
+// slice), the same for every problem of the launch.
+void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan) {
+  plan.nprob = 0;
+  plan.nblocks = plan.cast_blocks = 0;
+  if (probs.empty()) return;
+  const int P = probs[0]->P;
+  const Geom g = geom(P);
+  const int KPb = g.K32 * 32;
+  plan.P = P;
+  plan.rtb = rows_per_block(g, true, probs[0]->cfg.device, 1);
+  const int nrb = ceil_div(g.R, plan.rtb);
+  plan.lds = gemm_lds(g, true, plan.rtb, 1);
+  std::vector<GroupGemm> d;
+  std::vector<GroupCast> cd;
+  std::vector<int> st, cst;
+  int64_t acc = 0, cacc = 0;
+  for (Ctx *cp : probs) {
+    Ctx &c = *cp;
+    if (c.P != P || c.cfg.m2l_precision != EXAFMM_PREC_BF16X3) fail(EXAFMM_ERR_ARG, "mixed orders in a GEMM group");
+    if (!c.m2l_ready) fail(EXAFMM_ERR_STATE, "keyed M2L tiles missing");
+    const int nt = c.m2l_ntiles;
+    if (nt == 0) continue;
+    const int64_t nc = c.t.ncells;
+    const size_t per = size_t(g.K32) * g.R * 64 * 8;
+    const __bf16 *ophi = reinterpret_cast<const __bf16 *>(c.ops32.p);
+    __bf16 *mhi = reinterpret_cast<__bf16 *>(c.mt32.p);
+    GroupGemm q;
+    q.tfirst = c.tiles.p;
+    q.tcount = c.tiles.p + nt;
+    q.tbase = c.tiles.p + 2 * nt;
+    q.tU = c.tiles.p + 3 * nt;
+    q.sorted = c.tiles.p + 4 * nt;
+    q.tkeys = c.tile_keys.p;
+    q.tmat = c.tile_mat.p;
+    q.ophi = reinterpret_cast<const v8bf *>(ophi);
+    q.oplo = reinterpret_cast<const v8bf *>(ophi + per * c.lists.nkeys);
+    q.mhi = mhi;
+    q.mlo = mhi + size_t(nc + 1) * KPb;
+    q.L = c.L.p;
+    q.ncells = nc;
+    q.ntiles = nt;
+    d.push_back(q);
+    st.push_back(int(acc));
+    acc += int64_t(nt) * nrb;
+    GroupCast cq{c.M.p, mhi, mhi + size_t(nc + 1) * KPb, nc, g.PP, KPb};
+    cd.push_back(cq);
+    cst.push_back(int(cacc));
+    cacc += ceil_div((nc + 1) * KPb, 256);
+  }
+  if (acc > (int64_t(1) << 31) - 1 || cacc > (int64_t(1) << 31) - 1) fail(EXAFMM_ERR_CAPACITY, "batch too large");
+  plan.nprob = int(d.size());
+  plan.nblocks = int(acc);
+  plan.cast_blocks = int(cacc);
+  if (!plan.nprob) return;
+  st.push_back(int(acc));
+  cst.push_back(int(cacc));
+  plan.desc.ensure(sizeof(GroupGemm) * d.size());
+  plan.cast_desc.ensure(sizeof(GroupCast) * cd.size());
+  plan.start.ensure(st.size());
+  plan.cast_start.ensure(cst.size());
+  XF_HIP(hipMemcpy(plan.desc.p, d.data(), sizeof(GroupGemm) * d.size(), hipMemcpyHostToDevice));
+  XF_HIP(hipMemcpy(plan.cast_desc.p, cd.data(), sizeof(GroupCast) * cd.size(), hipMemcpyHostToDevice));
+  XF_HIP(hipMemcpy(plan.start.p, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
+  XF_HIP(hipMemcpy(plan.cast_start.p, cst.data(), sizeof(int) * cst.size(), hipMemcpyHostToDevice));
+  switch (plan.rtb) {
+    case 1: group_attr<1>(plan.lds); break;
+    case 2: group_attr<2>(plan.lds); break;
+    case 3: group_attr<3>(plan.lds); break;
+    case 4: group_attr<4>(plan.lds); break;
+    case 5: group_attr<5>(plan.lds); break;
+    case 6: group_attr<6>(plan.lds); break;
+    case 7: group_attr<7>(plan.lds); break;
+    case 8: group_attr<8>(plan.lds); break;
+    default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
+  }
+}
+
+void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s) {
+  if (!plan.nprob) return;
+  const Geom g = geom(plan.P);
+  const GroupCast *cd = reinterpret_cast<const GroupCast *>(plan.cast_desc.p);
+  hipLaunchKernelGGL(k_mt_grouped, dim3(plan.cast_blocks), dim3(256), 0, s, cd, plan.cast_start.p, plan.nprob);
+  XF_LAUNCH();
+  const GroupGemm *d = reinterpret_cast<const GroupGemm *>(plan.desc.p);
+  const int KPb = g.K32 * 32;
+  switch (plan.rtb) {
+#define XF_GROUP_CASE(n)                                                                                         \
+  case n:                                                                                                         \
+    hipLaunchKernelGGL((k_m2l_bf16x3_grouped<n>), dim3(plan.nblocks), dim3(kGemmThreads), plan.lds, s, d,         \
+                       plan.start.p, plan.nprob, g.PP, g.R, plan.rtb, g.K32, KPb);                                \
+    break;
+    XF_GROUP_CASE(1)
+    XF_GROUP_CASE(2)
+    XF_GROUP_CASE(3)
+    XF_GROUP_CASE(4)
+    XF_GROUP_CASE(5)
+    XF_GROUP_CASE(6)
+    XF_GROUP_CASE(7)
+    XF_GROUP_CASE(8)
+#undef XF_GROUP_CASE
+    default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
+  }
+  XF_LAUNCH();
 }
 
 }  // namespace xf

@@ -232,12 +232,15 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
   }
 }
 
-__global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32(
-    const int *__restrict__ leaves, const int *__restrict__ nleaves, const int *__restrict__ cbeg,
-    const int *__restrict__ ccnt, const double4 *__restrict__ cx, const float4 *__restrict__ xq,
-    const int *__restrict__ lbeg, const int *__restrict__ lcnt, const int *__restrict__ list,
-    const int *__restrict__ scbeg, const int *__restrict__ sccnt, const double4 *__restrict__ scx,
-    const float4 *__restrict__ sxq, double cycle, int same_tree, exafmm_body *__restrict__ bodies) {
+// One workgroup's share (block `bid`): kP2PWaves target leaves, one wave each.
+__device__ __attribute__((always_inline)) inline void p2p32_body(int bid, const int *__restrict__ leaves, const int *__restrict__ nleaves,
+                                  const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                  const double4 *__restrict__ cx, const float4 *__restrict__ xq,
+                                  const int *__restrict__ lbeg, const int *__restrict__ lcnt,
+                                  const int *__restrict__ list, const int *__restrict__ scbeg,
+                                  const int *__restrict__ sccnt, const double4 *__restrict__ scx,
+                                  const float4 *__restrict__ sxq, double cycle, int same_tree,
+                                  exafmm_body *__restrict__ bodies) {
   __shared__ int s_pre[kP2PWaves][kP2PSeg + 1];  // prefix of the entries' body counts
   __shared__ int s_beg[kP2PWaves][kP2PSeg];      // first source body of each entry
   __shared__ int s_cnt[kP2PWaves][kP2PSeg];      // body count of each entry
@@ -245,7 +248,7 @@ __global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32(
   // (targets stay in vector registers: a block of them in scalar registers
   // would spill)
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const int li = blockIdx.x * kP2PWaves + w;
+  const int li = bid * kP2PWaves + w;
   if (li >= *nleaves) return;  // whole wave; the kernel uses wave-level syncs only
   const int c = leaves[li];
   const int tb = cbeg[c], tn = ccnt[c];
@@ -261,6 +264,27 @@ __global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32(
       p2p_block<kP2PTC / 2>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle,
                             s_pre[w], s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
   }
+}
+
+__global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32(
+                                                            const int *__restrict__ leaves, const int *__restrict__ nleaves, const int *__restrict__ cbeg,
+                                                            const int *__restrict__ ccnt, const double4 *__restrict__ cx, const float4 *__restrict__ xq,
+                                                            const int *__restrict__ lbeg, const int *__restrict__ lcnt, const int *__restrict__ list,
+                                                            const int *__restrict__ scbeg, const int *__restrict__ sccnt, const double4 *__restrict__ scx,
+                                                            const float4 *__restrict__ sxq, double cycle, int same_tree, exafmm_body *__restrict__ bodies) {
+  p2p32_body(blockIdx.x, leaves, nleaves, cbeg, ccnt, cx, xq, lbeg, lcnt, list, scbeg, sccnt, scx, sxq, cycle,
+             same_tree, bodies);
+}
+
+// All problems of a batch in one launch: block b belongs to the problem i
+// with start[i] <= b < start[i + 1] (each problem's own tree is source and
+// target).
+__global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32_grouped(const GroupTree *__restrict__ g,
+                                                                    const int *__restrict__ start, int nprob) {
+  const int b = blockIdx.x, i = group_of(start, nprob, b);
+  const GroupTree &t = g[i];
+  p2p32_body(b - start[i], t.leaves, t.nleaves, t.cbeg, t.ccnt, t.cx, t.xq, t.lbeg, t.lcnt, t.list, t.cbeg, t.ccnt,
+             t.cx, t.xq, 0.0, 1, t.bodies);
 }
 
 // FP64 path: the reference's arithmetic per interaction (1/R2, q*sqrt, dX
@@ -311,6 +335,13 @@ __global__ void __launch_bounds__(64) k_p2p64(const int *__restrict__ leaves, co
   }
 }
 
+int p2p_group_blocks(int64_t nleaves) { return ceil_div(nleaves, kP2PWaves); }
+
+void p2p_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_p2p32_grouped, dim3(nblocks), dim3(kP2PWaves * kWave), 0, s, g, start, nprob);
+  XF_LAUNCH();
+}
+
 void p2p_apply(Ctx &tc, const Ctx &sc, hipStream_t s) {
   const Tree &T = tc.t, &S = sc.t;
   const Lists &L = tc.lists;
@@ -330,7 +361,7 @@ void p2p_apply(Ctx &tc, const Ctx &sc, hipStream_t s) {
 // FP64 oracle on the device: one workgroup per target, threads stride the
 // sources, fixed-order tree reduction (deterministic).
 __global__ void __launch_bounds__(256) k_direct(exafmm_body *__restrict__ t, int64_t nt,
-                                               const exafmm_body *__restrict__ s, int64_t ns) {
+                                                const exafmm_body *__restrict__ s, int64_t ns) {
 #pragma clang fp contract(off)
   __shared__ double red[4][256];
   const int64_t ti = blockIdx.x;

@@ -40,8 +40,8 @@ EXPORTS = [
     "exafmm_p2m", "exafmm_m2m", "exafmm_m2l", "exafmm_l2l", "exafmm_l2p", "exafmm_work_counts",
     "exafmm_dipole_correction", "exafmm_ewald", "exafmm_order_model_train", "exafmm_set_order_model",
     "exafmm_get_order_model", "exafmm_order_training_set", "exafmm_mlp_forward", "exafmm_mlp_gradient",
-    "exafmm_order_stats", "exafmm_graph_create", "exafmm_graph_launch", "exafmm_graph_synchronize",
-    "exafmm_graph_nodes", "exafmm_graph_destroy",
+    "exafmm_order_stats", "exafmm_graph_create", "exafmm_graph_create_grouped", "exafmm_graph_launch",
+    "exafmm_graph_synchronize", "exafmm_graph_nodes", "exafmm_graph_destroy",
 ]
 
 MLP_PARAMS = 1579  # EXAFMM_MLP_PARAMS
@@ -109,6 +109,7 @@ class Library:
         L.exafmm_mlp_gradient.argtypes = [_P, _P, _P, _P, _L, _P, _P]
         L.exafmm_order_stats.argtypes = [_P, _P, _I]
         L.exafmm_graph_create.argtypes = [_P, _I, _I, C.POINTER(_P)]
+        L.exafmm_graph_create_grouped.argtypes = [_P, _I, C.POINTER(_P)]
         L.exafmm_graph_launch.argtypes = [_P]
         L.exafmm_graph_synchronize.argtypes = [_P]
         L.exafmm_graph_nodes.argtypes = [_P, C.POINTER(C.c_int64)]
@@ -350,12 +351,19 @@ class Graph:
     bodies) with one launch; see exafmm_graph_create.  The contexts must
     outlive the graph."""
 
-    def __init__(self, contexts, streams: int = 8):
+    def __init__(self, contexts, streams: int = 8, grouped: bool = False):
+        """grouped: one launch per stage over all problems
+        (exafmm_graph_create_grouped) instead of per-problem launches on
+        `streams` parallel branches."""
         self.L = load()
         self.contexts = list(contexts)
         arr = (_P * len(self.contexts))(*[c.h for c in self.contexts])
         h = _P()
-        self.L.check(self.L.lib.exafmm_graph_create(arr, len(self.contexts), streams, C.byref(h)))
+        if grouped:
+            rc = self.L.lib.exafmm_graph_create_grouped(arr, len(self.contexts), C.byref(h))
+        else:
+            rc = self.L.lib.exafmm_graph_create(arr, len(self.contexts), streams, C.byref(h))
+        self.L.check(rc)
         self.h = h
 
     def launch(self):

@@ -9,6 +9,11 @@ Bars (written in each test):
                                          linear in q; scaling by 2 is exact)
   replay vs FP64 direct sum              relative L2 <= 1e-5 (P = 12), 1e-4
                                          (P = 8), 1e-3 (P = 6)
+  grouped graph vs eager evaluation      relative L2 of the difference <= 1e-6
+                                         (one GEMM shape per order instead of
+                                         the autotuned per-problem shapes: the
+                                         FP32 partial sums of M2L differ in
+                                         order); bit-exact between replays
 """
 import numpy as np
 import pytest
@@ -136,3 +141,73 @@ def test_graph_rejects_unevaluated_and_host_contexts(gpu):
     h.evaluate(hb)  # host-resident bodies
     with pytest.raises(fmm.ExaFMMError):
         fmm.Graph([h])
+
+
+def test_grouped_graph_matches_eager(gpu):
+    probs = _problems()
+    devs, ctxs = _setup(probs)
+    eager = [_host(d) for d in devs]
+    g = fmm.Graph(ctxs, grouped=True)
+    assert g.nodes() < 60  # one launch per stage, per level and per order
+    first = None
+    for rep in range(3):
+        for d in devs:
+            h = d.cpu().numpy().view(BODY_DTYPE)
+            h["p"] = 7.0
+            h["F"] = -3.0
+            d.copy_(torch.from_numpy(h.view(np.uint8).copy()).cuda())
+        torch.cuda.synchronize()
+        g.launch()
+        g.synchronize()
+        res = [_host(d) for d in devs]
+        for i, h in enumerate(res):
+            assert rel_l2(h["p"], eager[i]["p"]) <= 1e-6, (rep, i)
+            assert rel_l2(h["F"], eager[i]["F"]) <= 1e-6, (rep, i)
+        if first is None:
+            first = res
+        else:
+            for a, b in zip(first, res):
+                assert np.array_equal(a["p"], b["p"]) and np.array_equal(a["F"], b["F"])
+        x = torch.ones(1 << 20, device="cuda").sum()
+        assert float(x) == float(1 << 20)
+    g.close()
+
+
+def test_grouped_graph_charges_and_accuracy(gpu):
+    probs = _problems()
+    devs, ctxs = _setup(probs)
+    g = fmm.Graph(ctxs, grouped=True)
+    g.launch()
+    g.synchronize()
+    base = [_host(d) for d in devs]
+    bound = {6: 1e-3, 8: 1e-4, 12: 1e-5}
+    for (b, P), h, f in zip(probs, base, ctxs):
+        idx = sample_targets(len(h), 200)
+        ref = h[idx].copy()
+        ref["p"] = 0
+        ref["F"] = 0
+        f.direct(ref, h)
+        assert rel_l2(h["p"][idx], ref["p"]) <= bound[P], P
+        assert rel_l2(h["F"][idx], ref["F"]) <= bound[P], P
+    for d in devs:
+        h = d.cpu().numpy().view(BODY_DTYPE)
+        h["q"] *= 2.0
+        d.copy_(torch.from_numpy(h.view(np.uint8).copy()).cuda())
+    torch.cuda.synchronize()
+    g.launch()
+    g.synchronize()
+    for i, d in enumerate(devs):
+        h = _host(d)
+        assert np.array_equal(h["p"], 2.0 * base[i]["p"]), i
+        assert np.array_equal(h["F"], 2.0 * base[i]["F"]), i
+    g.close()
+
+
+def test_grouped_graph_rejects_other_precisions(gpu):
+    b, _ = _problems()[0]
+    d = torch.from_numpy(b.view(np.uint8).copy()).cuda()
+    f = fmm.FMM(P=6, m2l_precision=fmm.PREC_FP32)
+    f.evaluate(d.data_ptr(), n=len(b), on_device=True)
+    with pytest.raises(fmm.ExaFMMError):
+        fmm.Graph([f], grouped=True)
+    fmm.Graph([f]).close()  # the per-problem graph takes it
