@@ -351,6 +351,8 @@ def batched_workload(args, fmm, torch):
     g.close()
     g, create_s, ms = graph_steps(True)
     nodes = g.nodes()
+    shapes = g.gemm_shapes()
+    log("autotuned GEMM shapes (P, rows, keys per round):", shapes)
     log(f"plan {plan_s:.2f} s, eager pass {eager_ms:.2f} ms, per-problem graph {ms_pp:.3f} ms ({nodes_pp} nodes), "
         f"grouped graph create {create_s:.2f} s ({nodes} nodes), step {ms:.3f} ms")
     # accuracy on 24 problems (both kinds, every P), FP64 direct sum on sampled targets
@@ -378,7 +380,8 @@ def batched_workload(args, fmm, torch):
                                f"cube, half Plummer (a=0.1, r<=1), q=U[0,1)/N, P in {{6,8,12}}, theta={args.theta}, "
                                f"ncrit={args.ncrit}; one step = one hipGraph launch re-evaluating all problems "
                                f"with one grouped launch per stage",
-                   "m2l_precision": "bf16x3", "p2p_precision": "fp32", "graph_nodes": nodes},
+                   "m2l_precision": "bf16x3", "p2p_precision": "fp32", "graph_nodes": nodes,
+                   "gemm_shapes_autotuned": [{"P": a, "rows": b, "keys_per_round": c} for a, b, c in shapes]},
         "plan_s": plan_s, "graph_create_s": create_s, "eager_pass_ms": eager_ms,
         "eager_value": N / (eager_ms * 1e-3),
         "per_problem_graph": {"ms_per_step": ms_pp, "value": N / (ms_pp * 1e-3), "nodes": nodes_pp,

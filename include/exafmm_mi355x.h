@@ -245,6 +245,11 @@ int exafmm_graph_create_grouped(exafmm_ctx *const *ctxs, int count, exafmm_graph
 int exafmm_graph_launch(exafmm_graph *graph);      /* asynchronous */
 int exafmm_graph_synchronize(exafmm_graph *graph);
 int exafmm_graph_nodes(exafmm_graph *graph, int64_t *nodes);
+/* Keyed-GEMM shapes of a grouped graph, as picked by its autotuner (every
+ * candidate timed on the batch itself when the contexts' config has
+ * autotune = 1): per expansion order, shapes[3k..3k+2] = (P, operator rows
+ * per workgroup, keys per LDS round); *count = number of orders. */
+int exafmm_graph_gemm_shapes(exafmm_graph *graph, int32_t *shapes, int capacity, int *count);
 int exafmm_graph_destroy(exafmm_graph *graph);
 
 #ifdef __cplusplus

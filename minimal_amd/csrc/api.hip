@@ -879,7 +879,8 @@ void grouped_tables(exafmm_graph *g) {
   g->gemm.clear();
   for (auto &kv : byP) {
     g->gemm.emplace_back(new GemmGroupPlan);
-    m2l_group_setup(kv.second, *g->gemm.back());
+    m2l_group_setup(kv.second, *g->gemm.back(), g->main, kv.second[0]->cfg.autotune != 0);
+    XF_HIP(hipStreamSynchronize(g->main));
   }
 }
 
@@ -1077,6 +1078,23 @@ int exafmm_graph_nodes(exafmm_graph *g, int64_t *nodes) {
     size_t n = 0;
     XF_HIP(hipGraphGetNodes(g->graph, nullptr, &n));
     *nodes = int64_t(n);
+  });
+}
+
+int exafmm_graph_gemm_shapes(exafmm_graph *g, int32_t *shapes, int capacity, int *count) {
+  return guard(nullptr, [&] {
+    if (!g || !count || (capacity > 0 && !shapes)) fail(EXAFMM_ERR_ARG, "bad argument");
+    int k = 0;
+    for (auto &pl : g->gemm) {
+      if (!pl->nprob) continue;
+      if (k < capacity) {
+        shapes[3 * k] = pl->P;
+        shapes[3 * k + 1] = pl->rtb * 16;
+        shapes[3 * k + 2] = pl->kr;
+      }
+      k++;
+    }
+    *count = k;
   });
 }
 

@@ -258,12 +258,17 @@ void p2p_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, h
 // keyed M2L of every problem of one order (BF16x3, fixed order): cast +
 // GEMM, one launch each; the plan holds the device descriptor tables
 struct GemmGroupPlan {
-  int P = 0, rtb = 0, nprob = 0, nblocks = 0, cast_blocks = 0;
+  int P = 0, rtb = 0, kr = 1, nprob = 0, nblocks = 0, cast_blocks = 0;
   size_t lds = 0;
   Buf<char> desc, cast_desc;
   Buf<int> start, cast_start;
+  struct Trial {
+    int rtb, kr;
+    float ms;
+  };
+  std::vector<Trial> tuned;  // the autotuner's candidates and their times
 };
-void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan);
+void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, hipStream_t s, bool tune);
 void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s);
 void shift_ops(Ctx &c);
 void l2l_pass(Ctx &c);

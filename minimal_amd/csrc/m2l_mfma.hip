@@ -751,11 +751,13 @@ struct GroupGemm {
 template <int RH>
 __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_grouped(const GroupGemm *__restrict__ g,
                                                                      const int *__restrict__ start, int nprob,
-                                                                     int PP, int RT, int RTB, int K32, int KP) {
+                                                                     int PP, int RT, int RTB, int K32, int KP,
+                                                                     int kr) {
   const int b = blockIdx.x, i = group_of(start, nprob, b), lb = b - start[i];
   const GroupGemm &q = g[i];
-  m2l_bf16x3_body<RH, true>(lb % q.ntiles, lb / q.ntiles, 0, 1, PP, RT, RTB, K32, KP, q.tfirst, q.tcount, q.tbase, q.tU,
-                      q.sorted, q.tkeys, q.tmat, q.ophi, q.oplo, q.mhi, q.mlo, nullptr, 1, q.ncells, nullptr, q.L);
+  m2l_bf16x3_body<RH, true>(lb % q.ntiles, lb / q.ntiles, 0, 1, PP, RT, RTB, K32, KP, q.tfirst, q.tcount, q.tbase,
+                            q.tU, q.sorted, q.tkeys, q.tmat, q.ophi, q.oplo, q.mhi, q.mlo, nullptr, kr, q.ncells,
+                            nullptr, q.L);
 }
 
 
@@ -1111,10 +1113,10 @@ static void group_attr(size_t lds) {
 }
 
 // Descriptor tables of one order's problems (their tiles and operators from
-// m2l_prepare, their cast buffers sized by an earlier evaluation).  The
-// shape is the largest row block that fits (one key per LDS round, one key
-// slice), the same for every problem of the launch.
-void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan) {
+// m2l_prepare, their cast buffers sized by an earlier evaluation) for one
+// GEMM shape (rows per workgroup, keys per LDS round), the same for every
+// problem of the launch.
+static void group_tables(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, int rtb, int kr) {
   plan.nprob = 0;
   plan.nblocks = plan.cast_blocks = 0;
   if (probs.empty()) return;
@@ -1122,9 +1124,10 @@ void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan) {
   const Geom g = geom(P);
   const int KPb = g.K32 * 32;
   plan.P = P;
-  plan.rtb = rows_per_block(g, true, probs[0]->cfg.device, 1);
+  plan.rtb = rtb;
+  plan.kr = kr;
   const int nrb = ceil_div(g.R, plan.rtb);
-  plan.lds = gemm_lds(g, true, plan.rtb, 1);
+  plan.lds = gemm_lds(g, true, plan.rtb, kr);
   std::vector<GroupGemm> d;
   std::vector<GroupCast> cd;
   std::vector<int> st, cst;
@@ -1190,6 +1193,53 @@ void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan) {
   }
 }
 
+// Autotuner of a grouped GEMM: every (rows per workgroup, keys per LDS
+// round) candidate runs on the batch itself (alone on the device, after one
+// untimed launch), timed with HIP events; the fastest shape is kept.  The
+// runs add into the locals, which the graph's prologue clears.
+void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, hipStream_t s, bool tune) {
+  if (probs.empty()) return;
+  const Geom g = geom(probs[0]->P);
+  const int dev = probs[0]->cfg.device;
+  std::vector<std::pair<int, int>> cand;  // (rtb, kr)
+  for (int kr = 1; kr <= 3; kr++) {
+    const int rmax = rows_per_block(g, true, dev, kr);
+    if (rmax < 1) continue;
+    int last = -1;
+    for (int nb = ceil_div(g.R, rmax); nb <= ceil_div(g.R, rmax) + 1; nb++) {
+      const int r = ceil_div(g.R, nb);
+      if (r != last && r <= rmax) cand.push_back({r, kr});
+      last = r;
+    }
+  }
+  if (cand.empty()) fail(EXAFMM_ERR_UNSUPPORTED, "expansion order too large for the LDS operator stage");
+  std::pair<int, int> best = cand[0];
+  if (tune && cand.size() > 1) {
+    hipEvent_t e0, e1;
+    XF_HIP(hipEventCreate(&e0));
+    XF_HIP(hipEventCreate(&e1));
+    float bt = 1e30f;
+    for (const auto &c : cand) {
+      group_tables(probs, plan, c.first, c.second);
+      m2l_group_launch(plan, s);  // untimed
+      XF_HIP(hipEventRecord(e0, s));
+      m2l_group_launch(plan, s);
+      XF_HIP(hipEventRecord(e1, s));
+      XF_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      XF_HIP(hipEventElapsedTime(&ms, e0, e1));
+      plan.tuned.push_back({c.first, c.second, ms});
+      if (ms < bt) {
+        bt = ms;
+        best = c;
+      }
+    }
+    XF_HIP(hipEventDestroy(e0));
+    XF_HIP(hipEventDestroy(e1));
+  }
+  group_tables(probs, plan, best.first, best.second);
+}
+
 void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s) {
   if (!plan.nprob) return;
   const Geom g = geom(plan.P);
@@ -1202,7 +1252,7 @@ void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s) {
 #define XF_GROUP_CASE(n)                                                                                         \
   case n:                                                                                                         \
     hipLaunchKernelGGL((k_m2l_bf16x3_grouped<n>), dim3(plan.nblocks), dim3(kGemmThreads), plan.lds, s, d,         \
-                       plan.start.p, plan.nprob, g.PP, g.R, plan.rtb, g.K32, KPb);                                \
+                       plan.start.p, plan.nprob, g.PP, g.R, plan.rtb, g.K32, KPb, plan.kr);                       \
     break;
     XF_GROUP_CASE(1)
     XF_GROUP_CASE(2)

@@ -41,7 +41,7 @@ EXPORTS = [
     "exafmm_dipole_correction", "exafmm_ewald", "exafmm_order_model_train", "exafmm_set_order_model",
     "exafmm_get_order_model", "exafmm_order_training_set", "exafmm_mlp_forward", "exafmm_mlp_gradient",
     "exafmm_order_stats", "exafmm_graph_create", "exafmm_graph_create_grouped", "exafmm_graph_launch",
-    "exafmm_graph_synchronize", "exafmm_graph_nodes", "exafmm_graph_destroy",
+    "exafmm_graph_synchronize", "exafmm_graph_nodes", "exafmm_graph_gemm_shapes", "exafmm_graph_destroy",
 ]
 
 MLP_PARAMS = 1579  # EXAFMM_MLP_PARAMS
@@ -113,6 +113,7 @@ class Library:
         L.exafmm_graph_launch.argtypes = [_P]
         L.exafmm_graph_synchronize.argtypes = [_P]
         L.exafmm_graph_nodes.argtypes = [_P, C.POINTER(C.c_int64)]
+        L.exafmm_graph_gemm_shapes.argtypes = [_P, _P, _I, C.POINTER(C.c_int)]
         L.exafmm_graph_destroy.argtypes = [_P]
 
     def device_count(self) -> int:
@@ -376,6 +377,13 @@ class Graph:
         n = C.c_int64(0)
         self.L.check(self.L.lib.exafmm_graph_nodes(self.h, C.byref(n)))
         return n.value
+
+    def gemm_shapes(self) -> list:
+        """[(P, rows per workgroup, keys per LDS round)] of a grouped graph."""
+        buf = np.zeros(48, np.int32)
+        n = C.c_int(0)
+        self.L.check(self.L.lib.exafmm_graph_gemm_shapes(self.h, _ptr(buf), 16, C.byref(n)))
+        return [tuple(int(x) for x in buf[3 * k:3 * k + 3]) for k in range(min(n.value, 16))]
 
     def close(self):
         if getattr(self, "h", None):
