@@ -508,9 +508,25 @@ void upward(Ctx &c) {
   shift_ops(c);
   c.M.ensure(size_t(t.ncells) * PP);
   const int64_t nl = t.nleaves;
-  hipLaunchKernelGGL((k_p2m<kMaxP>), dim3(ceil_div(nl, kP2MWaves)), dim3(kP2MWaves * kWave), 0, s, P, t.leaves.p,
-                     t.misc.p + 17,
-                     t.cbeg.p, t.ccnt.p, t.cx.p, t.xs.p, c.M.p);
+  // the per-lane degree loop is unrolled to PMAX (masked past P): the
+  // smallest even bound >= P keeps the masked iterations few
+  const dim3 pg(ceil_div(nl, kP2MWaves)), pb(kP2MWaves * kWave);
+  switch (std::max(4, (P + 1) & ~1)) {
+#define XF_P2M_CASE(n)                                                                                      \
+  case n:                                                                                                    \
+    hipLaunchKernelGGL((k_p2m<n>), pg, pb, 0, s, P, t.leaves.p, t.misc.p + 17, t.cbeg.p, t.ccnt.p, t.cx.p, \
+                       t.xs.p, c.M.p);                                                                       \
+    break;
+    XF_P2M_CASE(4)
+    XF_P2M_CASE(6)
+    XF_P2M_CASE(8)
+    XF_P2M_CASE(10)
+    XF_P2M_CASE(12)
+    XF_P2M_CASE(14)
+    XF_P2M_CASE(16)
+#undef XF_P2M_CASE
+    default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
+  }
   XF_LAUNCH();
   for (int l = t.depth - 1; l >= 0; l--) {
     const int cnt = t.lstart[l + 1] - t.lstart[l];

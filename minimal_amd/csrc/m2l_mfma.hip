@@ -50,6 +50,7 @@ static Geom geom(int P) {
 }
 
 // ---------------------------------------------------------------- tiles
+constexpr int kTileUnroll = 8;  // list entries in flight per thread (list walks below)
 // MurmurHash3's 32-bit finaliser: spreads key ids before taking the minimum.
 __device__ inline unsigned mix32(unsigned h) {
   h ^= h >> 16;
@@ -85,7 +86,14 @@ __global__ void __launch_bounds__(256) k_group_keys(int ncells, const double4 *_
     if (minhash) {
       unsigned m = 0xffffffffu;
       const int b0 = m2l_beg[c];
-      for (int i = 0; i < n; i++) m = min(m, mix32(unsigned(list[b0 + i].y)));
+      for (int i0 = 0; i0 < n; i0 += kTileUnroll) {  // kTileUnroll entries in flight
+        int id[kTileUnroll];
+#pragma unroll
+        for (int k = 0; k < kTileUnroll; k++) id[k] = i0 + k < n ? list[b0 + i0 + k].y : -1;
+#pragma unroll
+        for (int k = 0; k < kTileUnroll; k++)
+          if (id[k] >= 0) m = min(m, mix32(unsigned(id[k])));
+      }
       mh = m >> 8;
     }
   }
@@ -104,8 +112,7 @@ __global__ void __launch_bounds__(256) k_group_hist(int ncells, const unsigned l
 }
 
 // Tile kernels: one thread per target column walks that column's M2L list
-// (entries (source << 5 | image, key id)), eight entries in flight.
-constexpr int kTileUnroll = 8;
+// (entries (source << 5 | image, key id)), kTileUnroll entries in flight.
 
 // Distinct keys of one tile: bitmap over key ids in LDS.
 __device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, const int *cnt, const int2 *list,

@@ -141,19 +141,19 @@ def plummer_background(n: int, seed: int) -> np.ndarray:
 
 def batched_problems(count: int, seed: int) -> list:
     """The batched workload: ``count`` independent problems, N log-uniform in
-    [10^3, 10^5], the first half uniform in [0,1)^3 and the second half
-    Plummer spheres (scale radius 0.1, truncated at 10 scale radii), charges
-    U[0,1)/N, expansion order drawn from {6, 8, 12}.  Returns a list of
-    (bodies, P)."""
+    [10^3, 10^5]; even-indexed problems uniform in [0,1)^3, odd-indexed ones
+    Plummer spheres (scale radius 0.1, truncated at 10 scale radii, centred
+    at 0.5); charges U[0,1)/N; expansion order drawn from {6, 8, 12}.
+    Returns a list of (bodies, P)."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
         n = int(round(10.0 ** rng.uniform(3.0, 5.0)))
         b = np.zeros(n, dtype=BODY_DTYPE)
-        if i < count // 2:
+        if i % 2 == 0:
             b["X"] = rng.random((n, 3))
         else:
-            b["X"] = plummer(n, 0.1, 1.0, rng, center=(0.0, 0.0, 0.0))
+            b["X"] = plummer(n, 0.1, 1.0, rng)
         b["q"] = rng.random(n) / n
         out.append((b, int(rng.choice([6, 8, 12]))))
     return out
