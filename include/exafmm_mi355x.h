@@ -252,6 +252,28 @@ int exafmm_graph_nodes(exafmm_graph *graph, int64_t *nodes);
 int exafmm_graph_gemm_shapes(exafmm_graph *graph, int32_t *shapes, int capacity, int *count);
 int exafmm_graph_destroy(exafmm_graph *graph);
 
+/* ---- batched evaluation through one context (the survey's
+ * exafmm_evaluate_batch; no counterpart in the reference).  Evaluates every
+ * problem (tree, lists, all passes) in sub-contexts that the context keeps
+ * with its own configuration and the problem's order P.  With
+ * EXAFMM_BATCH_REUSE_GEOMETRY and device-resident bodies, the call also
+ * records the problems in a graph (grouped launches when the configuration
+ * allows, else per-problem launches); a later call with the same problems
+ * (same arrays, sizes and orders) and the flag replays that graph: the
+ * positions of the previous call are kept (the arrays are in tree order
+ * after the first call), the charges are read afresh, p and F overwritten.
+ * Recording ends with one pass of the graph, so a recording call returns
+ * the graph's results (a grouped graph's GEMM shapes sum the M2L partials
+ * in another order than the eager pass: ~1e-7 relative differences). */
+typedef struct exafmm_problem {
+  exafmm_body *bodies;
+  int64_t n;
+  int32_t P;
+  int32_t on_device;
+} exafmm_problem;
+#define EXAFMM_BATCH_REUSE_GEOMETRY 1
+int exafmm_evaluate_batch(exafmm_ctx *ctx, const exafmm_problem *problems, int count, int flags);
+
 #ifdef __cplusplus
 }
 #endif

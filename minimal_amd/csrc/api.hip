@@ -12,6 +12,11 @@
 
 struct exafmm_ctx {
   xf::Ctx c;
+  // exafmm_evaluate_batch: one sub-context per problem slot, and the graph
+  // that re-evaluates them when the caller keeps the positions
+  std::vector<exafmm_ctx *> batch;
+  exafmm_graph *batch_graph = nullptr;
+  std::vector<std::pair<const void *, int64_t>> batch_sig;
 };
 
 namespace xf {
@@ -239,6 +244,10 @@ int exafmm_destroy(exafmm_ctx *ctx) {
     ctx->c.last_error = "the context is held by a graph (exafmm_graph_destroy it first)";
     return EXAFMM_ERR_STATE;
   }
+  if (ctx->batch_graph) exafmm_graph_destroy(ctx->batch_graph);
+  ctx->batch_graph = nullptr;
+  for (exafmm_ctx *b : ctx->batch) exafmm_destroy(b);
+  ctx->batch.clear();
   hipSetDevice(ctx->c.cfg.device);
   if (ctx->c.s0) hipStreamSynchronize(ctx->c.s0);
   if (ctx->c.s1) hipStreamSynchronize(ctx->c.s1);
@@ -1102,6 +1111,68 @@ int exafmm_graph_destroy(exafmm_graph *g) {
   if (!g) return EXAFMM_ERR_ARG;
   graph_release(g);
   return EXAFMM_OK;
+}
+
+
+int exafmm_evaluate_batch(exafmm_ctx *ctx, const exafmm_problem *problems, int count, int flags) {
+  return guard(ctx, [&] {
+    if (!ctx || !problems || count < 1) fail(EXAFMM_ERR_ARG, "bad argument");
+    const bool reuse = (flags & EXAFMM_BATCH_REUSE_GEOMETRY) != 0;
+    std::vector<std::pair<const void *, int64_t>> sig(count);
+    for (int i = 0; i < count; i++) {
+      const exafmm_problem &q = problems[i];
+      if (!q.bodies || q.n < 1) fail(EXAFMM_ERR_ARG, "a problem without bodies");
+      if (q.P < 2 || q.P > kMaxP) fail(EXAFMM_ERR_ARG, "P must be in [2, 16]");
+      sig[i] = {q.bodies, (q.n << 8) | (int64_t(q.P) << 2) | (q.on_device ? 1 : 0)};
+    }
+    if (reuse && ctx->batch_graph && sig == ctx->batch_sig) {  // same problems, positions kept
+      XF_HIP(hipSetDevice(ctx->c.cfg.device));
+      if (exafmm_graph_launch(ctx->batch_graph) != EXAFMM_OK) fail(EXAFMM_ERR_HIP, g_error);
+      if (exafmm_graph_synchronize(ctx->batch_graph) != EXAFMM_OK) fail(EXAFMM_ERR_HIP, g_error);
+      return;
+    }
+    if (ctx->batch_graph) exafmm_graph_destroy(ctx->batch_graph);
+    ctx->batch_graph = nullptr;
+    ctx->batch_sig.clear();
+    // one sub-context per slot, with this context's configuration and the
+    // problem's order
+    if (int(ctx->batch.size()) > count) {
+      for (size_t i = count; i < ctx->batch.size(); i++) exafmm_destroy(ctx->batch[i]);
+      ctx->batch.resize(count);
+    }
+    for (int i = 0; i < count; i++) {
+      if (i < int(ctx->batch.size()) && ctx->batch[i]->c.P == problems[i].P) continue;
+      exafmm_config cfg = ctx->c.cfg;
+      cfg.P = problems[i].P;
+      exafmm_ctx *sub = nullptr;
+      if (exafmm_create(&cfg, &sub) != EXAFMM_OK) fail(EXAFMM_ERR_ALLOC, g_error);
+      if (i < int(ctx->batch.size())) {
+        exafmm_destroy(ctx->batch[i]);
+        ctx->batch[i] = sub;
+      } else {
+        ctx->batch.push_back(sub);
+      }
+    }
+    for (int i = 0; i < count; i++) {
+      const exafmm_problem &q = problems[i];
+      if (exafmm_evaluate(ctx->batch[i], q.bodies, q.n, q.on_device) != EXAFMM_OK)
+        fail(EXAFMM_ERR_HIP, std::string("problem ") + std::to_string(i) + ": " + ctx->batch[i]->c.last_error);
+    }
+    for (int i = 0; i < count; i++) {
+      XF_HIP(hipStreamSynchronize(ctx->batch[i]->c.s1));
+      XF_HIP(hipStreamSynchronize(ctx->batch[i]->c.s0));
+    }
+    if (reuse) {
+      for (int i = 0; i < count; i++)
+        if (!problems[i].on_device) fail(EXAFMM_ERR_UNSUPPORTED, "reusing geometry needs device-resident bodies");
+      exafmm_graph *g = nullptr;
+      if (exafmm_graph_create_grouped(ctx->batch.data(), count, &g) != EXAFMM_OK &&
+          exafmm_graph_create(ctx->batch.data(), count, 8, &g) != EXAFMM_OK)
+        fail(EXAFMM_ERR_HIP, g_error);
+      ctx->batch_graph = g;
+      ctx->batch_sig = sig;
+    }
+  });
 }
 
 }  // extern "C"

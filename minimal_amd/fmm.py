@@ -42,11 +42,20 @@ EXPORTS = [
     "exafmm_get_order_model", "exafmm_order_training_set", "exafmm_mlp_forward", "exafmm_mlp_gradient",
     "exafmm_order_stats", "exafmm_graph_create", "exafmm_graph_create_grouped", "exafmm_graph_launch",
     "exafmm_graph_synchronize", "exafmm_graph_nodes", "exafmm_graph_gemm_shapes", "exafmm_graph_destroy",
+    "exafmm_evaluate_batch",
 ]
 
 MLP_PARAMS = 1579  # EXAFMM_MLP_PARAMS
 MLP_IN, MLP_OUT, ORDER_MIN = 4, 11, 4
 ORDER_FIXED, ORDER_LEARNED = 0, 1
+
+
+class Problem(C.Structure):
+    """exafmm_problem: one problem of exafmm_evaluate_batch."""
+    _fields_ = [("bodies", C.c_void_p), ("n", C.c_int64), ("P", C.c_int32), ("on_device", C.c_int32)]
+
+
+BATCH_REUSE_GEOMETRY = 1
 
 
 class Config(C.Structure):
@@ -114,6 +123,7 @@ class Library:
         L.exafmm_graph_synchronize.argtypes = [_P]
         L.exafmm_graph_nodes.argtypes = [_P, C.POINTER(C.c_int64)]
         L.exafmm_graph_gemm_shapes.argtypes = [_P, _P, _I, C.POINTER(C.c_int)]
+        L.exafmm_evaluate_batch.argtypes = [_P, _P, _I, _I]
         L.exafmm_graph_destroy.argtypes = [_P]
 
     def device_count(self) -> int:
@@ -201,6 +211,26 @@ class FMM:
         nt = len(targets) if nt is None else nt
         ns = len(sources) if ns is None else ns
         self._chk(self.L.lib.exafmm_direct(self.h, _ptr(targets), nt, _ptr(sources), ns, int(on_device)))
+
+    def evaluate_batch(self, problems, reuse_geometry=False):
+        """Evaluate many problems [(bodies, P) or (device pointer, n, P)] with
+        this context's configuration (exafmm_evaluate_batch).  Host numpy
+        arrays or device pointers (then every problem must be on the device).
+        With reuse_geometry, later calls with the same problems replay a HIP
+        graph (positions kept, charges read afresh)."""
+        arr = (Problem * len(problems))()
+        keep = []
+        for k, pr in enumerate(problems):
+            if len(pr) == 2:
+                b, P = pr
+                keep.append(b)
+                arr[k] = Problem(_ptr(b).value, len(b), P, 0)
+            else:
+                ptr, n, P = pr
+                arr[k] = Problem(ptr, n, P, 1)
+        self._keep_batch = (arr, keep)
+        self._chk(self.L.lib.exafmm_evaluate_batch(self.h, C.cast(arr, C.c_void_p), len(problems),
+                                                   BATCH_REUSE_GEOMETRY if reuse_geometry else 0))
 
     def dipole_correction(self):
         self._chk(self.L.lib.exafmm_dipole_correction(self.h))

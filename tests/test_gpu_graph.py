@@ -211,3 +211,42 @@ def test_grouped_graph_rejects_other_precisions(gpu):
     with pytest.raises(fmm.ExaFMMError):
         fmm.Graph([f], grouped=True)
     fmm.Graph([f]).close()  # the per-problem graph takes it
+
+
+def test_evaluate_batch_matches_single_contexts(gpu):
+    """exafmm_evaluate_batch on host arrays equals per-problem contexts bit
+    for bit; on device arrays with reused geometry the results are the
+    recorded graph's (1e-6 of the eager ones), and its replay after doubling
+    the charges doubles p and F (1e-6)."""
+    probs = _problems()
+    single = []
+    for b, P in probs:
+        f = fmm.FMM(P=P, ncrit=64, theta=0.4)
+        bb = b.copy()
+        f.evaluate(bb)
+        single.append(bb)
+    host = [b.copy() for b, _ in probs]
+    batch = fmm.FMM(ncrit=64, theta=0.4)
+    batch.evaluate_batch([(h, P) for h, (_, P) in zip(host, probs)])
+    for h, s in zip(host, single):
+        assert np.array_equal(h["X"], s["X"])
+        assert np.array_equal(h["p"], s["p"]) and np.array_equal(h["F"], s["F"])
+    devs = [torch.from_numpy(b.view(np.uint8).copy()).cuda() for b, _ in probs]
+    spec = [(d.data_ptr(), len(b), P) for d, (b, P) in zip(devs, probs)]
+    batch.evaluate_batch(spec, reuse_geometry=True)
+    # recording the graph ends with one pass of it, so these are the graph's
+    # results (grouped GEMM shapes: FP32 partial sums in another order)
+    first = [_host(d) for d in devs]
+    for h, s in zip(first, single):
+        assert np.array_equal(h["X"], s["X"])
+        assert rel_l2(h["p"], s["p"]) <= 1e-6 and rel_l2(h["F"], s["F"]) <= 1e-6
+    for d in devs:
+        h = d.cpu().numpy().view(BODY_DTYPE)
+        h["q"] *= 2.0
+        d.copy_(torch.from_numpy(h.view(np.uint8).copy()).cuda())
+    torch.cuda.synchronize()
+    batch.evaluate_batch(spec, reuse_geometry=True)  # graph replay
+    for d, f in zip(devs, first):
+        h = _host(d)
+        assert rel_l2(h["p"], 2.0 * f["p"]) <= 1e-6 and rel_l2(h["F"], 2.0 * f["F"]) <= 1e-6
+    batch.close()
