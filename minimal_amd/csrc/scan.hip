@@ -72,8 +72,10 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(ScanArrays<NA> a, i
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NA; k++) {
-    int carry = sums[kScanMaxBlocks * NA + k];
-    {
+    // one workgroup (sums == nullptr): no reduce pass, the running total
+    // comes straight from base
+    int carry = sums ? sums[kScanMaxBlocks * NA + k] : (base ? base[k] : 0);
+    if (sums) {
       int v = 0;
       for (int j = threadIdx.x; j < blockIdx.x; j += kScanThreads) v += sums[j * NA + k];
       carry += block_sum(v, red);
@@ -99,6 +101,7 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(ScanArrays<NA> a, i
       carry += tile;
       __syncthreads();
     }
+    __syncthreads();  // every thread has read base[k] before it is overwritten
     if (blockIdx.x == G - 1 && threadIdx.x == 0) {
       if (total) total[k] = carry;
       if (base) base[k] = carry;
@@ -112,11 +115,13 @@ static void scan_launch(const ScanArrays<NA> &a, int n_host, const int *n_dev, i
                         const int64_t *cap, int *flag, Buf<int> &scratch, hipStream_t s) {
   const int G = std::max(1, std::min(kScanMaxBlocks, ceil_div(std::max(n_bound, 1), 8 * kScanThreads)));
   scratch.ensure(size_t(kScanMaxBlocks + 1) * NA);
-  hipLaunchKernelGGL(k_scan_reduce<NA>, dim3(G), dim3(kScanThreads), 0, s, a, n_host, n_dev, scratch.p,
-                     (const int *)base);
-  XF_LAUNCH();
-  hipLaunchKernelGGL(k_scan_apply<NA>, dim3(G), dim3(kScanThreads), 0, s, a, n_host, n_dev, scratch.p, total, base,
-                     cap, flag);
+  if (G > 1) {
+    hipLaunchKernelGGL(k_scan_reduce<NA>, dim3(G), dim3(kScanThreads), 0, s, a, n_host, n_dev, scratch.p,
+                       (const int *)base);
+    XF_LAUNCH();
+  }
+  hipLaunchKernelGGL(k_scan_apply<NA>, dim3(G), dim3(kScanThreads), 0, s, a, n_host, n_dev,
+                     G > 1 ? (const int *)scratch.p : nullptr, total, base, cap, flag);
   XF_LAUNCH();
 }
 
