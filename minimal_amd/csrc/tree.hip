@@ -189,7 +189,8 @@ __global__ void __launch_bounds__(1024) k_hist_top(int *__restrict__ hist, int *
 __global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__restrict__ lcount,
                                                const int *__restrict__ lstart, const int *__restrict__ cbeg,
                                                const int *__restrict__ ccnt,
-                                               const unsigned long long *__restrict__ key, int *__restrict__ nch,
+                                               const unsigned long long *__restrict__ key,
+                                               const int *__restrict__ hist, int *__restrict__ nch,
                                                int *__restrict__ bnd) {
   const int t = blockIdx.x * 256 + threadIdx.x;
   if (t >= lcount[l]) return;
@@ -203,16 +204,24 @@ __global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__re
   int b[9];
   b[0] = beg;
   b[8] = beg + cnt;
-  for (int o = 1; o < 8; o++) {
-    int lo = b[o - 1], hi = beg + cnt;  // first index with digit >= o
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (int((key[mid] >> sh) & 7ull) < o)
-        lo = mid + 1;
-      else
-        hi = mid;
+  if (l + 1 <= kHistLevels) {
+    // the children's populations are the level-(l+1) prefix histogram
+    // entries of this node's prefix (all its bodies share it)
+    const unsigned long long pre = key[beg] >> (sh + 3);
+    const int *h = hist + hist_off(l + 1) + int(pre) * 8;
+    for (int o = 1; o < 8; o++) b[o] = b[o - 1] + h[o - 1];
+  } else {
+    for (int o = 1; o < 8; o++) {
+      int lo = b[o - 1], hi = beg + cnt;  // first index with digit >= o
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (int((key[mid] >> sh) & 7ull) < o)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      b[o] = lo;
     }
-    b[o] = lo;
   }
   int k = 0;
   for (int o = 0; o < 8; o++) {
@@ -436,7 +445,7 @@ void tree_build(Ctx &c) {
   for (int l = 0; l < Ds; l++) {
     const int g = ceil_div(ub[l], 256);
     hipLaunchKernelGGL(k_split, dim3(g), dim3(256), 0, s, l, ncrit, t.lcount.p, t.lstart_d.p, t.cbeg.p, t.ccnt.p, k1,
-                       t.tmp_nch.p, t.tmp_bnd.p);
+                       t.hist.p, t.tmp_nch.p, t.tmp_bnd.p);
     XF_LAUNCH();
     scan_exclusive_dev(t.tmp_nch.p, t.tmp_off.p, t.lcount.p + l, int(ub[l]), t.misc.p + 16, t.scan_tmp, s);
     hipLaunchKernelGGL(k_level_next, dim3(1), dim3(1), 0, s, l, t.lcount.p, t.lstart_d.p, t.misc.p + 16);
