@@ -128,7 +128,7 @@ void do_lists(Ctx &tc, const Ctx &sc) {
 }
 
 // P2P on the second stream from the current point of the main stream, so it
-// overlaps M2L (VALU against MFMA work).
+// overlaps whatever the main stream runs next (upward pass, M2L).
 void do_p2p(Ctx &tc, const Ctx &sc) {
   rec(tc, 14, tc.s0);
   XF_HIP(hipStreamWaitEvent(tc.s1, tc.ev[14], 0));
@@ -327,24 +327,13 @@ int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_devi
     set_bodies(c, bodies, n, on_device);
     do_build(c);
     do_lists(c, c);
-    if (c.cfg.m2l_precision != EXAFMM_PREC_FP64 && c.lists.nkeys > 0) {
-      // s0: upward pass, then the keyed M2L once its tiles are ready;
-      // s1: M2L tiles and operators (geometry only), then P2P
-      rec(c, 15, c.s0);
-      XF_HIP(hipStreamWaitEvent(c.s1, c.ev[15], 0));
-      do_upward(c);
-      m2l_prepare(c, c.s1);
-      rec(c, 7, c.s1);
-      rec(c, 5, c.s1);
-      p2p_apply(c, c, c.s1);
-      rec(c, 6, c.s1);
-      XF_HIP(hipStreamWaitEvent(c.s0, c.ev[7], 0));
-      do_far(c, c);
-    } else {
-      do_upward(c);
-      do_p2p(c, c);
-      do_far(c, c);
-    }
+    // P2P needs only the lists: it starts on s1 at once, and its long
+    // VALU-bound blocks fill the CUs that the latency-bound chains on s0
+    // (upward pass, M2L tiles and operators) leave idle; the keyed GEMM
+    // follows on s0
+    do_p2p(c, c);
+    do_upward(c);
+    do_far(c, c);
     do_downward(c);
     bodies_to_host(c);
   });
@@ -838,7 +827,7 @@ void grouped_tables(exafmm_graph *g) {
   per_problem(g->p2p, [](const Ctx &c) { return int64_t(p2p_group_blocks(c.t.nleaves)); });
   per_problem(g->l2p, [](const Ctx &c) { return int64_t(c.t.nleaves); });
   // per-level stages: the problems that have parents at level l
-  auto per_level = [&](int l, int mult) {
+  auto per_level = [&](int l, int mult, int (*rows)(int)) {
     exafmm_graph::Stage sg;
     sg.off = int(lv.size());
     const int soff = int(st.size());
@@ -848,7 +837,7 @@ void grouped_tables(exafmm_graph *g) {
       if (l >= c.t.depth) continue;
       const int cnt = c.t.lstart[l + 1] - c.t.lstart[l];
       if (cnt <= 0) continue;
-      GroupLevel e{i, c.t.lstart[l], cnt, ceil_div(cnt, 16), shift_group_rows(c.P)};
+      GroupLevel e{i, c.t.lstart[l], cnt, ceil_div(cnt, 16), rows(c.P)};
       lv.push_back(e);
       st.push_back(int(acc));
       acc += int64_t(e.nbx) * e.nby * mult;
@@ -860,8 +849,8 @@ void grouped_tables(exafmm_graph *g) {
     return std::make_pair(sg, soff);
   };
   std::vector<std::pair<exafmm_graph::Stage, int>> m2m, l2l;
-  for (int l = maxd - 1; l >= 0; l--) m2m.push_back(per_level(l, 1));
-  for (int l = 0; l < maxd; l++) l2l.push_back(per_level(l, 8));
+  for (int l = maxd - 1; l >= 0; l--) m2m.push_back(per_level(l, 1, m2m_group_rows));
+  for (int l = 0; l < maxd; l++) l2l.push_back(per_level(l, 8, shift_group_rows));
   g->gt.ensure(n);
   g->lv.ensure(std::max<size_t>(lv.size(), 1));
   g->starts.ensure(st.size());

@@ -235,46 +235,54 @@ __device__ inline int child_in(int p, int o, const int *__restrict__ cchild, con
 }
 
 // M~_parent = sum over children of T_M2M[octant] M~_child (3d/kernel.h:206).
-// grid: (parent tiles of 16, row blocks of 64); K runs over (octant, column).
+// grid: (parent tiles of 16, row tiles of 16).  The eight waves of a block
+// take one octant each (K = that octant's columns), so a small level's
+// launch waits for one octant's chain of loads, not eight; the eight
+// partial tiles are then added in LDS in octant order.
+constexpr int kM2MWaves = 8;
 __device__ __attribute__((always_inline)) inline void m2m_body(int bx, int by, int P, int lstart, int lcount,
                                 const int *__restrict__ cchild,
                                 const int *__restrict__ cnch,
                                 const double4 *__restrict__ cx,
                                 const double *__restrict__ T, double *__restrict__ M) {
   __shared__ int chl[8][16];
+  __shared__ v4d part[kM2MWaves][kWave];
   const int PP = P * P;
-  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave, o = threadIdx.x / kWave;
   const int j16 = lane & 15, kq = lane >> 4;
   const int p0 = bx * 16;
   if (threadIdx.x < 128) {
-    const int o = threadIdx.x / 16, q = threadIdx.x % 16;
-    chl[o][q] = p0 + q < lcount ? child_in(lstart + p0 + q, o, cchild, cnch, cx) : -1;
+    const int oo = threadIdx.x / 16, q = threadIdx.x % 16;
+    chl[oo][q] = p0 + q < lcount ? child_in(lstart + p0 + q, oo, cchild, cnch, cx) : -1;
   }
   __syncthreads();
-  const int r0 = (by * kShiftWaves + w) * 16;
-  if (r0 >= PP) return;
+  const int r0 = by * 16;  // the grid has ceil(PP / 16) row tiles: r0 < PP
   const int jmax = degree_of(min(r0 + 16, PP) - 1);
   const int ncol = (jmax + 1) * (jmax + 1);
   const int ksteps = (ncol + 3) / 4;
   const int row = min(r0 + j16, PP - 1);  // rows past PP compute garbage that is never stored
+  const int ch = chl[o][j16];
+  const double *To = T + int64_t(o) * PP * PP;
+  const double *Mc = M + int64_t(ch < 0 ? 0 : ch) * PP;
   v4d acc = {0, 0, 0, 0};
-  for (int o = 0; o < 8; o++) {
-    const int ch = chl[o][j16];
-    const double *To = T + int64_t(o) * PP * PP;
-    const double *Mc = M + int64_t(ch < 0 ? 0 : ch) * PP;
-    for (int k0 = 0; k0 < ksteps; k0 += kShiftBatch) {
-      double a[kShiftBatch], b[kShiftBatch];
+  for (int k0 = 0; k0 < ksteps; k0 += kShiftBatch) {
+    double a[kShiftBatch], b[kShiftBatch];
 #pragma unroll
-      for (int s = 0; s < kShiftBatch; s++) {
-        const int col = (k0 + s) * 4 + kq;
-        const bool in = col < ncol;
-        a[s] = in ? To[int64_t(col) * PP + row] : 0.0;
-        b[s] = (in && ch >= 0) ? Mc[col] : 0.0;
-      }
-#pragma unroll
-      for (int s = 0; s < kShiftBatch; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+    for (int s = 0; s < kShiftBatch; s++) {
+      const int col = (k0 + s) * 4 + kq;
+      const bool in = col < ncol;
+      a[s] = in ? To[int64_t(col) * PP + row] : 0.0;
+      b[s] = (in && ch >= 0) ? Mc[col] : 0.0;
     }
+#pragma unroll
+    for (int s = 0; s < kShiftBatch; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
   }
+  part[o][lane] = acc;
+  __syncthreads();
+  if (o != 0) return;
+  v4d sum = part[0][lane];
+#pragma unroll
+  for (int oo = 1; oo < kM2MWaves; oo++) sum += part[oo][lane];
   const int pl = p0 + j16;
   if (pl >= lcount) return;
   const int p = lstart + pl;
@@ -282,21 +290,21 @@ __device__ __attribute__((always_inline)) inline void m2m_body(int bx, int by, i
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const int r = r0 + kq + 4 * i;
-    if (r < PP) M[int64_t(p) * PP + r] = acc[i];
+    if (r < PP) M[int64_t(p) * PP + r] = sum[i];
   }
 }
 
-__global__ void __launch_bounds__(kShiftWaves *kWave) k_m2m(int P, int lstart, int lcount,
-                                                            const int *__restrict__ cchild,
-                                                            const int *__restrict__ cnch,
-                                                            const double4 *__restrict__ cx,
-                                                            const double *__restrict__ T, double *__restrict__ M) {
+__global__ void __launch_bounds__(kM2MWaves *kWave) k_m2m(int P, int lstart, int lcount,
+                                                          const int *__restrict__ cchild,
+                                                          const int *__restrict__ cnch,
+                                                          const double4 *__restrict__ cx,
+                                                          const double *__restrict__ T, double *__restrict__ M) {
   m2m_body(blockIdx.x, blockIdx.y, P, lstart, lcount, cchild, cnch, cx, T, M);
 }
 
 // One level of every problem of a batch in one launch: lv[i] describes
 // problem lv[i].prob's cells of this level as nbx x nby blocks.
-__global__ void __launch_bounds__(kShiftWaves *kWave) k_m2m_grouped(const GroupTree *__restrict__ g,
+__global__ void __launch_bounds__(kM2MWaves *kWave) k_m2m_grouped(const GroupTree *__restrict__ g,
                                                                     const GroupLevel *__restrict__ lv,
                                                                     const int *__restrict__ start, int n) {
   const int b = blockIdx.x, i = group_of(start, n, b), lb = b - start[i];
@@ -483,13 +491,14 @@ __global__ void __launch_bounds__(64) k_l2p_grouped(const GroupTree *__restrict_
 // ---------------------------------------------------------------- grouped launches
 int p2m_group_blocks(int64_t nleaves) { return ceil_div(nleaves, kP2MWaves); }
 int shift_group_rows(int P) { return ceil_div(P * P, 16 * kShiftWaves); }
+int m2m_group_rows(int P) { return ceil_div(P * P, 16); }
 
 void p2m_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, hipStream_t s) {
   hipLaunchKernelGGL(k_p2m_grouped, dim3(nblocks), dim3(kP2MWaves * kWave), 0, s, g, start, nprob);
   XF_LAUNCH();
 }
 void m2m_grouped(const GroupTree *g, const GroupLevel *lv, const int *start, int n, int nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(k_m2m_grouped, dim3(nblocks), dim3(kShiftWaves * kWave), 0, s, g, lv, start, n);
+  hipLaunchKernelGGL(k_m2m_grouped, dim3(nblocks), dim3(kM2MWaves * kWave), 0, s, g, lv, start, n);
   XF_LAUNCH();
 }
 void l2l_grouped(const GroupTree *g, const GroupLevel *lv, const int *start, int n, int nblocks, hipStream_t s) {
@@ -530,7 +539,7 @@ void upward(Ctx &c) {
   XF_LAUNCH();
   for (int l = t.depth - 1; l >= 0; l--) {
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_m2m, dim3(ceil_div(cnt, 16), ceil_div(PP, 16 * kShiftWaves)), dim3(kShiftWaves * kWave), 0,
+    hipLaunchKernelGGL(k_m2m, dim3(ceil_div(cnt, 16), m2m_group_rows(P)), dim3(kM2MWaves * kWave), 0,
                        s, c.P, t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
     XF_LAUNCH();
   }

@@ -249,6 +249,7 @@ void p2p_apply(Ctx &tgt, const Ctx &src, hipStream_t s);
 // grouped launches over the problems of a batch (exafmm_graph_create)
 int p2m_group_blocks(int64_t nleaves);
 int shift_group_rows(int P);
+int m2m_group_rows(int P);
 int p2p_group_blocks(int64_t nleaves);
 void p2m_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, hipStream_t s);
 void m2m_grouped(const GroupTree *g, const GroupLevel *lv, const int *start, int n, int nblocks, hipStream_t s);
