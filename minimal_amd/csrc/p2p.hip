@@ -9,8 +9,8 @@
 // list entries are staged in LDS (source range and the shift of the source
 // leaf into the target leaf's frame, periodic image included) with a prefix
 // of their body counts; lane l takes source 64k + l of the concatenated
-// list in round k.  Targets are wave-uniform (scalar registers), 16 at a
-// time, so each interaction reads nothing from LDS: per lane and target it
+// list in round k.  Targets are wave-uniform, 16 at a time (8 or 4 for the
+// last ones of a leaf), so each interaction reads nothing from LDS: per lane and target it
 // accumulates p and the three force components over its sources, and the
 // 64 lane partials are added with a fixed butterfly at the end.
 // Coordinates are stored relative to their own leaf centre, so every
@@ -209,10 +209,10 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
       b1 = step(k0 + kWave, b1, std::false_type());
     }
   }
-  // 64-lane sums: halving over the targets, then a butterfly over the lanes
-  // that share a target; lane l writes component l % 4 of target l / 4
-  // (TC = 16) or component l % 4 of target l / 8 for the lanes l % 8 < 4 (TC = 8)
-  constexpr int kLow = TC == 16 ? 4 : 8;  // lanes sharing a target after the halving
+  // 64-lane sums: halving over the targets, then a butterfly over the 64 / TC
+  // lanes that share a target; lane l writes component l % (64 / TC) of
+  // target l / (64 / TC) when that remainder is below 4
+  constexpr int kLow = kWave / TC;  // lanes sharing a target after the halving
   float sp = halving<TC>(ap, lane), sx = halving<TC>(ax, lane), sy = halving<TC>(ay, lane),
         sz = halving<TC>(az, lane);
 #pragma unroll
@@ -260,8 +260,11 @@ __device__ __attribute__((always_inline)) inline void p2p32_body(int bid, const 
     if (tc > kP2PTC / 2)
       p2p_block<kP2PTC>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle, s_pre[w],
                         s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
-    else
+    else if (tc > kP2PTC / 4)
       p2p_block<kP2PTC / 2>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle,
+                            s_pre[w], s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
+    else  // the last 1..4 targets of a leaf (a quarter of the leaves at 32 bodies each)
+      p2p_block<kP2PTC / 4>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle,
                             s_pre[w], s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
   }
 }

@@ -301,6 +301,8 @@ void traverse(Ctx &tc, const Ctx &sc) {
   if (Lr.cap_out < 96 * nc) Lr.cap_out = 96 * nc;
   if (keys && Lr.hcap == 0) Lr.hcap = 1 << 16;
 
+  int tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  bool done = false;
   for (int attempt = 0; attempt < 4; attempt++) {
     Lr.m2l.ensure(Lr.cap_m2l);
     Lr.p2p.ensure(Lr.cap_p2p);
@@ -357,7 +359,16 @@ void traverse(Ctx &tc, const Ctx &sc) {
       hipLaunchKernelGGL(k_traverse<true>, dim3(g), dim3(kTravWaves * kWave), 0, s, a);
       XF_LAUNCH();
     }
-    int tot[8];
+    if (keys) {
+      // distinct translation keys out of the hash table (discarded if the
+      // pass has to be repeated), counted into tot[6]
+      Lr.keys.ensure(size_t(Lr.hcap) * 2);
+      Lr.kscratch.ensure(size_t(Lr.hcap) * 2 + 8);
+      Lr.slot2id.ensure(Lr.hcap);
+      hipLaunchKernelGGL(k_hash_collect, dim3(ceil_div(Lr.hcap, 256)), dim3(256), 0, s, Lr.hslots.p, Lr.hcap,
+                         Lr.keys.p, Lr.kscratch.p, Lr.tot.p + 6);
+      XF_LAUNCH();
+    }
     XF_HIP(hipMemcpyAsync(tot, Lr.tot.p, sizeof(tot), hipMemcpyDeviceToHost, s));
     XF_HIP(hipStreamSynchronize(s));
     const int flags = tot[3];
@@ -375,24 +386,17 @@ void traverse(Ctx &tc, const Ctx &sc) {
     }
     Lr.nm2l = tot[0];
     Lr.np2p = tot[1];
+    done = true;
     break;
   }
+  if (!done) fail(EXAFMM_ERR_CAPACITY, "interaction lists or key table still full after 4 passes");
   Lr.valid = true;
   Lr.nkeys = 0;
   if (keys) {
-    Lr.keys.ensure(size_t(Lr.hcap) * 2);
-    Lr.kscratch.ensure(size_t(Lr.hcap) * 2 + 8);
-    Lr.slot2id.ensure(Lr.hcap);
-    int *cnt = Lr.tot.p + 6;
-    XF_HIP(hipMemsetAsync(cnt, 0, sizeof(int), s));
+    const int *cnt = Lr.tot.p + 6;
     unsigned long long *k1 = Lr.keys.p, *k2 = Lr.keys.p + Lr.hcap;
     int *v1 = Lr.kscratch.p, *v2 = Lr.kscratch.p + Lr.hcap;
-    hipLaunchKernelGGL(k_hash_collect, dim3(ceil_div(Lr.hcap, 256)), dim3(256), 0, s, Lr.hslots.p, Lr.hcap, k1, v1,
-                       cnt);
-    XF_LAUNCH();
-    int nk = 0;
-    XF_HIP(hipMemcpyAsync(&nk, cnt, sizeof(int), hipMemcpyDeviceToHost, s));
-    XF_HIP(hipStreamSynchronize(s));
+    const int nk = tot[6];
     if (nk <= kRankMax) {
       hipLaunchKernelGGL(k_key_rank, dim3(1), dim3(1024), 0, s, k1, v1, cnt, Lr.slot2id.p);
       XF_LAUNCH();

@@ -247,11 +247,15 @@ __global__ void __launch_bounds__(256) k_emit(int l, const int *__restrict__ lco
                                               double4 *__restrict__ cx, int *__restrict__ cbeg,
                                               int *__restrict__ ccnt, int *__restrict__ cchild,
                                               int *__restrict__ cnch, int *__restrict__ cparent,
-                                              int *__restrict__ clevel) {
+                                              int *__restrict__ clevel, int *__restrict__ nsplit) {
   const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= lcount[l]) return;
+  const bool in = t < lcount[l];
+  const int k = in ? nch[t] : 0;
+  // cells that split, counted per wave (leaves = cells - split cells)
+  const unsigned long long sp = __ballot(k > 0);
+  if ((threadIdx.x & (kWave - 1)) == 0 && sp) atomicAdd(nsplit, int(__popcll(sp)));
+  if (!in) return;
   const int c = lstart[l] + t;
-  const int k = nch[t];
   cnch[c] = k;
   if (k == 0) {
     cchild[c] = -1;
@@ -364,7 +368,8 @@ void tree_build(Ctx &c) {
   const int ncrit = c.cfg.ncrit;
   if (n > (int64_t(1) << 30)) fail(EXAFMM_ERR_ARG, "N above 2^30 is not supported");
 
-  // misc layout (ints): [0] flags, [1..8] level max, [16..] totals
+  // misc layout (ints): [0] flags, [1..8] level max, [16] level total,
+  // [17] leaves, [18] cells that split
   t.misc.ensure(64);
   t.bounds_part.ensure(256 * 6 + 8);
   double *root = t.bounds_part.p + 256 * 6;
@@ -452,17 +457,17 @@ void tree_build(Ctx &c) {
     XF_LAUNCH();
     hipLaunchKernelGGL(k_emit, dim3(g), dim3(256), 0, s, l, t.lcount.p, t.lstart_d.p, t.tmp_nch.p, t.tmp_off.p,
                        t.tmp_bnd.p, root, t.cx.p, t.cbeg.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.cparent.p,
-                       t.clevel.p);
+                       t.clevel.p, t.misc.p + 18);
     XF_LAUNCH();
   }
   hipLaunchKernelGGL(k_last_level, dim3(ceil_div(ub[Ds], 256)), dim3(256), 0, s, Ds, ncrit, t.lcount.p,
                      t.lstart_d.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.misc.p);
   XF_LAUNCH();
 
-  // sync 2: level sizes
-  std::vector<int> lc(kMaxLevel + 2), fl(1);
+  // sync 2: level sizes, flags, number of cells that split
+  std::vector<int> lc(kMaxLevel + 2), fl(19);
   XF_HIP(hipMemcpyAsync(lc.data(), t.lcount.p, sizeof(int) * (kMaxLevel + 2), hipMemcpyDeviceToHost, s));
-  XF_HIP(hipMemcpyAsync(fl.data(), t.misc.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipMemcpyAsync(fl.data(), t.misc.p, sizeof(int) * 19, hipMemcpyDeviceToHost, s));
   XF_HIP(hipStreamSynchronize(s));
   if (fl[0] & 2) fail(EXAFMM_ERR_DEPTH, "more than ncrit bodies share a 21-level octant path");
   t.lstart.assign(1, 0);
@@ -496,10 +501,9 @@ void tree_build(Ctx &c) {
   XF_LAUNCH();
   hipLaunchKernelGGL(k_writeback, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, t.xs.p, c.bodies);
   XF_LAUNCH();
-  int nl = 0;
-  XF_HIP(hipMemcpyAsync(&nl, t.misc.p + 17, sizeof(int), hipMemcpyDeviceToHost, s));
-  XF_HIP(hipStreamSynchronize(s));
-  t.nleaves = nl;
+  // every cell either split or is a leaf (the scan above writes the same
+  // count to misc[17] for the kernels)
+  t.nleaves = nc - fl[18];
   t.built = true;
 }
 
