@@ -221,17 +221,27 @@ __device__ inline int degree_of(int row) {
   return j;
 }
 
-// Child of parent p in octant o, or -1 (the octant is recomputed from the
-// centres, strict >, 3d/build_tree.h:50-56).
-__device__ inline int child_in(int p, int o, const int *__restrict__ cchild, const int *__restrict__ cnch,
-                               const double4 *__restrict__ cx) {
+// Children of the parents p0 .. p0 + 15 of a level by octant: thread t < 128
+// takes child t % 8 of parent p0 + t / 8 and files it under the octant its
+// centre lies in relative to the parent's (strict >, 3d/build_tree.h:50-56);
+// octants without a child keep -1.  Every load is issued at once (no walk
+// over the children per octant).  `only` >= 0 keeps that octant alone, in
+// row 0.  The caller fills chl with -1 and puts a barrier before and after.
+__device__ inline void file_children(int tid, int p0, int lstart, int lcount, const int *__restrict__ cchild,
+                                     const int *__restrict__ cnch, const double4 *__restrict__ cx, int only,
+                                     int (*chl)[16]) {
+  if (tid >= 128) return;
+  const int q = tid >> 3, j = tid & 7;
+  if (p0 + q >= lcount) return;
+  const int p = lstart + p0 + q;
   const int nch = cnch[p], first = cchild[p];
-  const double4 Cp = cx[p];
-  for (int ch = first; ch < first + nch; ch++) {
-    const double4 Cc = cx[ch];
-    if ((Cc.x > Cp.x) + ((Cc.y > Cp.y) << 1) + ((Cc.z > Cp.z) << 2) == o) return ch;
-  }
-  return -1;
+  if (j >= nch) return;
+  const double4 Cp = cx[p], Cc = cx[first + j];
+  const int o = (Cc.x > Cp.x) + ((Cc.y > Cp.y) << 1) + ((Cc.z > Cp.z) << 2);
+  if (only < 0)
+    chl[o][q] = first + j;
+  else if (o == only)
+    chl[0][q] = first + j;
 }
 
 // M~_parent = sum over children of T_M2M[octant] M~_child (3d/kernel.h:206).
@@ -251,10 +261,9 @@ __device__ __attribute__((always_inline)) inline void m2m_body(int bx, int by, i
   const int lane = threadIdx.x % kWave, o = threadIdx.x / kWave;
   const int j16 = lane & 15, kq = lane >> 4;
   const int p0 = bx * 16;
-  if (threadIdx.x < 128) {
-    const int oo = threadIdx.x / 16, q = threadIdx.x % 16;
-    chl[oo][q] = p0 + q < lcount ? child_in(lstart + p0 + q, oo, cchild, cnch, cx) : -1;
-  }
+  if (threadIdx.x < 128) chl[threadIdx.x / 16][threadIdx.x % 16] = -1;
+  __syncthreads();
+  file_children(threadIdx.x, p0, lstart, lcount, cchild, cnch, cx, -1, chl);
   __syncthreads();
   const int r0 = by * 16;  // the grid has ceil(PP / 16) row tiles: r0 < PP
   const int jmax = degree_of(min(r0 + 16, PP) - 1);
@@ -322,13 +331,14 @@ __device__ __attribute__((always_inline)) inline void l2l_body(int bx, int by, i
                                 const int *__restrict__ cnch,
                                 const double4 *__restrict__ cx,
                                 const double *__restrict__ T, double *__restrict__ L) {
-  __shared__ int chl[16];
+  __shared__ int chl[1][16];
   const int PP = P * P;
   const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
   const int j16 = lane & 15, kq = lane >> 4;
   const int p0 = bx * 16, o = bz;
-  if (threadIdx.x < 16)
-    chl[threadIdx.x] = p0 + threadIdx.x < lcount ? child_in(lstart + p0 + threadIdx.x, o, cchild, cnch, cx) : -1;
+  if (threadIdx.x < 16) chl[0][threadIdx.x] = -1;
+  __syncthreads();
+  file_children(threadIdx.x, p0, lstart, lcount, cchild, cnch, cx, o, chl);
   __syncthreads();
   const int r0 = (by * kShiftWaves + w) * 16;
   if (r0 >= PP) return;
@@ -336,7 +346,7 @@ __device__ __attribute__((always_inline)) inline void l2l_body(int bx, int by, i
   const int c0 = jmin * jmin;
   const int ksteps = (PP - c0 + 3) / 4;
   const int row = min(r0 + j16, PP - 1);
-  const int ch = chl[j16];
+  const int ch = chl[0][j16];
   const double *To = T + (int64_t(8) + o) * PP * PP;
   const int pl = min(p0 + j16, lcount - 1);
   const double *Lp = L + int64_t(lstart + pl) * PP;

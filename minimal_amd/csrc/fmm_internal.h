@@ -133,6 +133,7 @@ struct Tree {
   Buf<unsigned long long> key, key2;
   Buf<int> val, val2;
   Buf<int> hist;       // dense prefix histograms, levels 1..7
+  Buf<int> hist_bmax;  // per-block largest populations of levels 3..7
   Buf<int> lcount, lstart_d, tmp_nch, tmp_off, tmp_bnd, misc;
   Buf<double> bounds_part;
   Buf<int> rs_hist;

@@ -123,62 +123,99 @@ __global__ void __launch_bounds__(256) k_keys(const exafmm_body *__restrict__ b,
   atomicAdd(&hist7[int(k >> (3 * (kMaxLevel - kHistLevels)))], 1);
 }
 
-// Prefix populations of levels 6 and 5 by summing children (prefixes are
-// in Morton order, so a level-l prefix p has children 8p..8p+7), and the
-// largest population per level: block maxima, one atomic per block.
-__global__ void __launch_bounds__(256) k_hist_up(int *__restrict__ hist, int *__restrict__ lmax) {
-  __shared__ int m[3][256];
-  const int p5 = blockIdx.x * 256 + threadIdx.x;  // one thread per level-5 prefix
-  const int *h7 = hist + hist_off(7) + p5 * 64;
-  int s5 = 0, m6 = 0, m7 = 0;
-  for (int c6 = 0; c6 < 8; c6++) {
-    int s6 = 0;
-    for (int c7 = 0; c7 < 8; c7++) {
-      const int v = h7[c6 * 8 + c7];
-      s6 += v;
-      m7 = max(m7, v);
-    }
-    hist[hist_off(6) + p5 * 8 + c6] = s6;
-    m6 = max(m6, s6);
-    s5 += s6;
+// Levels 6..3 of the prefix histogram from level 7 (prefixes are in Morton
+// order, so prefix p of level l has children 8p..8p+7 at level l+1).  Block
+// b sums the 4096 level-7 bins under level-3 prefix b: thread t loads bins
+// 4t..4t+3 in one 16-byte load, lane shuffles give levels 6 and 5, LDS
+// levels 4 and 3.  The block's largest population per level goes to
+// bmax[8b + l] (l = 3..7).
+static_assert(kHistLevels == 7, "k_hist_pyramid is shaped for 8^7 level-7 bins");
+__global__ void __launch_bounds__(1024) k_hist_pyramid(int *__restrict__ hist, int *__restrict__ bmax) {
+  __shared__ int w4[16], wm[16][3], s4[8];
+  const int t = threadIdx.x, b = blockIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const int4 v = reinterpret_cast<const int4 *>(hist + hist_off(7))[b * 1024 + t];
+  int m7 = max(max(v.x, v.y), max(v.z, v.w));
+  int s6 = v.x + v.y + v.z + v.w;
+  s6 += __shfl_xor(s6, 1);  // 8 level-7 bins: 2 threads
+  int s5 = s6 + __shfl_xor(s6, 2);
+  s5 += __shfl_xor(s5, 4);
+  s5 += __shfl_xor(s5, 8);  // 64 bins: 16 threads
+  int h = s5 + __shfl_xor(s5, 16);
+  h += __shfl_xor(h, 32);  // the wave's half of a level-4 prefix
+  if ((t & 1) == 0) hist[hist_off(6) + b * 512 + (t >> 1)] = s6;
+  if ((t & 15) == 0) hist[hist_off(5) + b * 64 + (t >> 4)] = s5;
+  int m6 = s6, m5 = s5;
+  for (int o = 1; o < kWave; o <<= 1) {
+    m7 = max(m7, __shfl_xor(m7, o));
+    m6 = max(m6, __shfl_xor(m6, o));
+    m5 = max(m5, __shfl_xor(m5, o));
   }
-  hist[hist_off(5) + p5] = s5;
-  m[0][threadIdx.x] = s5;
-  m[1][threadIdx.x] = m6;
-  m[2][threadIdx.x] = m7;
+  if (lane == 0) {
+    w4[w] = h;
+    wm[w][0] = m7;
+    wm[w][1] = m6;
+    wm[w][2] = m5;
+  }
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o)
-      for (int k = 0; k < 3; k++) m[k][threadIdx.x] = max(m[k][threadIdx.x], m[k][threadIdx.x + o]);
-    __syncthreads();
+  if (t < 8) {
+    s4[t] = w4[2 * t] + w4[2 * t + 1];
+    hist[hist_off(4) + b * 8 + t] = s4[t];
   }
-  if (threadIdx.x == 0) {
-    atomicMax(&lmax[5], m[0][0]);
-    atomicMax(&lmax[6], m[1][0]);
-    atomicMax(&lmax[7], m[2][0]);
+  __syncthreads();
+  if (t == 0) {
+    int a7 = 0, a6 = 0, a5 = 0, a4 = 0, s3 = 0;
+    for (int i = 0; i < 16; i++) {
+      a7 = max(a7, wm[i][0]);
+      a6 = max(a6, wm[i][1]);
+      a5 = max(a5, wm[i][2]);
+    }
+    for (int i = 0; i < 8; i++) {
+      a4 = max(a4, s4[i]);
+      s3 += s4[i];
+    }
+    hist[hist_off(3) + b] = s3;
+    bmax[8 * b + 3] = s3;
+    bmax[8 * b + 4] = a4;
+    bmax[8 * b + 5] = a5;
+    bmax[8 * b + 6] = a6;
+    bmax[8 * b + 7] = a7;
   }
 }
 
-// Levels 4..1 from level 5 in one workgroup.
-__global__ void __launch_bounds__(1024) k_hist_top(int *__restrict__ hist, int *__restrict__ lmax) {
-  __shared__ int red[1024];
-  for (int l = 4; l >= 1; l--) {
-    const int nb = 1 << (3 * l);
-    int mx = 0;
-    for (int p = threadIdx.x; p < nb; p += 1024) {
-      int s = 0;
-      for (int c = 0; c < 8; c++) s += hist[hist_off(l + 1) + p * 8 + c];
-      hist[hist_off(l) + p] = s;
-      mx = max(mx, s);
-    }
-    red[threadIdx.x] = mx;
+// Levels 2 and 1 from level 3, and the largest population of every level
+// (lmax[l], l = 1..7) from the blocks' maxima: one workgroup of 512.
+__global__ void __launch_bounds__(512) k_hist_top(int *__restrict__ hist, const int *__restrict__ bmax,
+                                                  int *__restrict__ lmax) {
+  __shared__ int red[512], s2[64];
+  const int t = threadIdx.x;
+  for (int l = 3; l <= 7; l++) {
+    red[t] = bmax[8 * t + l];
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + o]);
+    for (int o = 256; o > 0; o >>= 1) {
+      if (t < o) red[t] = max(red[t], red[t + o]);
       __syncthreads();
     }
-    if (threadIdx.x == 0) lmax[l] = red[0];
+    if (t == 0) lmax[l] = red[0];
     __syncthreads();
+  }
+  if (t < 64) {
+    int s = 0;
+    for (int c = 0; c < 8; c++) s += hist[hist_off(3) + t * 8 + c];
+    s2[t] = s;
+    hist[hist_off(2) + t] = s;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int a2 = 0, a1 = 0;
+    for (int i = 0; i < 64; i++) a2 = max(a2, s2[i]);
+    for (int o = 0; o < 8; o++) {
+      int s = 0;
+      for (int c = 0; c < 8; c++) s += s2[o * 8 + c];
+      hist[hist_off(1) + o] = s;
+      a1 = max(a1, s);
+    }
+    lmax[2] = a2;
+    lmax[1] = a1;
   }
 }
 
@@ -389,9 +426,11 @@ void tree_build(Ctx &c) {
   hipLaunchKernelGGL(k_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, c.bodies, n, root, t.key.p, t.val.p,
                      t.hist.p + hist_off(kHistLevels));
   XF_LAUNCH();
-  hipLaunchKernelGGL(k_hist_up, dim3((1 << 15) / 256), dim3(256), 0, s, t.hist.p, t.misc.p);
+  // one block per level-3 prefix (4096 level-7 bins each)
+  t.hist_bmax.ensure(8 << (3 * 3));
+  hipLaunchKernelGGL(k_hist_pyramid, dim3(1 << (3 * 3)), dim3(1024), 0, s, t.hist.p, t.hist_bmax.p);
   XF_LAUNCH();
-  hipLaunchKernelGGL(k_hist_top, dim3(1), dim3(1024), 0, s, t.hist.p, t.misc.p);
+  hipLaunchKernelGGL(k_hist_top, dim3(1), dim3(1 << (3 * 3)), 0, s, t.hist.p, t.hist_bmax.p, t.misc.p);
   XF_LAUNCH();
 
   // sync 1: bounds, input check, depth needed
