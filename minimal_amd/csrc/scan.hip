@@ -33,12 +33,24 @@ __device__ inline int block_sum(int v, int *red) {
   return s;
 }
 
+constexpr int kScanChunk = 8 * kScanThreads;  // elements per workgroup range granule (one tile)
 __device__ inline void scan_range(int n, int G, int &b0, int &e) {
   int chunk = (n + G - 1) / G;
-  chunk = (chunk + kScanThreads - 1) / kScanThreads * kScanThreads;
+  chunk = (chunk + kScanChunk - 1) / kScanChunk * kScanChunk;
   b0 = blockIdx.x * chunk;
   e = min(b0 + chunk, n);
 }
+
+// Items per thread in one pass over a tile of kScanThreads * kScanItems
+// elements: the tile is loaded with every load in flight (coalesced, item j
+// of thread t at j * kScanThreads + t), transposed through LDS so that each
+// thread owns kScanItems consecutive elements, scanned in registers, and
+// written back the same way.  Padding one int per 32 keeps the transposed
+// reads free of bank conflicts.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+static_assert(kScanTile == kScanChunk, "a workgroup range is a whole number of tiles");
+__device__ inline int scan_pad(int i) { return i + (i >> 5); }
 
 template <int NA>
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(ScanArrays<NA> a, int n_host, const int *n_dev,
@@ -51,7 +63,16 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(ScanArrays<NA> a, 
 #pragma unroll
   for (int k = 0; k < NA; k++) {
     int v = 0;
-    for (int i = b0 + threadIdx.x; i < e; i += kScanThreads) v += a.in[k][i];
+    for (int t0 = b0; t0 < e; t0 += kScanTile) {
+      int x[kScanItems];
+#pragma unroll
+      for (int j = 0; j < kScanItems; j++) {
+        const int i = t0 + j * kScanThreads + threadIdx.x;
+        x[j] = i < e ? a.in[k][i] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < kScanItems; j++) v += x[j];
+    }
     v = block_sum(v, red);
     if (threadIdx.x == 0) sums[blockIdx.x * NA + k] = v;
   }
@@ -65,11 +86,12 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(ScanArrays<NA> a, i
                                                              const int64_t *cap, int *flag) {
   __shared__ int red[kScanThreads / 64];
   __shared__ int wtot[kScanThreads / 64];
+  __shared__ int sm[kScanTile + kScanTile / 32];
   const int n = n_dev ? *n_dev : n_host;
   const int G = gridDim.x;
   int b0, e;
   scan_range(n, G, b0, e);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
 #pragma unroll
   for (int k = 0; k < NA; k++) {
     // one workgroup (sums == nullptr): no reduce pass, the running total
@@ -77,13 +99,27 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(ScanArrays<NA> a, i
     int carry = sums ? sums[kScanMaxBlocks * NA + k] : (base ? base[k] : 0);
     if (sums) {
       int v = 0;
-      for (int j = threadIdx.x; j < blockIdx.x; j += kScanThreads) v += sums[j * NA + k];
+      for (int j = tid; j < blockIdx.x; j += kScanThreads) v += sums[j * NA + k];
       carry += block_sum(v, red);
     }
-    for (int t0 = b0; t0 < e; t0 += kScanThreads) {
-      const int i = t0 + threadIdx.x;
-      const int v = i < e ? a.in[k][i] : 0;
-      int incl = v;
+    for (int t0 = b0; t0 < e; t0 += kScanTile) {
+      int x[kScanItems];
+#pragma unroll
+      for (int j = 0; j < kScanItems; j++) {
+        const int i = t0 + j * kScanThreads + tid;
+        x[j] = i < e ? a.in[k][i] : 0;
+      }
+#pragma unroll
+      for (int j = 0; j < kScanItems; j++) sm[scan_pad(j * kScanThreads + tid)] = x[j];
+      __syncthreads();
+      // this thread's kScanItems consecutive elements
+      int run = 0;
+#pragma unroll
+      for (int j = 0; j < kScanItems; j++) {
+        x[j] = sm[scan_pad(tid * kScanItems + j)];
+        run += x[j];
+      }
+      int incl = run;
 #pragma unroll
       for (int d = 1; d < 64; d <<= 1) {
         const int u = __shfl_up(incl, d);
@@ -91,18 +127,28 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_apply(ScanArrays<NA> a, i
       }
       if (lane == 63) wtot[w] = incl;
       __syncthreads();
-      int before = carry;
-      int tile = 0;
+      int before = carry, tile = 0;
       for (int q = 0; q < kScanThreads / 64; q++) {
         if (q < w) before += wtot[q];
         tile += wtot[q];
       }
-      if (i < e) a.out[k][i] = before + incl - v;
+      before += incl - run;
+#pragma unroll
+      for (int j = 0; j < kScanItems; j++) {
+        sm[scan_pad(tid * kScanItems + j)] = before;
+        before += x[j];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int j = 0; j < kScanItems; j++) {
+        const int i = t0 + j * kScanThreads + tid;
+        if (i < e) a.out[k][i] = sm[scan_pad(j * kScanThreads + tid)];
+      }
       carry += tile;
       __syncthreads();
     }
     __syncthreads();  // every thread has read base[k] before it is overwritten
-    if (blockIdx.x == G - 1 && threadIdx.x == 0) {
+    if (blockIdx.x == G - 1 && tid == 0) {
       if (total) total[k] = carry;
       if (base) base[k] = carry;
       if (cap && flag && carry > cap[k]) atomicOr(flag, 1);
