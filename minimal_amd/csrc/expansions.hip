@@ -393,12 +393,87 @@ __global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l_grouped(const GroupT
 
 
 // ---------------------------------------------------------------- L2P
-// One workgroup per leaf, one thread per body.  With u = (x - Xc)/R and
+// One thread per body.  With u = (x - Xc)/R and
 // Lh_n^m = L_n^m R^n = L~/R, phi = Re sum_{n, m>=0} w_m Lh_n^m Y_n^m(u)
 // (w_0 = 1, w_m = 2, 3d/kernel.h:302-316).  The gradient uses
 //   dz Y_n^m = -Y_{n-1}^m,  (dx + i dy) Y_n^m = -Y_{n-1}^{m+1},
 //   (dx - i dy) Y_n^m = Y_{n-1}^{m-1} (m >= 1), = -conj(Y_{n-1}^1) (m = 0),
 // collected per harmonic Y_n^m so each column m is generated once.
+// Potential and field of one body at u = (x, y, z) (leaf units) from the
+// leaf's folded coefficients Lh (LDS) and the recurrence factors rec.
+template <class T>
+__device__ __attribute__((always_inline)) inline void l2p_eval(int P, T x, T y, T z, const cpx<T> *Lh, const T *rec,
+                                                              T &pot, T &Fx2, T &Fy2, T &Dz) {
+  const T r2 = x * x + y * y + z * z;
+  T Ar = 0, Ai = 0, Br = 0, Bi = 0;
+  pot = 0;
+  Dz = 0;
+  T ymr = 1, ymi = 0;  // Y_m^m
+  for (int m = 0; m < P; m++) {
+    if (m > 0) {
+      const T inv = rec[nms(m, m)];
+      const T tr = (ymr * x - ymi * y) * inv, ti = (ymr * y + ymi * x) * inv;
+      ymr = tr;
+      ymi = ti;
+    }
+    T pr = 0, pi = 0, cr = ymr, ci = ymi;
+    for (int n = m; n < P; n++) {
+      if (n > m) {
+        T nr, ni;
+        if (n == m + 1) {
+          nr = -z * cr;
+          ni = -z * ci;
+        } else {
+          const T s = rec[nms(n, m)];
+          const T cz = T(2 * n - 1) * z;
+          nr = (cz * cr + r2 * pr) * s;
+          ni = (cz * ci + r2 * pi) * s;
+        }
+        pr = cr;
+        pi = ci;
+        cr = nr;
+        ci = ni;
+      }
+      const cpx<T> l0 = Lh[nms(n, m)];
+      pot += l0.re * cr - l0.im * ci;
+      if (n + 1 < P) {
+        const cpx<T> lz = Lh[nms(n + 1, m)];
+        Dz -= lz.re * cr - lz.im * ci;
+        if (m >= 1) {
+          const cpx<T> la = Lh[nms(n + 1, m - 1)];
+          Ar -= la.re * cr - la.im * ci;
+          Ai -= la.re * ci + la.im * cr;
+        }
+        const cpx<T> lb = Lh[nms(n + 1, m + 1)];
+        Br += lb.re * cr - lb.im * ci;
+        Bi += lb.re * ci + lb.im * cr;
+        if (m == 1) {  // the m = 0 terms' (dx - i dy): -Lh_{n+1}^0 conj(Y_n^1)
+          const cpx<T> l00 = Lh[nms(n + 1, 0)];
+          Br -= l00.re * cr + l00.im * ci;
+          Bi -= l00.im * cr - l00.re * ci;
+        }
+      }
+    }
+  }
+  Fx2 = Ar + Br;  // 2 dphi/dx
+  Fy2 = Ai - Bi;  // 2 dphi/dy
+}
+
+// Lh (w_m and L~ -> L R^n folded in) and the recurrence factors of one leaf,
+// entries e = lane, lane + stride, ...
+template <class T>
+__device__ inline void l2p_stage(int P, int e0, int stride, const double *__restrict__ Lc, double invR, cpx<T> *Lh,
+                                 T *rec) {
+  const int NT = P * (P + 1) / 2;
+  for (int e = e0; e < NT; e += stride) {
+    int n, m;
+    jk_of(e, n, m);
+    const double w = (m == 0 ? 1.0 : 2.0) * invR;
+    Lh[e] = {T(Lc[rf(n, m, 0)] * w), T(m ? Lc[rf(n, m, 1)] * w : 0.0)};
+    if (rec) rec[e] = n == m ? T(m ? kInv.v[2 * m] : 0.0) : (n >= m + 2 ? T(-kInv.v[(n + m) * (n - m)]) : T(0));
+  }
+}
+
 template <int MAXP, class T>
 __device__ __attribute__((always_inline)) inline void l2p_body(int bx, int P, const int *__restrict__ leaves, const int *__restrict__ nleaves,
                                 const int *__restrict__ cbeg, const int *__restrict__ ccnt,
@@ -411,74 +486,80 @@ __device__ __attribute__((always_inline)) inline void l2p_body(int bx, int P, co
   const int c = leaves[li];
   const double4 C = cx[c];
   const double invR = 1.0 / C.w;
-  const int NT = P * (P + 1) / 2;
-  const double *Lc = L + int64_t(c) * P * P;
-  for (int e = threadIdx.x; e < NT; e += 64) {
-    int n, m;
-    jk_of(e, n, m);
-    const double w = (m == 0 ? 1.0 : 2.0) * invR;  // fold w_m and L~ -> L R^n
-    Lh[e] = {T(Lc[rf(n, m, 0)] * w), T(m ? Lc[rf(n, m, 1)] * w : 0.0)};
-    rec[e] = n == m ? T(m ? kInv.v[2 * m] : 0.0) : (n >= m + 2 ? T(-kInv.v[(n + m) * (n - m)]) : T(0));
-  }
+  l2p_stage<T>(P, threadIdx.x, 64, L + int64_t(c) * P * P, invR, Lh, rec);
   __syncthreads();
   const int beg = cbeg[c], cnt = ccnt[c];
   for (int b = beg + threadIdx.x; b < beg + cnt; b += 64) {
     const double4 B = xs[b];
-    const T x = T((B.x - C.x) * invR), y = T((B.y - C.y) * invR), z = T((B.z - C.z) * invR);
-    const T r2 = x * x + y * y + z * z;
-    T pot = 0, Ar = 0, Ai = 0, Br = 0, Bi = 0, Dz = 0;
-    T ymr = 1, ymi = 0;  // Y_m^m
-    for (int m = 0; m < P; m++) {
-      if (m > 0) {
-        const T inv = rec[nms(m, m)];
-        const T tr = (ymr * x - ymi * y) * inv, ti = (ymr * y + ymi * x) * inv;
-        ymr = tr;
-        ymi = ti;
-      }
-      T pr = 0, pi = 0, cr = ymr, ci = ymi;
-      for (int n = m; n < P; n++) {
-        if (n > m) {
-          T nr, ni;
-          if (n == m + 1) {
-            nr = -z * cr;
-            ni = -z * ci;
-          } else {
-            const T s = rec[nms(n, m)];
-            const T cz = T(2 * n - 1) * z;
-            nr = (cz * cr + r2 * pr) * s;
-            ni = (cz * ci + r2 * pi) * s;
-          }
-          pr = cr;
-          pi = ci;
-          cr = nr;
-          ci = ni;
-        }
-        const cpx<T> l0 = Lh[nms(n, m)];
-        pot += l0.re * cr - l0.im * ci;
-        if (n + 1 < P) {
-          const cpx<T> lz = Lh[nms(n + 1, m)];
-          Dz -= lz.re * cr - lz.im * ci;
-          if (m >= 1) {
-            const cpx<T> la = Lh[nms(n + 1, m - 1)];
-            Ar -= la.re * cr - la.im * ci;
-            Ai -= la.re * ci + la.im * cr;
-          }
-          const cpx<T> lb = Lh[nms(n + 1, m + 1)];
-          Br += lb.re * cr - lb.im * ci;
-          Bi += lb.re * ci + lb.im * cr;
-          if (m == 1) {  // the m = 0 terms' (dx - i dy): -Lh_{n+1}^0 conj(Y_n^1)
-            const cpx<T> l00 = Lh[nms(n + 1, 0)];
-            Br -= l00.re * cr + l00.im * ci;
-            Bi -= l00.im * cr - l00.re * ci;
-          }
-        }
-      }
-    }
+    T pot, fx2, fy2, dz;
+    l2p_eval<T>(P, T((B.x - C.x) * invR), T((B.y - C.y) * invR), T((B.z - C.z) * invR), Lh, rec, pot, fx2, fy2, dz);
     exafmm_body *o = bodies + b;
     o->p += double(pot);
-    o->F[0] += 0.5 * double(Ar + Br) * invR;
-    o->F[1] += 0.5 * double(Ai - Bi) * invR;
-    o->F[2] += double(Dz) * invR;
+    o->F[0] += 0.5 * double(fx2) * invR;
+    o->F[1] += 0.5 * double(fy2) * invR;
+    o->F[2] += double(dz) * invR;
+  }
+}
+
+// FP32 L2P over the bodies in tree order, 64 per wave (full waves whatever
+// the leaf sizes; one wave per leaf leaves half its lanes idle at 32 bodies
+// a leaf).  The wave's bodies fall into runs of equal leaf; up to kL2PSlots
+// runs at a time have their coefficients staged in LDS (one pass for
+// uniform trees, a few when many small leaves share a wave), and each lane
+// evaluates its own body against its run's slot.
+constexpr int kL2PSlots = 8;
+template <int MAXP>
+__global__ void __launch_bounds__(64) k_l2p_flat(int P, int64_t n, const int *__restrict__ body_leaf,
+                                                 const double4 *__restrict__ cx, const double *__restrict__ L,
+                                                 const double4 *__restrict__ xs, exafmm_body *__restrict__ bodies) {
+  constexpr int NTM = MAXP * (MAXP + 1) / 2;
+  __shared__ cpx<float> Lh[kL2PSlots][NTM];
+  __shared__ float rec[NTM];
+  __shared__ int sleaf[kL2PSlots];
+  const int lane = threadIdx.x;
+  const int64_t b = int64_t(blockIdx.x) * kWave + lane;
+  const bool valid = b < n;
+  const int leaf = valid ? body_leaf[b] : -1;
+  const int prev = __shfl_up(leaf, 1);
+  const unsigned long long starts = __ballot(valid && (lane == 0 || prev != leaf));
+  const int nruns = __popcll(starts);
+  const int run = valid ? __popcll(starts & (~0ull >> (63 - lane))) - 1 : -1;
+  const int NT = P * (P + 1) / 2;
+  for (int e = lane; e < NT; e += kWave) {
+    int nn, m;
+    jk_of(e, nn, m);
+    rec[e] = nn == m ? (m ? float(kInv.v[2 * m]) : 0.0f) : (nn >= m + 2 ? float(-kInv.v[(nn + m) * (nn - m)]) : 0.0f);
+  }
+  double4 C = make_double4(0, 0, 0, 1), B = make_double4(0, 0, 0, 0);
+  if (valid) {
+    C = cx[leaf];
+    B = xs[b];
+  }
+  const double invR = 1.0 / C.w;
+  for (int g0 = 0; g0 < nruns; g0 += kL2PSlots) {
+    const int ns = min(kL2PSlots, nruns - g0);
+    if (valid && run >= g0 && run < g0 + ns && (lane == 0 || prev != leaf)) sleaf[run - g0] = leaf;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    for (int e = lane; e < ns * NT; e += kWave) {
+      const int sl = e / NT, k = e - sl * NT, c = sleaf[sl];
+      l2p_stage<float>(P, k, NT, L + int64_t(c) * P * P, 1.0 / cx[c].w, Lh[sl], nullptr);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (valid && run >= g0 && run < g0 + ns) {
+      float pot, fx2, fy2, dz;
+      l2p_eval<float>(P, float((B.x - C.x) * invR), float((B.y - C.y) * invR), float((B.z - C.z) * invR),
+                      Lh[run - g0], rec, pot, fx2, fy2, dz);
+      exafmm_body *o = bodies + b;
+      o->p += double(pot);
+      o->F[0] += 0.5 * double(fx2) * invR;
+      o->F[1] += 0.5 * double(fy2) * invR;
+      o->F[2] += double(dz) * invR;
+    }
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
@@ -578,8 +659,8 @@ void l2p_pass(Ctx &c) {
     hipLaunchKernelGGL((k_l2p<kMaxP, double>), dim3(t.nleaves), dim3(64), 0, s, P, t.leaves.p, t.misc.p + 17,
                        t.cbeg.p, t.ccnt.p, t.cx.p, c.L.p, t.xs.p, c.bodies);
   else
-    hipLaunchKernelGGL((k_l2p<kMaxP, float>), dim3(t.nleaves), dim3(64), 0, s, P, t.leaves.p, t.misc.p + 17,
-                       t.cbeg.p, t.ccnt.p, t.cx.p, c.L.p, t.xs.p, c.bodies);
+    hipLaunchKernelGGL((k_l2p_flat<kMaxP>), dim3(ceil_div(t.n, kWave)), dim3(kWave), 0, s, P, t.n, t.body_leaf.p,
+                       t.cx.p, c.L.p, t.xs.p, c.bodies);
   XF_LAUNCH();
 }
 
