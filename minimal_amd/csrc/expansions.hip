@@ -75,38 +75,56 @@ __device__ __attribute__((always_inline)) inline void p2m_body(int bx, int P, co
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (m < P) {
-      for (int bi = g; bi < nb; bi += 4) {
-        const double4 B = sb[w][bi];
-        const double x = (B.x - C.x) * invR, y = -(B.y - C.y) * invR, z = (B.z - C.z) * invR;  // conjugate
-        const double r2 = x * x + y * y + z * z;
-        double yr = 1.0, yi = 0;  // (x + iy)^m
+      // two bodies (bi, bi + 4) per pass: independent recurrence chains
+      // interleave; a missing second body has charge 0 and adds exact zeros
+      for (int bi = g; bi < nb; bi += 8) {
+        const bool two = bi + 4 < nb;
+        const double4 B0 = sb[w][bi], B1 = sb[w][two ? bi + 4 : bi];
+        const double x0 = (B0.x - C.x) * invR, y0 = -(B0.y - C.y) * invR, z0 = (B0.z - C.z) * invR;  // conjugate
+        const double x1 = (B1.x - C.x) * invR, y1 = -(B1.y - C.y) * invR, z1 = (B1.z - C.z) * invR;
+        const double r20 = x0 * x0 + y0 * y0 + z0 * z0, r21 = x1 * x1 + y1 * y1 + z1 * z1;
+        double yr0 = 1.0, yi0 = 0, yr1 = 1.0, yi1 = 0;  // (x + iy)^m
         for (int k = 1; k <= m; k++) {
-          const double tr = yr * x - yi * y, ti = yr * y + yi * x;
-          yr = tr;
-          yi = ti;
+          const double tr0 = yr0 * x0 - yi0 * y0, ti0 = yr0 * y0 + yi0 * x0;
+          const double tr1 = yr1 * x1 - yi1 * y1, ti1 = yr1 * y1 + yi1 * x1;
+          yr0 = tr0;
+          yi0 = ti0;
+          yr1 = tr1;
+          yi1 = ti1;
         }
-        const double qc = B.w * cm;  // q * Y_m^m = q (x+iy)^m / (2^m m!)
-        double pr = 0, pi = 0, cr = yr * qc, ci = yi * qc;
+        const double qc0 = B0.w * cm, qc1 = two ? B1.w * cm : 0.0;  // q * Y_m^m = q (x+iy)^m / (2^m m!)
+        double pr0 = 0, pi0 = 0, cr0 = yr0 * qc0, ci0 = yi0 * qc0;
+        double pr1 = 0, pi1 = 0, cr1 = yr1 * qc1, ci1 = yi1 * qc1;
 #pragma unroll
         for (int n = 0; n < PMAX; n++) {
           if (n < m || n >= P) continue;
           if (n > m) {
-            double nr, ni;
+            double nr0, ni0, nr1, ni1;
             if (n == m + 1) {
-              nr = -z * cr;
-              ni = -z * ci;
+              nr0 = -z0 * cr0;
+              ni0 = -z0 * ci0;
+              nr1 = -z1 * cr1;
+              ni1 = -z1 * ci1;
             } else {
-              const double cz = double(2 * n - 1) * z;
-              nr = (cz * cr + r2 * pr) * sn[n];
-              ni = (cz * ci + r2 * pi) * sn[n];
+              const double c2 = double(2 * n - 1);
+              nr0 = (c2 * z0 * cr0 + r20 * pr0) * sn[n];
+              ni0 = (c2 * z0 * ci0 + r20 * pi0) * sn[n];
+              nr1 = (c2 * z1 * cr1 + r21 * pr1) * sn[n];
+              ni1 = (c2 * z1 * ci1 + r21 * pi1) * sn[n];
             }
-            pr = cr;
-            pi = ci;
-            cr = nr;
-            ci = ni;
+            pr0 = cr0;
+            pi0 = ci0;
+            cr0 = nr0;
+            ci0 = ni0;
+            pr1 = cr1;
+            pi1 = ci1;
+            cr1 = nr1;
+            ci1 = ni1;
           }
-          ar[n] += cr;
-          ai[n] += ci;
+          ar[n] += cr0;
+          ai[n] += ci0;
+          ar[n] += cr1;
+          ai[n] += ci1;
         }
       }
     }
