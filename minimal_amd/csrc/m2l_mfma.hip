@@ -10,10 +10,10 @@
 // per expansion: Re M_n^0, Re/Im M_n^m for m > 0) T is a dense P^2 x P^2
 // matrix, generated once per distinct key in FP64.
 //
-// Targets are grouped in tiles of 64 cells of one level and one octant
+// Targets are grouped in tiles of 256 cells of one level and one octant
 // position (cells with the same position relative to their parent have the
 // same key set away from the domain boundary).  For each key in a tile's
-// key union the 64 source columns are gathered and multiplied by the key's
+// key union the 256 source columns are gathered and multiplied by the key's
 // operator; all keys of a tile accumulate into the same MFMA accumulators,
 // so each target's L is written once, without atomics, in a fixed order.
 //
@@ -868,9 +868,10 @@ static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16
 // low-order keys) and key slices per tile (a tile's keys split over
 // several workgroups whose FP32 partial locals a second kernel adds in a
 // fixed order: more parallelism when tiles are few and keys many).  The first time a (order, precision, order policy, tile
-// count class) is seen, every candidate runs once into a scratch copy of
-// the locals, alone on the device, timed with HIP events; the fastest is
-// kept for the context's lifetime.
+// count class) is seen, every candidate (GEMM and, with key slices, the
+// combining kernel) runs twice into a scratch copy of the locals, alone on
+// the device, timed with HIP events; the fastest run's shape is kept for
+// the context's lifetime.
 static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb) {
   int tclass = 0;
   while ((1 << tclass) < ntiles) tclass++;
