@@ -188,6 +188,7 @@ struct Ctx {
   bool capturing = false;
   int graph_locks = 0;
   Buf<double> shift_T;             // scaled M2M / L2L operators per octant
+  Buf<double> dipole_part;         // dipole moment: per-workgroup partials, then the sum
   int shift_P = -1;
 
   // M2L fast path

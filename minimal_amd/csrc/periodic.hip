@@ -123,11 +123,10 @@ void periodic_far(Ctx &c) {
 }
 
 // ---------------------------------------------------------------- dipole
-__global__ void __launch_bounds__(256) k_dipole(int64_t n, const exafmm_body *__restrict__ b, double *__restrict__ out) {
-  __shared__ double s[3][256];
-  double d[3] = {0, 0, 0};
-  for (int64_t i = threadIdx.x; i < n; i += 256)
-    for (int k = 0; k < 3; k++) d[k] += b[i].X[k] * b[i].q;
+// Dipole moment sum_i q_i x_i: kDipoleBlocks workgroups sum grid-strided
+// ranges, then one workgroup adds their partials in a fixed order.
+constexpr int kDipoleBlocks = 256;
+__device__ inline void block_sum3(double (&d)[3], double (*s)[256]) {
   for (int k = 0; k < 3; k++) s[k][threadIdx.x] = d[k];
   __syncthreads();
   for (int o = 128; o > 0; o >>= 1) {
@@ -135,6 +134,22 @@ __global__ void __launch_bounds__(256) k_dipole(int64_t n, const exafmm_body *__
       for (int k = 0; k < 3; k++) s[k][threadIdx.x] += s[k][threadIdx.x + o];
     __syncthreads();
   }
+}
+
+__global__ void __launch_bounds__(256) k_dipole(int64_t n, const exafmm_body *__restrict__ b, double *__restrict__ part) {
+  __shared__ double s[3][256];
+  double d[3] = {0, 0, 0};
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    for (int k = 0; k < 3; k++) d[k] += b[i].X[k] * b[i].q;
+  block_sum3(d, s);
+  if (threadIdx.x < 3) part[blockIdx.x * 3 + threadIdx.x] = s[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256) k_dipole_final(const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double s[3][256];
+  double d[3] = {0, 0, 0};
+  for (int k = 0; k < 3; k++) d[k] = part[threadIdx.x * 3 + k];
+  block_sum3(d, s);
   if (threadIdx.x < 3) out[threadIdx.x] = s[threadIdx.x][0];
 }
 
@@ -148,9 +163,11 @@ __global__ void __launch_bounds__(256) k_dipole_apply(int64_t n, double coef, co
 }
 
 void dipole_correction(Ctx &c) {
-  c.far_sup.ensure(c.NT > 2 ? c.NT : 2);
-  double *D = reinterpret_cast<double *>(c.far_sup.p);
-  hipLaunchKernelGGL(k_dipole, dim3(1), dim3(256), 0, c.s0, c.t.n, c.bodies, D);
+  c.dipole_part.ensure(3 * kDipoleBlocks + 3);
+  double *D = c.dipole_part.p + 3 * kDipoleBlocks;
+  hipLaunchKernelGGL(k_dipole, dim3(kDipoleBlocks), dim3(256), 0, c.s0, c.t.n, c.bodies, c.dipole_part.p);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_dipole_final, dim3(1), dim3(kDipoleBlocks), 0, c.s0, c.dipole_part.p, D);
   XF_LAUNCH();
   const double cyc = c.cfg.cycle;
   const double coef = 4 * M_PI / (3 * cyc * cyc * cyc);
