@@ -4,8 +4,9 @@
 // a super-cell starts as the root (centre X0, multipole M_root); at each of
 // images-1 levels the root's local expansion receives M2L from the 26 x 27
 // copies at offsets (3i + c) * cycle (i != 0), then the super-cell absorbs
-// its 26 neighbours by M2M and cycle triples.  Each level is one launch of
-// 702 workgroups writing partial locals, summed in a fixed order.
+// its 26 neighbours by M2M and cycle triples.  Each level's M2L is one
+// launch of 702 workgroups writing partial locals, summed in a fixed order;
+// the M2M is one workgroup per neighbour, summed in neighbour order.
 //
 // Dipole correction (3dp/fmm.cxx:63-75): p -= 4 pi/(3 cycle^3) |D|^2 / N / q,
 // F -= 4 pi/(3 cycle^3) D, D = sum q X.  Bodies with q = 0 keep p (the
@@ -43,21 +44,33 @@ __global__ void __launch_bounds__(64) k_far_m2l(int P, int NT, int fillP, double
   }
 }
 
-// Sum of the partial locals in launch order, added to the root's scaled L~.
-__global__ void __launch_bounds__(64) k_far_sum(int P, int nparts, double R0, const double2 *__restrict__ part,
-                                                double *__restrict__ Lroot) {
-  const int NT = P * (P + 1) / 2;
-  for (int jk = threadIdx.x; jk < NT; jk += 64) {
-    double ax = 0, ay = 0;
-    for (int e = 0; e < nparts; e++) {
-      ax += part[int64_t(e) * NT + jk].x;
-      ay += part[int64_t(e) * NT + jk].y;
+// Sum of the partial locals, added to the root's scaled L~: one workgroup
+// per coefficient, the parts added by a fixed-order tree.
+__global__ void __launch_bounds__(256) k_far_sum(int P, int nparts, double R0, const double2 *__restrict__ part,
+                                                 double *__restrict__ Lroot) {
+  __shared__ double sx[256], sy[256];
+  const int NT = P * (P + 1) / 2, jk = blockIdx.x, t = threadIdx.x;
+  double ax = 0, ay = 0;
+  for (int e = t; e < nparts; e += 256) {
+    ax += part[int64_t(e) * NT + jk].x;
+    ay += part[int64_t(e) * NT + jk].y;
+  }
+  sx[t] = ax;
+  sy[t] = ay;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o) {
+      sx[t] += sx[t + o];
+      sy[t] += sy[t + o];
     }
+    __syncthreads();
+  }
+  if (t == 0) {
     int j, k;
     jk_of(jk, j, k);
     const double s = pow(R0, double(j + 1));
-    Lroot[rf(j, k, 0)] += ax * s;
-    if (k > 0) Lroot[rf(j, k, 1)] += ay * s;
+    Lroot[rf(j, k, 0)] += sx[0] * s;
+    if (k > 0) Lroot[rf(j, k, 1)] += sy[0] * s;
   }
 }
 
@@ -73,31 +86,35 @@ __global__ void __launch_bounds__(64) k_far_init(int P, double R0, const double 
   }
 }
 
-// Msup += sum over the 26 neighbours (child centre X + i*cycle) of M2M(Msup).
-__global__ void __launch_bounds__(64) k_far_m2m(int P, int NT, double cyc, double2 *__restrict__ Msup) {
+// Msup += sum over the 26 neighbours (child centre X + i*cycle) of
+// M2M(Msup): one workgroup per neighbour writes its term, then the terms
+// are added in neighbour order.
+__global__ void __launch_bounds__(64) k_far_m2m(int P, int NT, double cyc, const double2 *__restrict__ Msup,
+                                                double2 *__restrict__ part) {
   __shared__ cpx<double> Y[kMaxP * (kMaxP + 1) / 2];
   __shared__ double2 Mc[kMaxP * (kMaxP + 1) / 2];
+  const int b = blockIdx.x, nb = b < 13 ? b : b + 1;  // skip the centre
+  const int ix = nb / 9 - 1, iy = (nb / 3) % 3 - 1, iz = nb % 3 - 1;
   for (int i = threadIdx.x; i < NT; i += 64) Mc[i] = Msup[i];
-  double2 acc[3] = {};
-  for (int nb = 0; nb < 27; nb++) {
-    if (nb == 13) continue;
-    const int ix = nb / 9 - 1, iy = (nb / 3) % 3 - 1, iz = nb % 3 - 1;
-    __syncthreads();
-    if (threadIdx.x == 0) regular_all<double>(P, -ix * cyc, -iy * cyc, -iz * cyc, Y);  // Xparent - Xchild
-    __syncthreads();
-    for (int r = 0; r * 64 + int(threadIdx.x) < NT; r++) {
-      int j, k;
-      jk_of(r * 64 + threadIdx.x, j, k);
-      const double2 v = m2m_coef(j, k, Mc, Y);
-      acc[r].x += v.x;
-      acc[r].y += v.y;
-    }
-  }
+  if (threadIdx.x == 0) regular_all<double>(P, -ix * cyc, -iy * cyc, -iz * cyc, Y);  // Xparent - Xchild
   __syncthreads();
-  for (int r = 0; r * 64 + int(threadIdx.x) < NT; r++) {
-    const int jk = r * 64 + threadIdx.x;
-    Msup[jk].x = Mc[jk].x + acc[r].x;
-    Msup[jk].y = Mc[jk].y + acc[r].y;
+  for (int jk = threadIdx.x; jk < NT; jk += 64) {
+    int j, k;
+    jk_of(jk, j, k);
+    part[int64_t(b) * NT + jk] = m2m_coef(j, k, Mc, Y);
+  }
+}
+
+__global__ void __launch_bounds__(64) k_far_m2m_sum(int NT, const double2 *__restrict__ part,
+                                                    double2 *__restrict__ Msup) {
+  for (int jk = threadIdx.x; jk < NT; jk += 64) {
+    double2 acc = make_double2(0, 0);
+    for (int b = 0; b < 26; b++) {
+      acc.x += part[int64_t(b) * NT + jk].x;
+      acc.y += part[int64_t(b) * NT + jk].y;
+    }
+    Msup[jk].x += acc.x;
+    Msup[jk].y += acc.y;
   }
 }
 
@@ -114,9 +131,12 @@ void periodic_far(Ctx &c) {
   for (int lev = 0; lev < c.cfg.images - 1; lev++) {
     hipLaunchKernelGGL(k_far_m2l, dim3(702), dim3(64), 0, c.s0, P, NT, fillP, cyc, sup.p, part.p);
     XF_LAUNCH();
-    hipLaunchKernelGGL(k_far_sum, dim3(1), dim3(64), 0, c.s0, P, 702, c.t.R0, part.p, c.L.p);
+    hipLaunchKernelGGL(k_far_sum, dim3(NT), dim3(256), 0, c.s0, P, 702, c.t.R0, part.p, c.L.p);
     XF_LAUNCH();
-    hipLaunchKernelGGL(k_far_m2m, dim3(1), dim3(64), 0, c.s0, P, NT, cyc, sup.p);
+    // the 26 neighbour terms reuse the partial-local buffer (read above)
+    hipLaunchKernelGGL(k_far_m2m, dim3(26), dim3(64), 0, c.s0, P, NT, cyc, sup.p, part.p);
+    XF_LAUNCH();
+    hipLaunchKernelGGL(k_far_m2m_sum, dim3(1), dim3(64), 0, c.s0, NT, part.p, sup.p);
     XF_LAUNCH();
     cyc *= 3;
   }
