@@ -224,3 +224,26 @@ def test_fast_path_reference_truncation(gpu):
     f.evaluate(b)
     assert rel_l2(b["p"], g["p"]) <= 3e-6
     assert rel_l2(b["F"], g["F"]) <= 3e-6
+
+
+def test_small_leaves_many_levels(gpu):
+    """2^16 uniform bodies with ncrit = 4: six or more levels, levels of more than
+    8,192 cells (workgroup-spanning scans in the tree build and the
+    traversal), leaves of at most 4 bodies (P2P blocks of 4 targets, L2P
+    waves spanning more than 8 leaves).  BF16x3 M2L at P = 8 against the FP64
+    direct sum on 300 sampled targets: relative L2 <= 1e-4 for p and F (the
+    P = 8 bar of the batched tests)."""
+    from minimal_amd.inputs import sample_targets, uniform_cube
+    n = 1 << 16
+    b = uniform_cube(n, 3, "sym")
+    f = fmm.FMM(P=8, ncrit=4, theta=0.4)
+    f.evaluate(b)
+    idx = sample_targets(n, 300)
+    ref = b[idx].copy()
+    ref["p"] = 0
+    ref["F"] = 0
+    f.direct(ref, b)
+    assert rel_l2(b["p"][idx], ref["p"]) <= 1e-4
+    assert rel_l2(b["F"][idx], ref["F"]) <= 1e-4
+    w = f.work_counts()
+    assert w["depth"] >= 5 and w["cells"] > 8192
