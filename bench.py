@@ -187,9 +187,10 @@ def plain_workload(args, fmm, torch):
     host = uniform_cube(n, 0, "sym")
     dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
     ptr = dev.data_ptr()
+    p2p = {"fp32": fmm.PREC_FP32, "fp32_newton": fmm.PREC_FP32_NEWTON}[args.p2p]
     f = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=trunc, m2l_precision=prec,
-                p2p_precision=fmm.PREC_FP32)
-    log(f"bench: N={n} P={args.P} m2l={args.m2l} truncation={args.truncation}")
+                p2p_precision=p2p)
+    log(f"bench: N={n} P={args.P} m2l={args.m2l} truncation={args.truncation} p2p={args.p2p}")
     ms, kern = timed(f, lambda: f.evaluate(ptr, n=n, on_device=True), args.steps, args.warmup, torch)
     work = f.work_counts()
     log("stages ms (mean of timed steps):", json.dumps(kern), "work:", json.dumps(work))
@@ -205,7 +206,7 @@ def plain_workload(args, fmm, torch):
     os.environ["EXAFMM_SERIAL_STREAMS"] = "1"
     try:
         fi = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=trunc, m2l_precision=prec,
-                     p2p_precision=fmm.PREC_FP32)
+                     p2p_precision=p2p)
     finally:
         del os.environ["EXAFMM_SERIAL_STREAMS"]
     dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
@@ -217,6 +218,12 @@ def plain_workload(args, fmm, torch):
                            ro["kernel"]: {k: ro[k] for k in ("ms", "achieved", "frac")},
                            ro["other"]["kernel"]: {k: ro["other"][k] for k in ("ms", "achieved", "frac")}}
     log(f"one stream: {ms_i:.3f} ms/step, stages {json.dumps(kern_i)}")
+    # The C ABI also takes bodies in host memory (copied in and out over
+    # PCIe inside each call): the same step on a host copy, reported beside
+    # the device-resident value, never as it.
+    hb = host.copy()
+    ms_h, _ = timed(f, lambda: f.evaluate(hb), 3, 1, torch)
+    log(f"host buffers: {ms_h:.3f} ms/step")
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -227,10 +234,12 @@ def plain_workload(args, fmm, torch):
         "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"plain 3D Laplace FMM, N={n} uniform [0,1)^3, q~U[-1,1), P={args.P}, "
                                f"theta={args.theta}, ncrit={args.ncrit}",
-                   "m2l_truncation": args.truncation, "m2l_precision": args.m2l, "p2p_precision": "fp32",
+                   "m2l_truncation": args.truncation, "m2l_precision": args.m2l, "p2p_precision": args.p2p,
                    "order_policy": "fixed"},
         "error": {"p": err_p, "F": err_F, "targets": int(nt), "vs": "FP64 direct sum"},
         "stages_ms": kern, "work": work, "roofline": roof, "cpu_baseline": cpu,
+        "host_buffers": {"ms_per_step": ms_h, "value": n / (ms_h * 1e-3),
+                         "note": "bodies in pageable host memory, copied in and out over PCIe inside each call"},
     }
     print(json.dumps(out), flush=True)
 
@@ -477,6 +486,8 @@ def main():
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--m2l", choices=["bf16x3", "fp32", "fp64"], default="bf16x3")
+    ap.add_argument("--p2p", choices=["fp32", "fp32_newton"], default="fp32",
+                    help="plain workload: P2P precision (fp32_newton: Newton-refined 1/sqrt, compensated sums)")
     ap.add_argument("--truncation", choices=["full", "reference"], default="full")
     ap.add_argument("--targets", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")

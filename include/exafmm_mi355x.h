@@ -51,7 +51,9 @@ enum exafmm_m2l_truncation {
 enum exafmm_precision {
   EXAFMM_PREC_FP64 = 0,   /* FP64 everywhere (parity mode) */
   EXAFMM_PREC_FP32 = 1,   /* FP32 P2P / FP32 MFMA M2L */
-  EXAFMM_PREC_BF16X3 = 2  /* M2L only: 3-term BF16 split products, FP32 accumulate */
+  EXAFMM_PREC_BF16X3 = 2, /* M2L only: 3-term BF16 split products, FP32 accumulate */
+  EXAFMM_PREC_FP32_NEWTON = 3 /* P2P only: FP32 with one Newton step on the hardware
+                                 1/sqrt and Kahan-compensated partial sums */
 };
 
 enum exafmm_order_policy {
@@ -85,7 +87,7 @@ typedef struct exafmm_config {
   double theta;           /* multipole acceptance criterion */
   int32_t m2l_truncation; /* enum exafmm_m2l_truncation */
   int32_t m2l_precision;  /* enum exafmm_precision */
-  int32_t p2p_precision;  /* EXAFMM_PREC_FP64 or EXAFMM_PREC_FP32 */
+  int32_t p2p_precision;  /* EXAFMM_PREC_FP64, EXAFMM_PREC_FP32 or EXAFMM_PREC_FP32_NEWTON */
   int32_t images;         /* 0: free space; >=1: periodic, 3dp semantics */
   double cycle;           /* periodic cycle (images >= 1) */
   int32_t order_policy;   /* enum exafmm_order_policy */

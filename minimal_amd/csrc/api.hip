@@ -53,7 +53,8 @@ void check_config(const exafmm_config &cfg) {
     fail(EXAFMM_ERR_ARG, "bad m2l_truncation");
   if (cfg.m2l_precision < EXAFMM_PREC_FP64 || cfg.m2l_precision > EXAFMM_PREC_BF16X3)
     fail(EXAFMM_ERR_ARG, "bad m2l_precision");
-  if (cfg.p2p_precision != EXAFMM_PREC_FP64 && cfg.p2p_precision != EXAFMM_PREC_FP32)
+  if (cfg.p2p_precision != EXAFMM_PREC_FP64 && cfg.p2p_precision != EXAFMM_PREC_FP32 &&
+      cfg.p2p_precision != EXAFMM_PREC_FP32_NEWTON)
     fail(EXAFMM_ERR_ARG, "bad p2p_precision");
   if (cfg.images < 0 || cfg.images > 8) fail(EXAFMM_ERR_ARG, "images must be in [0, 8]");
   if (cfg.images > 0 && !(cfg.cycle > 0)) fail(EXAFMM_ERR_ARG, "periodic cycle must be positive");

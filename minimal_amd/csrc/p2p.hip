@@ -15,7 +15,9 @@
 // 64 lane partials are added with a fixed butterfly at the end.
 // Coordinates are stored relative to their own leaf centre, so every
 // difference is formed from small FP32 numbers.  1/r is the hardware
-// reciprocal square root (optionally refined by one Newton step).
+// reciprocal square root; EXAFMM_PREC_FP32_NEWTON refines it by one Newton
+// step and keeps Kahan-compensated partial sums (a second kernel, so the
+// default one keeps its register budget).
 #include <type_traits>
 
 #include "fmm_internal.h"
@@ -67,25 +69,45 @@ __device__ inline float halving(float (&v)[TC], int lane) {
   return v[0];
 }
 
+// Kahan step: s + v with the rounding error carried in c (s - c is the sum).
+__device__ inline void kahan_add(float &s, float &c, float v) {
+  const float y = v - c;
+  const float t = s + y;
+  c = (t - s) - y;
+  s = t;
+}
+
 // The interactions of one staged source per lane with the TC targets (the
 // TC chains are independent, so they interleave).  CHECK skips pairs at zero distance (3d/kernel.h:139); only
 // rounds that can hold such pairs (the target leaf's own bodies, or any
 // round when sources and targets come from different trees) use it.
-template <bool CHECK, int TC>
+// REFINE (EXAFMM_PREC_FP32_NEWTON) refines the hardware 1/sqrt by one
+// Newton step, y (3 - r^2 y^2) / 2, and adds each term into Kahan-
+// compensated partial sums (c*).
+template <bool CHECK, int TC, bool REFINE>
 __device__ inline void interact_block(const float3 (&xt)[TC], float4 sj, float (&ap)[TC], float (&ax)[TC],
-                                      float (&ay)[TC], float (&az)[TC]) {
+                                      float (&ay)[TC], float (&az)[TC], float (&cp)[TC], float (&cx)[TC],
+                                      float (&cy)[TC], float (&cz)[TC]) {
 #pragma unroll
   for (int t = 0; t < TC; t++) {
     const float dx = xt[t].x - sj.x, dy = xt[t].y - sj.y, dz = xt[t].z - sj.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    const float v = __builtin_amdgcn_rsqf(r2);
+    float v = __builtin_amdgcn_rsqf(r2);
+    if (REFINE) v = v * fmaf(-0.5f * r2 * v, v, 1.5f);
     const float inv = CHECK ? (r2 > 0.0f ? v : 0.0f) : v;
     const float qi = sj.w * inv;
     const float q3 = qi * inv * inv;
-    ap[t] += qi;
-    ax[t] += dx * q3;
-    ay[t] += dy * q3;
-    az[t] += dz * q3;
+    if (REFINE) {
+      kahan_add(ap[t], cp[t], qi);
+      kahan_add(ax[t], cx[t], dx * q3);
+      kahan_add(ay[t], cy[t], dy * q3);
+      kahan_add(az[t], cz[t], dz * q3);
+    } else {
+      ap[t] += qi;
+      ax[t] += dx * q3;
+      ay[t] += dy * q3;
+      az[t] += dz * q3;
+    }
   }
 }
 
@@ -93,9 +115,8 @@ __device__ inline void interact_block(const float3 (&xt)[TC], float4 sj, float (
 // Every target slot is computed (slots past tc repeat the last target and
 // are not written).  The entry of the target leaf itself is staged first,
 // so only the first rounds need the zero-distance check.  Unused source
-// lanes sit at 1e30 with q = 0: r^2 overflows to inf, rsqrt gives 0 and the
-// pair adds exact zeros.
-template <int TC>
+// lanes sit 1e18 away with q = 0: the pair adds exact zeros.
+template <int TC, bool REFINE>
 __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int b0, int nl, double4 Cc,
                                  int self_cell, const int *__restrict__ list, const int *__restrict__ scbeg,
                                  const int *__restrict__ sccnt, const double4 *__restrict__ scx,
@@ -107,9 +128,9 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
     const float4 v = xt_src[min(t, tc - 1)];
     xt[t] = make_float3(v.x, v.y, v.z);
   }
-  float ap[TC], ax[TC], ay[TC], az[TC];
+  float ap[TC], ax[TC], ay[TC], az[TC], cp[TC], cx[TC], cy[TC], cz[TC];
 #pragma unroll
-  for (int t = 0; t < TC; t++) ap[t] = ax[t] = ay[t] = az[t] = 0.0f;
+  for (int t = 0; t < TC; t++) ap[t] = ax[t] = ay[t] = az[t] = cp[t] = cx[t] = cy[t] = cz[t] = 0.0f;
   constexpr int kPer = kP2PSeg / kWave;
   for (int seg = 0; seg < nl; seg += kP2PSeg) {
     const int ne = min(kP2PSeg, nl - seg);
@@ -169,10 +190,11 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
     // forward.  Sources are fetched two rounds ahead into two register
     // buffers used alternately (no copies, so a fetch stays in flight while
     // the round before it computes).  A lane past the end fetches a valid
-    // body and gets the offset (1e30, 1e30, 1e30) and charge factor 0: r^2
-    // overflows to inf, rsqrt gives 0 and the pair adds exact zeros (body 0
-    // of the source tree always exists).
-    const float4 far_off = make_float4(1e30f, 1e30f, 1e30f, 0.0f);
+    // body and gets the offset (1e18, 1e18, 1e18) and charge factor 0: r^2
+    // is about 3e36 (finite, so a Newton step on its rsqrt stays finite),
+    // and the pair adds exact zeros (body 0 of the source tree always
+    // exists).
+    const float4 far_off = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
     int e = 0;
     struct Src {
       float4 c, o;
@@ -190,7 +212,7 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
     auto step = [&](int k0, Src b, auto check) {
       const float4 sj = make_float4(b.c.x + b.o.x, b.c.y + b.o.y, b.c.z + b.o.z, b.c.w * b.o.w);
       const Src nb = fetch(k0 + 2 * kWave + lane);
-      interact_block<decltype(check)::value, TC>(xt, sj, ap, ax, ay, az);
+      interact_block<decltype(check)::value, TC, REFINE>(xt, sj, ap, ax, ay, az, cp, cx, cy, cz);
       return nb;
     };
     Src b0 = fetch(lane), b1 = fetch(kWave + lane);
@@ -213,6 +235,15 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
   // lanes that share a target; lane l writes component l % (64 / TC) of
   // target l / (64 / TC) when that remainder is below 4
   constexpr int kLow = kWave / TC;  // lanes sharing a target after the halving
+  if (REFINE) {
+#pragma unroll
+    for (int t = 0; t < TC; t++) {
+      ap[t] -= cp[t];
+      ax[t] -= cx[t];
+      ay[t] -= cy[t];
+      az[t] -= cz[t];
+    }
+  }
   float sp = halving<TC>(ap, lane), sx = halving<TC>(ax, lane), sy = halving<TC>(ay, lane),
         sz = halving<TC>(az, lane);
 #pragma unroll
@@ -233,6 +264,7 @@ __device__ inline void p2p_block(int tc, const float4 *__restrict__ xt_src, int 
 }
 
 // One workgroup's share (block `bid`): kP2PWaves target leaves, one wave each.
+template <bool REFINE>
 __device__ __attribute__((always_inline)) inline void p2p32_body(int bid, const int *__restrict__ leaves, const int *__restrict__ nleaves,
                                   const int *__restrict__ cbeg, const int *__restrict__ ccnt,
                                   const double4 *__restrict__ cx, const float4 *__restrict__ xq,
@@ -255,32 +287,41 @@ __device__ __attribute__((always_inline)) inline void p2p32_body(int bid, const 
   const double4 Cc = cx[c];
   const int b0 = lbeg[c], nl = lcnt[c];
   const int self_cell = same_tree ? c : -2;  // -2: every round checks zero distances
-  for (int t0 = 0; t0 < tn; t0 += kP2PTC) {
-    const int tc = min(kP2PTC, tn - t0);
-    if (tc > kP2PTC / 2)
-      p2p_block<kP2PTC>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle, s_pre[w],
-                        s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
-    else if (tc > kP2PTC / 4)
-      p2p_block<kP2PTC / 2>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle,
-                            s_pre[w], s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
+  // REFINE carries a compensation per accumulator: blocks of at most 8
+  constexpr int kTC = REFINE ? kP2PTC / 2 : kP2PTC;
+  for (int t0 = 0; t0 < tn; t0 += kTC) {
+    const int tc = min(kTC, tn - t0);
+    if (tc > kTC / 2)
+      p2p_block<kTC, REFINE>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle, s_pre[w],
+                             s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
+    else if (!REFINE && tc > kTC / 4)
+      p2p_block<kTC / 2, REFINE>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle,
+                                 s_pre[w], s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
     else  // the last 1..4 targets of a leaf (a quarter of the leaves at 32 bodies each)
-      p2p_block<kP2PTC / 4>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle,
-                            s_pre[w], s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
+      p2p_block<kP2PTC / 4, REFINE>(tc, xq + tb + t0, b0, nl, Cc, self_cell, list, scbeg, sccnt, scx, sxq, cycle,
+                                    s_pre[w], s_beg[w], s_off[w], s_cnt[w], lane, bodies + tb + t0);
   }
 }
 
-// Three waves per SIMD (at most 168 VGPRs; two at the unconstrained 204):
-// the interaction loops keep every value in registers, the few spills fall
-// in the staging and the final sums, and the third wave hides the latency
-// of the source fetches (measured 1.82 -> 1.60 ms at N = 2^20).
 __global__ void __launch_bounds__(kP2PWaves *kWave, 3) k_p2p32(
                                                             const int *__restrict__ leaves, const int *__restrict__ nleaves, const int *__restrict__ cbeg,
                                                             const int *__restrict__ ccnt, const double4 *__restrict__ cx, const float4 *__restrict__ xq,
                                                             const int *__restrict__ lbeg, const int *__restrict__ lcnt, const int *__restrict__ list,
                                                             const int *__restrict__ scbeg, const int *__restrict__ sccnt, const double4 *__restrict__ scx,
                                                             const float4 *__restrict__ sxq, double cycle, int same_tree, exafmm_body *__restrict__ bodies) {
-  p2p32_body(blockIdx.x, leaves, nleaves, cbeg, ccnt, cx, xq, lbeg, lcnt, list, scbeg, sccnt, scx, sxq, cycle,
-             same_tree, bodies);
+  p2p32_body<false>(blockIdx.x, leaves, nleaves, cbeg, ccnt, cx, xq, lbeg, lcnt, list, scbeg, sccnt, scx, sxq,
+                    cycle, same_tree, bodies);
+}
+
+// EXAFMM_PREC_FP32_NEWTON: Newton-refined 1/sqrt and compensated sums.
+__global__ void __launch_bounds__(kP2PWaves *kWave) k_p2p32_refined(
+    const int *__restrict__ leaves, const int *__restrict__ nleaves, const int *__restrict__ cbeg,
+    const int *__restrict__ ccnt, const double4 *__restrict__ cx, const float4 *__restrict__ xq,
+    const int *__restrict__ lbeg, const int *__restrict__ lcnt, const int *__restrict__ list,
+    const int *__restrict__ scbeg, const int *__restrict__ sccnt, const double4 *__restrict__ scx,
+    const float4 *__restrict__ sxq, double cycle, int same_tree, exafmm_body *__restrict__ bodies) {
+  p2p32_body<true>(blockIdx.x, leaves, nleaves, cbeg, ccnt, cx, xq, lbeg, lcnt, list, scbeg, sccnt, scx, sxq,
+                   cycle, same_tree, bodies);
 }
 
 // All problems of a batch in one launch: block b belongs to the problem i
@@ -290,7 +331,7 @@ __global__ void __launch_bounds__(kP2PWaves *kWave, 3) k_p2p32_grouped(const Gro
                                                                     const int *__restrict__ start, int nprob) {
   const int b = blockIdx.x, i = group_of(start, nprob, b);
   const GroupTree &t = g[i];
-  p2p32_body(b - start[i], t.leaves, t.nleaves, t.cbeg, t.ccnt, t.cx, t.xq, t.lbeg, t.lcnt, t.list, t.cbeg, t.ccnt,
+  p2p32_body<false>(b - start[i], t.leaves, t.nleaves, t.cbeg, t.ccnt, t.cx, t.xq, t.lbeg, t.lcnt, t.list, t.cbeg, t.ccnt,
              t.cx, t.xq, 0.0, 1, t.bodies);
 }
 
@@ -357,9 +398,10 @@ void p2p_apply(Ctx &tc, const Ctx &sc, hipStream_t s) {
     hipLaunchKernelGGL(k_p2p64, dim3(T.nleaves), dim3(64), 0, s, T.leaves.p, T.misc.p + 17, T.cbeg.p, T.ccnt.p,
                        T.xs.p, L.p2p_beg.p, L.p2p_cnt.p, L.p2p.p, S.cbeg.p, S.ccnt.p, S.xs.p, cycle, tc.bodies);
   } else {
-    hipLaunchKernelGGL(k_p2p32, dim3(ceil_div(T.nleaves, kP2PWaves)), dim3(kP2PWaves * kWave), 0, s,
-                       T.leaves.p, T.misc.p + 17, T.cbeg.p, T.ccnt.p, T.cx.p, T.xq.p, L.p2p_beg.p, L.p2p_cnt.p,
-                       L.p2p.p, S.cbeg.p, S.ccnt.p, S.cx.p, S.xq.p, cycle, int(&tc == &sc), tc.bodies);
+    auto *k = tc.cfg.p2p_precision == EXAFMM_PREC_FP32_NEWTON ? k_p2p32_refined : k_p2p32;
+    hipLaunchKernelGGL(k, dim3(ceil_div(T.nleaves, kP2PWaves)), dim3(kP2PWaves * kWave), 0, s, T.leaves.p,
+                       T.misc.p + 17, T.cbeg.p, T.ccnt.p, T.cx.p, T.xq.p, L.p2p_beg.p, L.p2p_cnt.p, L.p2p.p,
+                       S.cbeg.p, S.ccnt.p, S.cx.p, S.xq.p, cycle, int(&tc == &sc), tc.bodies);
   }
   XF_LAUNCH();
 }

@@ -25,7 +25,7 @@ CELL_DTYPE = np.dtype([("nchild", "<i4"), ("nbody", "<i4"), ("child", "<i8"), ("
                        ("X", "<f8", (3,)), ("R", "<f8")])
 
 M2L_REFERENCE, M2L_FULL = 0, 1
-PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16X3, PREC_FP32_NEWTON = 0, 1, 2, 3
 ORDER_FIXED, ORDER_LEARNED = 0, 1
 
 ERRORS = {-1: "EXAFMM_ERR_ARG", -2: "EXAFMM_ERR_STATE", -3: "EXAFMM_ERR_DEPTH", -4: "EXAFMM_ERR_ALLOC",

@@ -6,6 +6,7 @@ Bars (written in each test):
   FP64 mode (M2L, P2P in FP64)     |diff| <= 1e-11 * max |ref|   (reference truncation, vs fixtures)
   upward multipoles (FP64)         <= 1e-12 relative
   FP32 P2P                         relative L2 of p, F vs oracle <= 2e-6
+  FP32 P2P, Newton + compensated   <= 2e-6 and no worse than plain FP32 (x1.05)
   direct sum (FP64)                <= 1e-13 relative
   operator surface (FP64)          <= 1e-12 relative
 """
@@ -97,6 +98,24 @@ def test_fp32_p2p_vs_oracle(gpu, orc, name):
     f.evaluate(b)
     assert rel_l2(b["p"], g["p"]) <= 2e-6
     assert rel_l2(b["F"], g["F"]) <= 2e-6
+
+
+@pytest.mark.parametrize("name", TREES)
+def test_fp32_newton_p2p_vs_oracle(gpu, name):
+    """P2P with the Newton-refined 1/sqrt and compensated sums (everything
+    else FP64, reference truncation) against the reference's p, F: no worse
+    than the plain FP32 P2P on the same set, and within 2e-6."""
+    g = golden(name)
+    errs = {}
+    for prec in (fmm.PREC_FP32, fmm.PREC_FP32_NEWTON):
+        f = _ctx(g, m2l_precision=fmm.PREC_FP64, p2p_precision=prec, m2l_truncation=fmm.M2L_REFERENCE)
+        b = g["input"].copy()
+        f.evaluate(b)
+        errs[prec] = (rel_l2(b["p"], g["p"]), rel_l2(b["F"], g["F"]))
+    print("fp32", errs[fmm.PREC_FP32], "newton", errs[fmm.PREC_FP32_NEWTON])
+    for k in (0, 1):
+        assert errs[fmm.PREC_FP32_NEWTON][k] <= 2e-6
+        assert errs[fmm.PREC_FP32_NEWTON][k] <= errs[fmm.PREC_FP32][k] * 1.05
 
 
 def test_direct_fp64(gpu, orc):
