@@ -221,33 +221,45 @@ __device__ inline int order_of(const float (&y)[kMlpOut], float log_tol) {
 // takes slots in turn and reduces the shapes of the slot's 256 columns
 // (coalesced row reads); then one thread per slot runs the MLP, takes the
 // smallest order under the tolerance and caps it at the context's order.
-__global__ void __launch_bounds__(256) k_order_select(const int *__restrict__ tfirst, const int *__restrict__ tcount,
-                                                      const int *__restrict__ tbase, const int *__restrict__ tU,
-                                                      const int *__restrict__ sorted, const int *__restrict__ tkeys,
-                                                      const int *__restrict__ tmat, const unsigned long long *keys,
-                                                      const int *__restrict__ clevel,
-                                                      const float *__restrict__ shape, const float *__restrict__ w,
-                                                      float log_tol, double R0, double cycle, int tn, int pmax,
-                                                      unsigned char *__restrict__ tile_p) {
+__global__ void __launch_bounds__(1024) k_order_select(const int *__restrict__ tfirst, const int *__restrict__ tcount,
+                                                       const int *__restrict__ tbase, const int *__restrict__ tU,
+                                                       const int *__restrict__ sorted, const int *__restrict__ tkeys,
+                                                       const int *__restrict__ tmat, const unsigned long long *keys,
+                                                       const int *__restrict__ clevel,
+                                                       const float *__restrict__ shape, const float *__restrict__ w,
+                                                       float log_tol, double R0, double cycle, int tn, int pmax,
+                                                       unsigned char *__restrict__ tile_p) {
   __shared__ float W[kMlpParams];
-  __shared__ float smax_s[256];
+  __shared__ float smax_s[1024];
   load_weights(w, W);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;  // 16 waves
   const int tile = blockIdx.x;
   const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
   const int lt = clevel[sorted[tfirst[tile]]];
-  for (int u0 = 0; u0 < U; u0 += 256) {
-    const int nu = min(256, U - u0);
-    for (int j = wv; j < nu; j += 4) {
-      const int *row = tmat + (int64_t(base) + u0 + j) * tn;
-      float m = -1e30f;
+  for (int u0 = 0; u0 < U; u0 += 1024) {
+    const int nu = min(1024, U - u0);
+    // largest source shape per key: a wave per key, two keys (j, j + 16)
+    // per pass so both rows' loads are in flight together
+    for (int j = wv; j < nu; j += 32) {
+      const int j2 = j + 16;
+      const bool two = j2 < nu;
+      const int *ra = tmat + (int64_t(base) + u0 + j) * tn;
+      const int *rb = tmat + (int64_t(base) + u0 + (two ? j2 : j)) * tn;
+      float ma = -1e30f, mb = -1e30f;
       for (int c = lane; c < ncol; c += 64) {
-        const int sc = row[c];
-        if (sc >= 0) m = fmaxf(m, shape[sc]);
+        const int sa = ra[c], sb = rb[c];
+        if (sa >= 0) ma = fmaxf(ma, shape[sa]);
+        if (sb >= 0) mb = fmaxf(mb, shape[sb]);
       }
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-      if (lane == 0) smax_s[j] = m;
+      for (int o = 32; o > 0; o >>= 1) {
+        ma = fmaxf(ma, __shfl_xor(ma, o, 64));
+        mb = fmaxf(mb, __shfl_xor(mb, o, 64));
+      }
+      if (lane == 0) {
+        smax_s[j] = ma;
+        if (two) smax_s[j2] = mb;
+      }
     }
     __syncthreads();
     if (int(threadIdx.x) < nu) {
@@ -411,7 +423,7 @@ void order_select(Ctx &c, const int *tfirst, const int *tcount, const int *tbase
   hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 4)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, c.mshape.p);
   XF_LAUNCH();
   const double cyc = c.cfg.images > 0 ? c.cfg.cycle : 0.0;
-  hipLaunchKernelGGL(k_order_select, dim3(ntiles), dim3(256), 0, s, tfirst, tcount, tbase, tU, sorted,
+  hipLaunchKernelGGL(k_order_select, dim3(ntiles), dim3(1024), 0, s, tfirst, tcount, tbase, tU, sorted,
                      c.tile_keys.p, c.tile_mat.p, c.lists.keys.p, c.t.clevel.p, c.mshape.p, c.mlp_w.p,
                      c.order_log_tol, c.t.R0, cyc, tn, c.P, c.tile_p.p);
   XF_LAUNCH();
