@@ -179,6 +179,38 @@ def copy_workload(args, torch):
                       "bytes_written_per_launch": y.numel() * 4, "launches": 3}), flush=True)
 
 
+def plain_learned(args, fmm, torch, dev, host, ptr, n, trunc, prec, p2p, ms_fixed):
+    """Learned against fixed order on the headline bodies: an order MLP
+    trained on the device (pair errors of a P = 14 context on another
+    uniform 2^20 set, 16,384 pairs, 400 Adam epochs; the training time is
+    reported, not timed in the steps) picks an order <= P per GEMM slot at
+    log10 tolerance --plain-learned-tol; same steps, same accuracy check."""
+    from minimal_amd.inputs import uniform_cube
+    t0 = time.perf_counter()
+    tr = fmm.FMM(P=14, ncrit=args.ncrit, theta=args.theta, m2l_precision=prec)
+    tr.build_tree(uniform_cube(1 << 20, 11, "sym"))
+    tr.upward()
+    tr.horizontal()
+    loss = tr.train_order_model(nsamples=16384, epochs=400, lr=3e-3)
+    weights = tr.order_model()
+    train_s = time.perf_counter() - t0
+    del tr
+    f = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=trunc, m2l_precision=prec,
+                p2p_precision=p2p, order_policy=fmm.ORDER_LEARNED)
+    f.set_order_model(weights, args.plain_learned_tol)
+    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+    ms, kern = timed(f, lambda: f.evaluate(ptr, n=n, on_device=True), args.steps, args.warmup, torch)
+    hist = f.order_stats()
+    err_p, err_F, nt = accuracy(f, dev, host, n, args.targets, torch, fmm)
+    log(f"learned order (tol {args.plain_learned_tol}): {ms:.3f} ms/step, err p {err_p:.3e} F {err_F:.3e}")
+    return {"value": n / (ms * 1e-3), "ms_per_step": ms, "speedup_vs_fixed": ms_fixed / ms,
+            "error": {"p": err_p, "F": err_F, "targets": int(nt), "vs": "FP64 direct sum"},
+            "log10_tolerance": args.plain_learned_tol, "orders": f"4..{args.P}",
+            "gemm_slot_orders": {str(p): int(c) for p, c in enumerate(hist) if c}, "stages_ms": kern,
+            "training": {"seconds": train_s, "loss": loss, "pairs": 16384, "epochs": 400,
+                         "set": "another uniform 2^20 set, P=14"}}
+
+
 def plain_workload(args, fmm, torch):
     from minimal_amd.inputs import uniform_cube
     prec = {"bf16x3": fmm.PREC_BF16X3, "fp32": fmm.PREC_FP32, "fp64": fmm.PREC_FP64}[args.m2l]
@@ -224,6 +256,9 @@ def plain_workload(args, fmm, torch):
     hb = host.copy()
     ms_h, _ = timed(f, lambda: f.evaluate(hb), 3, 1, torch)
     log(f"host buffers: {ms_h:.3f} ms/step")
+    lvf = None
+    if not args.no_learned:
+        lvf = plain_learned(args, fmm, torch, dev, host, ptr, n, trunc, prec, p2p, ms)
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -240,6 +275,7 @@ def plain_workload(args, fmm, torch):
         "stages_ms": kern, "work": work, "roofline": roof, "cpu_baseline": cpu,
         "host_buffers": {"ms_per_step": ms_h, "value": n / (ms_h * 1e-3),
                          "note": "bodies in pageable host memory, copied in and out over PCIe inside each call"},
+        "learned_vs_fixed": lvf,
     }
     print(json.dumps(out), flush=True)
 
@@ -486,6 +522,9 @@ def main():
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--m2l", choices=["bf16x3", "fp32", "fp64"], default="bf16x3")
+    ap.add_argument("--no-learned", action="store_true", help="plain workload: skip the learned-order comparison")
+    ap.add_argument("--plain-learned-tol", type=float, default=-7.0,
+                    help="plain workload: log10 tolerance of the learned-order comparison")
     ap.add_argument("--p2p", choices=["fp32", "fp32_newton"], default="fp32",
                     help="plain workload: P2P precision (fp32_newton: Newton-refined 1/sqrt, compensated sums)")
     ap.add_argument("--truncation", choices=["full", "reference"], default="full")
