@@ -88,23 +88,27 @@ print(json.dumps({{"kind": kind, "times": [float(x) for x in t]}}))
 
 def timed(f, step, steps: int, warmup: int, torch) -> tuple[float, dict]:
     """Warm up, then time `steps` calls of step() between device
-    synchronisations.  Returns ms per step and the per-stage durations
-    (HIP events recorded by the library on its streams) averaged over the
-    timed steps."""
+    synchronisations (ms per step).  Then the same number of steps again,
+    each followed by a read of the per-stage durations (HIP events recorded
+    by the library on its streams, which waits for both streams): their
+    means are the stage breakdown, kept out of the timed region so that its
+    extra per-step wait does not count."""
     for _ in range(warmup):
         step()
     f.synchronize()
     torch.cuda.synchronize()
-    acc = {}
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
-        st = f.stage_times()  # waits for both streams of the context
-        for k, v in st.items():
-            acc[k] = acc.get(k, 0.0) + v
     f.synchronize()
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) * 1e3 / steps
+    acc = {}
+    for _ in range(steps):
+        step()
+        st = f.stage_times()
+        for k, v in st.items():
+            acc[k] = acc.get(k, 0.0) + v
     return ms, {k: v / steps for k, v in acc.items()}
 
 
