@@ -61,8 +61,8 @@ __device__ inline unsigned mix32(unsigned h) {
   return h;
 }
 
-// Sort key of a target cell: group (level, octant) in bits 24..31 (255 = no
-// M2L), and with `minhash` the top 24 bits of the minimum hashed key id of
+// Sort key of a target cell: group (level, octant) in bits 16..23 (255 = no
+// M2L), and with `minhash` the top 16 bits of the minimum hashed key id of
 // its list below: cells with similar key sets share that minimum with a
 // probability equal to the sets' Jaccard similarity, so sorting by it packs
 // cells of like key sets into the same tile (fewer empty GEMM slots).
@@ -94,10 +94,10 @@ __global__ void __launch_bounds__(256) k_group_keys(int ncells, const double4 *_
         for (int k = 0; k < kTileUnroll; k++)
           if (id[k] >= 0) m = min(m, mix32(unsigned(id[k])));
       }
-      mh = m >> 8;
+      mh = m >> 16;  // 16 bits: three 8-bit sort passes over the 24-bit key
     }
   }
-  key[c] = (unsigned long long)(g << 24 | mh);
+  key[c] = (unsigned long long)(g << 16 | mh);
   val[c] = c;
 }
 
@@ -106,7 +106,7 @@ __global__ void __launch_bounds__(256) k_group_hist(int ncells, const unsigned l
   __shared__ int h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
-  for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) atomicAdd(&h[int(key[c] >> 24)], 1);
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < ncells; c += gridDim.x * 256) atomicAdd(&h[int(key[c] >> 16)], 1);
   __syncthreads();
   if (h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
 }
@@ -973,7 +973,7 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   XF_LAUNCH();
   unsigned long long *k1 = c.t.key.p, *k2 = c.t.key2.p;
   int *v1 = c.t.val.p, *v2 = c.t.val2.p;
-  radix_sort_pairs(k1, v1, k2, v2, nc, minhash ? 0 : 24, 32, c.m2l_rs_hist, s);
+  radix_sort_pairs(k1, v1, k2, v2, nc, minhash ? 0 : 16, 24, c.m2l_rs_hist, s);
   c.m2l_misc.ensure(512);
   XF_HIP(hipMemsetAsync(c.m2l_misc.p, 0, 256 * sizeof(int), s));
   hipLaunchKernelGGL(k_group_hist, dim3(std::min(ceil_div(nc, 256), 64)), dim3(256), 0, s, nc, k1, c.m2l_misc.p);
