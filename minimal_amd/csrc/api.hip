@@ -596,11 +596,11 @@ int exafmm_order_stats(exafmm_ctx *ctx, int64_t *hist, int capacity) {
     const int nt = c.m2l_ntiles_last, U = c.m2l_totalU;
     if (nt == 0 || U == 0) return;
     std::vector<int> tU(nt);
-    XF_HIP(hipMemcpy(tU.data(), c.tiles.p + 3 * nt, sizeof(int) * nt, hipMemcpyDeviceToHost));
+    XF_HIP(hipMemcpy(tU.data(), c.tiles.p + 3 * c.m2l_tstride, sizeof(int) * nt, hipMemcpyDeviceToHost));
     std::vector<unsigned char> tp(U, (unsigned char)c.P);
     if (c.learned) XF_HIP(hipMemcpy(tp.data(), c.tile_p.p, U, hipMemcpyDeviceToHost));
     std::vector<int> cnt(nt);
-    XF_HIP(hipMemcpy(cnt.data(), c.tiles.p + nt, sizeof(int) * nt, hipMemcpyDeviceToHost));
+    XF_HIP(hipMemcpy(cnt.data(), c.tiles.p + c.m2l_tstride, sizeof(int) * nt, hipMemcpyDeviceToHost));
     int64_t u = 0;
     for (int t = 0; t < nt; t++)
       for (int k = 0; k < tU[t]; k++, u++)

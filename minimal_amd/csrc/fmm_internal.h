@@ -200,6 +200,7 @@ struct Ctx {
   // keyed M2L tiles built by m2l_prepare for the current lists
   bool m2l_ready = false;
   int m2l_ntiles = 0;
+  int m2l_tstride = 0;      // stride of the tile arrays in `tiles` (a bound on the tile count)
   int m2l_ncell_tiled = 0;  // cells that receive M2L (the tiles' columns)
   Buf<int> m2l_misc, m2l_rs_hist, m2l_scan;
   Buf<float> m2l_part;      // FP32 partial locals of the key slices [slice][cell][P^2]

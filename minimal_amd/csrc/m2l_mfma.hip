@@ -135,12 +135,44 @@ __device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, c
   __syncthreads();
 }
 
+// Tiles from the group populations gc[0..254] (group 255 = cells without
+// M2L, not tiled): each group's cells, in sorted order, are cut into tiles
+// of at most kTN.  Writes each tile's first cell and count, and plan[0] =
+// tiles, plan[1] = cells in tiles.
+__global__ void __launch_bounds__(256) k_tile_plan(const int *__restrict__ gc, int *__restrict__ tfirst,
+                                                   int *__restrict__ tcount, int *__restrict__ plan) {
+  __shared__ int tb[256], cb[256];
+  const int gi = threadIdx.x;
+  const int n = gi < 255 ? gc[gi] : 0;
+  tb[gi] = (n + kTN - 1) / kTN;
+  cb[gi] = n;
+  __syncthreads();
+  if (gi == 0) {  // exclusive prefixes of tiles and cells over the groups
+    int t = 0, cc = 0;
+    for (int i = 0; i < 256; i++) {
+      const int a = tb[i], b = cb[i];
+      tb[i] = t;
+      cb[i] = cc;
+      t += a;
+      cc += b;
+    }
+    plan[0] = t;
+    plan[1] = cc;
+  }
+  __syncthreads();
+  for (int j = 0; j * kTN < n; j++) {
+    tfirst[tb[gi] + j] = cb[gi] + j * kTN;
+    tcount[tb[gi] + j] = min(kTN, n - j * kTN);
+  }
+}
+
 __global__ void __launch_bounds__(kTN) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                     const int *__restrict__ sorted, const int *__restrict__ beg,
                                                     const int *__restrict__ cnt, const int2 *__restrict__ list,
-                                                    int nkeys, int *__restrict__ tU) {
+                                                    int nkeys, const int *__restrict__ plan, int *__restrict__ tU) {
   extern __shared__ unsigned bm[];
   __shared__ int tot;
+  if (int(blockIdx.x) >= plan[0]) return;  // launched over the bound of tiles
   const int nwords = (nkeys + 31) / 32;
   const int *cells = sorted + tfirst[blockIdx.x];
   const int ncol = tcount[blockIdx.x];
@@ -798,9 +830,9 @@ template <int RH>
 static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, int RTB,
                         int kr, int ksplit, float *Lpart, double *Lout) {
   Ctx &c = tc;
-  const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ntiles, *tbase = c.tiles.p + 2 * ntiles,
-            *tU = c.tiles.p + 3 * ntiles;
-  const int *sorted = c.tiles.p + 4 * ntiles;
+  const int ts = c.m2l_tstride;
+  const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ts, *tbase = c.tiles.p + 2 * ts, *tU = c.tiles.p + 3 * ts;
+  const int *sorted = c.tiles.p + 4 * ts;
   const dim3 grid(ntiles, ceil_div(g.R, RTB), ksplit);
   if (bf16) {
     const v8bf *ophi = reinterpret_cast<const v8bf *>(c.ops32.p);
@@ -856,7 +888,7 @@ static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16
   if (sh.ksplit > 1) {
     const int64_t n = int64_t(c.m2l_ncell_tiled) * g.PP;
     hipLaunchKernelGGL(k_combine_slices, dim3(ceil_div(n, 256)), dim3(256), 0, c.s0, g.PP, int64_t(c.t.ncells),
-                       sh.ksplit, c.tiles.p + 4 * ntiles, c.m2l_ncell_tiled, part, Lout);
+                       sh.ksplit, c.tiles.p + 4 * c.m2l_tstride, c.m2l_ncell_tiled, part, Lout);
     XF_LAUNCH();
   }
 }
@@ -978,41 +1010,36 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   XF_HIP(hipMemsetAsync(c.m2l_misc.p, 0, 256 * sizeof(int), s));
   hipLaunchKernelGGL(k_group_hist, dim3(std::min(ceil_div(nc, 256), 64)), dim3(256), 0, s, nc, k1, c.m2l_misc.p);
   XF_LAUNCH();
-  std::vector<int> gc(256);
+  // tiles of at most kTN cells per group, laid out on the device (no host
+  // round trip before the per-tile counts); arrays strided by the bound
+  // ntmax: [first | count | base | U] x ntmax, then the sorted cells
+  const int ntmax = ceil_div(nc, kTN) + 255;
+  c.tiles.ensure(size_t(4) * ntmax + nc);
+  int *d_first = c.tiles.p, *d_count = c.tiles.p + ntmax, *d_base = c.tiles.p + 2 * ntmax,
+      *d_U = c.tiles.p + 3 * ntmax, *d_sorted = c.tiles.p + 4 * ntmax;
+  int *d_plan = c.m2l_misc.p + 257;  // [ntiles, cells in tiles]
+  hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(256), 0, s, c.m2l_misc.p, d_first, d_count, d_plan);
+  XF_LAUNCH();
+  XF_HIP(hipMemcpyAsync(d_sorted, v1, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
+  const int nwords = (nkeys + 31) / 32;
+  hipLaunchKernelGGL(k_tile_count, dim3(ntmax), dim3(kTN), sizeof(unsigned) * nwords, s, d_first, d_count,
+                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_plan, d_U);
+  XF_LAUNCH();
+  scan_exclusive_dev(d_U, d_base, d_plan, ntmax, c.m2l_misc.p + 256, c.m2l_scan, s);
+  // one host round trip: tile count, slots, cells in tiles, and the key set
+  int plan[3] = {0, 0, 0};
   std::vector<unsigned long long> keys(nkeys);
-  XF_HIP(hipMemcpyAsync(gc.data(), c.m2l_misc.p, 256 * sizeof(int), hipMemcpyDeviceToHost, s));
+  XF_HIP(hipMemcpyAsync(plan, c.m2l_misc.p + 256, sizeof(plan), hipMemcpyDeviceToHost, s));
   if (nkeys)
     XF_HIP(hipMemcpyAsync(keys.data(), c.lists.keys.p, sizeof(unsigned long long) * nkeys, hipMemcpyDeviceToHost,
                           s));
   XF_HIP(hipStreamSynchronize(s));
-  std::vector<int> tfirst, tcount;
-  int off = 0;
-  for (int gi = 0; gi < 255; gi++) {
-    for (int i = 0; i < gc[gi]; i += kTN) {
-      tfirst.push_back(off + i);
-      tcount.push_back(std::min(kTN, gc[gi] - i));
-    }
-    off += gc[gi];
-  }
-  const int ntiles = int(tfirst.size());
+  const int totalU = plan[0], ntiles = plan[1], off = plan[2];
+  c.m2l_tstride = ntmax;
   if (ntiles == 0) {
     c.m2l_ready = true;
     return;
   }
-  c.tiles.ensure(size_t(4) * ntiles + nc);
-  XF_HIP(hipMemcpyAsync(c.tiles.p, tfirst.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, s));
-  XF_HIP(hipMemcpyAsync(c.tiles.p + ntiles, tcount.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, s));
-  XF_HIP(hipMemcpyAsync(c.tiles.p + 4 * ntiles, v1, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
-  int *d_first = c.tiles.p, *d_count = c.tiles.p + ntiles, *d_base = c.tiles.p + 2 * ntiles,
-      *d_U = c.tiles.p + 3 * ntiles, *d_sorted = c.tiles.p + 4 * ntiles;
-  const int nwords = (nkeys + 31) / 32;
-  hipLaunchKernelGGL(k_tile_count, dim3(ntiles), dim3(kTN), sizeof(unsigned) * nwords, s, d_first, d_count,
-                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_U);
-  XF_LAUNCH();
-  scan_exclusive(d_U, d_base, ntiles, c.m2l_misc.p + 256, c.m2l_scan, s);
-  int totalU = 0;
-  XF_HIP(hipMemcpyAsync(&totalU, c.m2l_misc.p + 256, sizeof(int), hipMemcpyDeviceToHost, s));
-  XF_HIP(hipStreamSynchronize(s));
   c.tile_keys.ensure(std::max(totalU, 1));
   c.tile_mat.ensure(size_t(std::max(totalU, 1)) * kTN);
   hipLaunchKernelGGL(k_tile_fill, dim3(ntiles), dim3(kTN), sizeof(unsigned) * 2 * nwords, s, d_first, d_count,
@@ -1082,8 +1109,8 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   // per-key orders of a learned-order evaluation (need the multipoles)
   c.learned = c.cfg.order_policy == EXAFMM_ORDER_LEARNED && c.have_model;
   if (c.learned)
-    order_select(c, c.tiles.p, c.tiles.p + ntiles, c.tiles.p + 2 * ntiles, c.tiles.p + 3 * ntiles,
-                 c.tiles.p + 4 * ntiles, ntiles, c.m2l_totalU, kTN, s);
+    order_select(c, c.tiles.p, c.tiles.p + c.m2l_tstride, c.tiles.p + 2 * c.m2l_tstride,
+                 c.tiles.p + 3 * c.m2l_tstride, c.tiles.p + 4 * c.m2l_tstride, ntiles, c.m2l_totalU, kTN, s);
 
   // 3. GEMM over tiles, shape from the autotuner
   const GemmShape sh = gemm_shape(c, sc, g, ntiles, bf16, KP, KPb);
@@ -1153,11 +1180,12 @@ static void group_tables(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, i
     const __bf16 *ophi = reinterpret_cast<const __bf16 *>(c.ops32.p);
     __bf16 *mhi = reinterpret_cast<__bf16 *>(c.mt32.p);
     GroupGemm q;
+    const int ts = c.m2l_tstride;
     q.tfirst = c.tiles.p;
-    q.tcount = c.tiles.p + nt;
-    q.tbase = c.tiles.p + 2 * nt;
-    q.tU = c.tiles.p + 3 * nt;
-    q.sorted = c.tiles.p + 4 * nt;
+    q.tcount = c.tiles.p + ts;
+    q.tbase = c.tiles.p + 2 * ts;
+    q.tU = c.tiles.p + 3 * ts;
+    q.sorted = c.tiles.p + 4 * ts;
     q.tkeys = c.tile_keys.p;
     q.tmat = c.tile_mat.p;
     q.ophi = reinterpret_cast<const v8bf *>(ophi);
