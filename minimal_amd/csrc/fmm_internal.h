@@ -225,6 +225,7 @@ struct Ctx {
   Buf<int> tiles;                  // M2L tile descriptors
   Buf<int> tile_mat;               // per tile [U][kTileN] source indices
   Buf<int> tile_keys;
+  Buf<unsigned> tile_bits;         // per tile: bitmap of its key set (k_tile_count -> k_tile_fill)
   Buf<int> misc;
   // periodic far images
   Buf<double2> far_sup, far_part;

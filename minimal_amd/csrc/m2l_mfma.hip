@@ -169,7 +169,8 @@ __global__ void __launch_bounds__(256) k_tile_plan(const int *__restrict__ gc, i
 __global__ void __launch_bounds__(kTN) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                     const int *__restrict__ sorted, const int *__restrict__ beg,
                                                     const int *__restrict__ cnt, const int2 *__restrict__ list,
-                                                    int nkeys, const int *__restrict__ plan, int *__restrict__ tU) {
+                                                    int nkeys, const int *__restrict__ plan, int *__restrict__ tU,
+                                                    unsigned *__restrict__ tbits) {
   extern __shared__ unsigned bm[];
   __shared__ int tot;
   if (int(blockIdx.x) >= plan[0]) return;  // launched over the bound of tiles
@@ -180,77 +181,92 @@ __global__ void __launch_bounds__(kTN) k_tile_count(const int *__restrict__ tfir
   if (threadIdx.x == 0) tot = 0;
   __syncthreads();
   int s = 0;
-  for (int i = threadIdx.x; i < nwords; i += blockDim.x) s += __popc(bm[i]);
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) {
+    s += __popc(bm[i]);
+    tbits[int64_t(blockIdx.x) * nwords + i] = bm[i];  // the key set, for k_tile_fill
+  }
   atomicAdd(&tot, s);
   __syncthreads();
   if (threadIdx.x == 0) tU[blockIdx.x] = tot;
 }
 
 // Key list (ascending) and source matrix [U][kTN] (-1 = no pair) of a tile.
-__global__ void __launch_bounds__(kTN) k_tile_fill(const int *__restrict__ tfirst, const int *__restrict__ tcount,
+// Workgroup (tile, q) owns columns 64q .. 64q + 63: it marks them empty in
+// every key row, then four threads per column walk that column's M2L list
+// (entries i = sub mod 4) and write the source cell into the key's row.
+// The key set comes from k_tile_count's bitmap.
+constexpr int kFillParts = kTN / 64;
+__global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                    const int *__restrict__ sorted, const int *__restrict__ beg,
                                                    const int *__restrict__ cnt, const int2 *__restrict__ list,
                                                    int nkeys, const int *__restrict__ tbase,
-                                                   int *__restrict__ tkeys, int *__restrict__ tmat) {
+                                                   const unsigned *__restrict__ tbits, int *__restrict__ tkeys,
+                                                   int *__restrict__ tmat) {
   extern __shared__ unsigned bm[];
   const int nwords = (nkeys + 31) / 32;
   unsigned *rank = bm + nwords;  // exclusive popcount prefix per word
-  const int *cells = sorted + tfirst[blockIdx.x];
-  const int ncol = tcount[blockIdx.x];
-  tile_bitmap(cells, ncol, beg, cnt, list, bm, nwords);
-  // exclusive prefix of the word popcounts (block scan in chunks of kTN words)
+  const int tile = blockIdx.x, q = blockIdx.y;
+  const int *cells = sorted + tfirst[tile];
+  const int ncol = tcount[tile];
+  for (int i = threadIdx.x; i < nwords; i += blockDim.x) bm[i] = tbits[int64_t(tile) * nwords + i];
+  // exclusive prefix of the word popcounts (block scan in chunks of 256 words)
   __shared__ int carry;
+  __shared__ int wsum[256 / 64];
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
-  for (int w0 = 0; w0 < nwords; w0 += kTN) {
+  for (int w0 = 0; w0 < nwords; w0 += 256) {
     const int i = w0 + threadIdx.x;
     const int v = i < nwords ? __popc(bm[i]) : 0;
     int incl = v;
-    // wave scan, then wave totals
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const int u = __shfl_up(incl, d);
       if (lane >= d) incl += u;
     }
-    __shared__ int wsum[kTN / 64];
     if (lane == 63) wsum[wv] = incl;
     __syncthreads();
     int before = carry;
     for (int k = 0; k < wv; k++) before += wsum[k];
     if (i < nwords) rank[i] = before + incl - v;
     __syncthreads();
-    if (threadIdx.x == kTN - 1) carry = before + incl;
+    if (threadIdx.x == 255) carry = before + incl;
     __syncthreads();
   }
-  const int base = tbase[blockIdx.x];
-  for (int i = threadIdx.x; i < nwords; i += blockDim.x) {
-    unsigned w = bm[i];
-    int r = rank[i];
-    while (w) {
-      const int b = __ffs(w) - 1;
-      w &= w - 1;
-      tkeys[base + r++] = i * 32 + b;
+  const int base = tbase[tile];
+  const int U = carry;
+  if (q == 0) {
+    for (int i = threadIdx.x; i < nwords; i += blockDim.x) {
+      unsigned w = bm[i];
+      int r = rank[i];
+      while (w) {
+        const int bt = __ffs(w) - 1;
+        w &= w - 1;
+        tkeys[base + r++] = i * 32 + bt;
+      }
     }
   }
-  const int U = carry;
-  for (int i = threadIdx.x; i < U * kTN; i += blockDim.x) tmat[int64_t(base) * kTN + i] = -1;
+  for (int i = threadIdx.x; i < U * 64; i += blockDim.x)
+    tmat[(int64_t(base) + i / 64) * kTN + q * 64 + (i & 63)] = -1;
   __syncthreads();
-  const int t = threadIdx.x;
-  if (t < ncol) {
-    const int c = cells[t];
+  const int col = q * 64 + (threadIdx.x >> 2), sub = threadIdx.x & 3;
+  if (col < ncol) {
+    const int c = cells[col];
     const int b = beg[c], n = cnt[c];
-    for (int i0 = 0; i0 < n; i0 += kTileUnroll) {
+    for (int i0 = sub; i0 < n; i0 += 4 * kTileUnroll) {
       int2 en[kTileUnroll];
 #pragma unroll
-      for (int k = 0; k < kTileUnroll; k++) en[k] = i0 + k < n ? list[b + i0 + k] : make_int2(0, -1);
+      for (int k = 0; k < kTileUnroll; k++) {
+        const int i = i0 + 4 * k;
+        en[k] = i < n ? list[b + i] : make_int2(0, -1);
+      }
 #pragma unroll
       for (int k = 0; k < kTileUnroll; k++) {
         const int id = en[k].y;
         if (id < 0) continue;
         const unsigned w = bm[id >> 5];
         const int r = rank[id >> 5] + __popc(w & ((1u << (id & 31)) - 1u));
-        tmat[(int64_t(base) + r) * kTN + t] = en[k].x >> 5;  // source cell (the image is part of the key)
+        tmat[(int64_t(base) + r) * kTN + col] = en[k].x >> 5;  // source cell (the image is part of the key)
       }
     }
   }
@@ -1022,8 +1038,10 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   XF_LAUNCH();
   XF_HIP(hipMemcpyAsync(d_sorted, v1, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
   const int nwords = (nkeys + 31) / 32;
+  c.tile_bits.ensure(size_t(ntmax) * std::max(nwords, 1));
   hipLaunchKernelGGL(k_tile_count, dim3(ntmax), dim3(kTN), sizeof(unsigned) * nwords, s, d_first, d_count,
-                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_plan, d_U);
+                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_plan, d_U,
+                     c.tile_bits.p);
   XF_LAUNCH();
   scan_exclusive_dev(d_U, d_base, d_plan, ntmax, c.m2l_misc.p + 256, c.m2l_scan, s);
   // one host round trip: tile count, slots, cells in tiles, and the key set
@@ -1042,9 +1060,9 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   }
   c.tile_keys.ensure(std::max(totalU, 1));
   c.tile_mat.ensure(size_t(std::max(totalU, 1)) * kTN);
-  hipLaunchKernelGGL(k_tile_fill, dim3(ntiles), dim3(kTN), sizeof(unsigned) * 2 * nwords, s, d_first, d_count,
-                     d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_base,
-                     c.tile_keys.p, c.tile_mat.p);
+  hipLaunchKernelGGL(k_tile_fill, dim3(ntiles, kFillParts), dim3(256), sizeof(unsigned) * 2 * nwords, s, d_first,
+                     d_count, d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_base,
+                     c.tile_bits.p, c.tile_keys.p, c.tile_mat.p);
   XF_LAUNCH();
 
   // 2. operators (cached by key set)
