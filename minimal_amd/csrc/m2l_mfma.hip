@@ -66,13 +66,16 @@ __device__ inline unsigned mix32(unsigned h) {
 // its list below: cells with similar key sets share that minimum with a
 // probability equal to the sets' Jaccard similarity, so sorting by it packs
 // cells of like key sets into the same tile (fewer empty GEMM slots).
+// One wave per cell: the lanes take the list entries in turn and the
+// minimum hash is a wave reduction.
 __global__ void __launch_bounds__(256) k_group_keys(int ncells, const double4 *__restrict__ cx,
                                                     const int *__restrict__ cparent, const int *__restrict__ clevel,
                                                     const int *__restrict__ m2l_beg, const int *__restrict__ m2l_cnt,
                                                     const int2 *__restrict__ list, int minhash,
                                                     unsigned long long *__restrict__ key, int *__restrict__ val) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c >= ncells) return;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int c = blockIdx.x * (256 / kWave) + int(threadIdx.x / kWave);
+  if (c >= ncells) return;  // whole wave
   unsigned g = 255, mh = 0;
   const int n = m2l_cnt[c];
   if (n > 0) {
@@ -86,19 +89,16 @@ __global__ void __launch_bounds__(256) k_group_keys(int ncells, const double4 *_
     if (minhash) {
       unsigned m = 0xffffffffu;
       const int b0 = m2l_beg[c];
-      for (int i0 = 0; i0 < n; i0 += kTileUnroll) {  // kTileUnroll entries in flight
-        int id[kTileUnroll];
+      for (int i = lane; i < n; i += kWave) m = min(m, mix32(unsigned(list[b0 + i].y)));
 #pragma unroll
-        for (int k = 0; k < kTileUnroll; k++) id[k] = i0 + k < n ? list[b0 + i0 + k].y : -1;
-#pragma unroll
-        for (int k = 0; k < kTileUnroll; k++)
-          if (id[k] >= 0) m = min(m, mix32(unsigned(id[k])));
-      }
+      for (int d = 32; d > 0; d >>= 1) m = min(m, unsigned(__shfl_xor(int(m), d)));
       mh = m >> 16;  // 16 bits: three 8-bit sort passes over the 24-bit key
     }
   }
-  key[c] = (unsigned long long)(g << 16 | mh);
-  val[c] = c;
+  if (lane == 0) {
+    key[c] = (unsigned long long)(g << 16 | mh);
+    val[c] = c;
+  }
 }
 
 __global__ void __launch_bounds__(256) k_group_hist(int ncells, const unsigned long long *__restrict__ key,
@@ -1015,7 +1015,7 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   // (measured 0.50 -> 0.58 on a 2^22-body Plummer set); on uniform trees,
   // where BFS neighbours already share key sets, it changes little.
   const int minhash = 1;
-  hipLaunchKernelGGL(k_group_keys, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, c.t.cx.p, c.t.cparent.p,
+  hipLaunchKernelGGL(k_group_keys, dim3(ceil_div(nc, 256 / kWave)), dim3(256), 0, s, nc, c.t.cx.p, c.t.cparent.p,
                      c.t.clevel.p, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, minhash, c.t.key.p,
                      c.t.val.p);
   XF_LAUNCH();
