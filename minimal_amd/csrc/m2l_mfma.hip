@@ -115,18 +115,19 @@ __global__ void __launch_bounds__(256) k_group_hist(int ncells, const unsigned l
 // (entries (source << 5 | image, key id)), kTileUnroll entries in flight.
 
 // Distinct keys of one tile: bitmap over key ids in LDS.
+// Four threads per column (blockDim.x == 4 kTN), entries i = sub mod 4.
 __device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, const int *cnt, const int2 *list,
                                    unsigned *bm, int nwords) {
   for (int i = threadIdx.x; i < nwords; i += blockDim.x) bm[i] = 0;
   __syncthreads();
-  const int t = threadIdx.x;  // blockDim.x == kTN
+  const int t = threadIdx.x >> 2, sub = threadIdx.x & 3;
   if (t < ncol) {
     const int c = cells[t];
     const int b = beg[c], n = cnt[c];
-    for (int i0 = 0; i0 < n; i0 += kTileUnroll) {
+    for (int i0 = sub; i0 < n; i0 += 4 * kTileUnroll) {
       int id[kTileUnroll];
 #pragma unroll
-      for (int k = 0; k < kTileUnroll; k++) id[k] = i0 + k < n ? list[b + i0 + k].y : -1;
+      for (int k = 0; k < kTileUnroll; k++) id[k] = i0 + 4 * k < n ? list[b + i0 + 4 * k].y : -1;
 #pragma unroll
       for (int k = 0; k < kTileUnroll; k++)
         if (id[k] >= 0) atomicOr(&bm[id[k] >> 5], 1u << (id[k] & 31));
@@ -166,7 +167,7 @@ __global__ void __launch_bounds__(256) k_tile_plan(const int *__restrict__ gc, i
   }
 }
 
-__global__ void __launch_bounds__(kTN) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
+__global__ void __launch_bounds__(4 * kTN) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                     const int *__restrict__ sorted, const int *__restrict__ beg,
                                                     const int *__restrict__ cnt, const int2 *__restrict__ list,
                                                     int nkeys, const int *__restrict__ plan, int *__restrict__ tU,
@@ -1039,7 +1040,7 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   XF_HIP(hipMemcpyAsync(d_sorted, v1, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
   const int nwords = (nkeys + 31) / 32;
   c.tile_bits.ensure(size_t(ntmax) * std::max(nwords, 1));
-  hipLaunchKernelGGL(k_tile_count, dim3(ntmax), dim3(kTN), sizeof(unsigned) * nwords, s, d_first, d_count,
+  hipLaunchKernelGGL(k_tile_count, dim3(ntmax), dim3(4 * kTN), sizeof(unsigned) * nwords, s, d_first, d_count,
                      d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_plan, d_U,
                      c.tile_bits.p);
   XF_LAUNCH();
