@@ -223,22 +223,13 @@ __global__ void __launch_bounds__(512) k_hist_top(int *__restrict__ hist, const 
 // Node t of level l: if it holds more than ncrit bodies, find the body
 // sub-ranges of its 8 octants (keys are sorted by the level-(l+1) digit
 // inside the node's range) by binary search.
-__global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__restrict__ lcount,
-                                               const int *__restrict__ lstart, const int *__restrict__ cbeg,
-                                               const int *__restrict__ ccnt,
-                                               const unsigned long long *__restrict__ key,
-                                               const int *__restrict__ hist, int *__restrict__ nch,
-                                               int *__restrict__ bnd) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= lcount[l]) return;
-  const int c = lstart[l] + t;
-  const int beg = cbeg[c], cnt = ccnt[c];
-  if (cnt <= ncrit || l >= kMaxLevel) {
-    nch[t] = 0;
-    return;
-  }
+// Child boundaries of a level-l cell holding bodies [beg, beg + cnt) of the
+// sorted keys: b[o] .. b[o + 1] are octant o's bodies.  Returns the number of
+// non-empty children (0: the cell is a leaf, b untouched).
+__device__ inline int split_bounds(int l, int ncrit, int beg, int cnt, const unsigned long long *__restrict__ key,
+                                   const int *__restrict__ hist, int b[9]) {
+  if (cnt <= ncrit || l >= kMaxLevel) return 0;
   const int sh = 3 * (kMaxLevel - 1 - l);
-  int b[9];
   b[0] = beg;
   b[8] = beg + cnt;
   if (l + 1 <= kHistLevels) {
@@ -261,12 +252,50 @@ __global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__re
     }
   }
   int k = 0;
-  for (int o = 0; o < 8; o++) {
-    bnd[t * 9 + o] = b[o];
-    k += b[o + 1] > b[o];
-  }
-  bnd[t * 9 + 8] = b[8];
+  for (int o = 0; o < 8; o++) k += b[o + 1] > b[o];
+  return k;
+}
+
+__global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__restrict__ lcount,
+                                               const int *__restrict__ lstart, const int *__restrict__ cbeg,
+                                               const int *__restrict__ ccnt,
+                                               const unsigned long long *__restrict__ key,
+                                               const int *__restrict__ hist, int *__restrict__ nch,
+                                               int *__restrict__ bnd) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= lcount[l]) return;
+  const int c = lstart[l] + t;
+  int b[9];
+  const int k = split_bounds(l, ncrit, cbeg[c], ccnt[c], key, hist, b);
   nch[t] = k;
+  if (k == 0) return;
+  for (int o = 0; o < 9; o++) bnd[t * 9 + o] = b[o];
+}
+
+// Writes the non-empty children of level-l cell c (boundaries b) to cells
+// first, first + 1, ... in octant order.
+__device__ inline void emit_children(int l, int c, int first, const int b[9], const double *__restrict__ root,
+                                     double4 *__restrict__ cx, int *__restrict__ cbeg, int *__restrict__ ccnt,
+                                     int *__restrict__ cchild, int *__restrict__ cparent, int *__restrict__ clevel) {
+  cchild[c] = first;
+  const double4 P = cx[c];
+  const double r = root[3] / (1 << (l + 1));
+  int j = 0;
+  for (int o = 0; o < 8; o++) {
+    const int b0 = b[o], b1 = b[o + 1];
+    if (b1 <= b0) continue;
+    const int ch = first + j++;
+    double4 X;
+    X.x = P.x + ((o & 1) ? r : -r);
+    X.y = P.y + ((o & 2) ? r : -r);
+    X.z = P.z + ((o & 4) ? r : -r);
+    X.w = r;
+    cx[ch] = X;
+    cbeg[ch] = b0;
+    ccnt[ch] = b1 - b0;
+    cparent[ch] = c;
+    clevel[ch] = l + 1;
+  }
 }
 
 __global__ void k_level_next(int l, int *__restrict__ lcount, int *__restrict__ lstart, const int *__restrict__ total) {
@@ -298,25 +327,80 @@ __global__ void __launch_bounds__(256) k_emit(int l, const int *__restrict__ lco
     cchild[c] = -1;
     return;
   }
-  const int first = lstart[l + 1] + off[t];
-  cchild[c] = first;
-  const double4 P = cx[c];
-  const double r = root[3] / (1 << (l + 1));
-  int j = 0;
-  for (int o = 0; o < 8; o++) {
-    const int b0 = bnd[t * 9 + o], b1 = bnd[t * 9 + o + 1];
-    if (b1 <= b0) continue;
-    const int ch = first + j++;
-    double4 X;
-    X.x = P.x + ((o & 1) ? r : -r);
-    X.y = P.y + ((o & 2) ? r : -r);
-    X.z = P.z + ((o & 4) ? r : -r);
-    X.w = r;
-    cx[ch] = X;
-    cbeg[ch] = b0;
-    ccnt[ch] = b1 - b0;
-    cparent[ch] = c;
-    clevel[ch] = l + 1;
+  int b[9];
+  for (int o = 0; o < 9; o++) b[o] = bnd[t * 9 + o];
+  emit_children(l, c, lstart[l + 1] + off[t], b, root, cx, cbeg, ccnt, cchild, cparent, clevel);
+}
+
+// The top levels, 0 .. lend - 1, split by one workgroup: per level the
+// k_split / exclusive scan / k_level_next / k_emit sequence above, with a
+// workgroup scan and barriers in place of launches.  Level l holds at most
+// 8^l cells, so levels up to kTopLevels - 1 fit kTopPer cells per thread;
+// thread t owns cells kTopPer t .. kTopPer t + kTopPer - 1 of its level.
+// Level 4 (4096 cells) stays on the multi-workgroup kernels: one workgroup
+// walking four cells per thread through their dependent loads (counts, key,
+// histogram, centre) took longer than the four launches it saves.
+constexpr int kTopThreads = 1024, kTopPer = 1, kTopLevels = 4;
+static_assert(1 << (3 * (kTopLevels - 1)) <= kTopThreads * kTopPer, "top levels exceed the workgroup");
+__global__ void __launch_bounds__(kTopThreads)
+    k_top_levels(int lend, int ncrit, int *__restrict__ lcount, int *__restrict__ lstart,
+                 const unsigned long long *__restrict__ key, const int *__restrict__ hist,
+                 const double *__restrict__ root, double4 *__restrict__ cx, int *__restrict__ cbeg,
+                 int *__restrict__ ccnt, int *__restrict__ cchild, int *__restrict__ cnch,
+                 int *__restrict__ cparent, int *__restrict__ clevel, int *__restrict__ nsplit) {
+  __shared__ int wsum[kTopThreads / kWave];
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid / kWave;
+  int cnt = lcount[0], start = lstart[0];
+  for (int l = 0; l < lend; l++) {
+    int b[kTopPer][9], k[kTopPer];
+    int mine = 0, nspl = 0;
+#pragma unroll
+    for (int j = 0; j < kTopPer; j++) {
+      const int t = tid * kTopPer + j;
+      k[j] = 0;
+      if (t < cnt) {
+        const int c = start + t;
+        k[j] = split_bounds(l, ncrit, cbeg[c], ccnt[c], key, hist, b[j]);
+      }
+      mine += k[j];
+      nspl += k[j] > 0;
+    }
+    // exclusive scan of the children counts over the workgroup
+    int incl = mine;
+    for (int d = 1; d < kWave; d <<= 1) {
+      const int v = __shfl_up(incl, d);
+      if (lane >= d) incl += v;
+    }
+    for (int d = kWave / 2; d > 0; d >>= 1) nspl += __shfl_xor(nspl, d);
+    if (lane == 0 && nspl) atomicAdd(nsplit, nspl);  // cells that split
+    if (lane == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    int off = incl - mine, total = 0;
+    for (int i = 0; i < kTopThreads / kWave; i++) {
+      off += i < w ? wsum[i] : 0;
+      total += wsum[i];
+    }
+    const int next = start + cnt;
+#pragma unroll
+    for (int j = 0; j < kTopPer; j++) {
+      const int t = tid * kTopPer + j;
+      if (t < cnt) {
+        const int c = start + t;
+        cnch[c] = k[j];
+        if (k[j] == 0)
+          cchild[c] = -1;
+        else
+          emit_children(l, c, next + off, b[j], root, cx, cbeg, ccnt, cchild, cparent, clevel);
+        off += k[j];
+      }
+    }
+    if (tid == 0) {
+      lcount[l + 1] = total;
+      lstart[l + 1] = next;
+    }
+    __syncthreads();  // children written before the next level reads them
+    start = next;
+    cnt = total;
   }
 }
 
@@ -486,7 +570,14 @@ void tree_build(Ctx &c) {
   hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, n, root, t.cx.p, t.cbeg.p, t.ccnt.p, t.cparent.p, t.clevel.p,
                      t.lcount.p, t.lstart_d.p);
   XF_LAUNCH();
-  for (int l = 0; l < Ds; l++) {
+  const int lfuse = std::min(Ds, kTopLevels);
+  if (lfuse > 0) {
+    hipLaunchKernelGGL(k_top_levels, dim3(1), dim3(kTopThreads), 0, s, lfuse, ncrit, t.lcount.p, t.lstart_d.p, k1,
+                       t.hist.p, root, t.cx.p, t.cbeg.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.cparent.p, t.clevel.p,
+                       t.misc.p + 18);
+    XF_LAUNCH();
+  }
+  for (int l = lfuse; l < Ds; l++) {
     const int g = ceil_div(ub[l], 256);
     hipLaunchKernelGGL(k_split, dim3(g), dim3(256), 0, s, l, ncrit, t.lcount.p, t.lstart_d.p, t.cbeg.p, t.ccnt.p, k1,
                        t.hist.p, t.tmp_nch.p, t.tmp_bnd.p);
