@@ -419,59 +419,74 @@ __global__ void __launch_bounds__(kShiftWaves *kWave) k_l2l_grouped(const GroupT
 // collected per harmonic Y_n^m so each column m is generated once.
 // Potential and field of one body at u = (x, y, z) (leaf units) from the
 // leaf's folded coefficients Lh (LDS) and the recurrence factors rec.
-template <class T>
-__device__ __attribute__((always_inline)) inline void l2p_eval(int P, T x, T y, T z, const cpx<T> *Lh, const T *rec,
-                                                              T &pot, T &Fx2, T &Fy2, T &Dz) {
+// PC > 0: the order is the compile-time PC and the (n, m) loops unroll, so
+// the LDS offsets and the branches below resolve at compile time.
+template <class T, int PC = 0>
+__device__ __attribute__((always_inline)) inline void l2p_eval(int Prt, T x, T y, T z, const cpx<T> *Lh,
+                                                              const T *rec, T &pot, T &Fx2, T &Fy2, T &Dz) {
+  const int P = PC > 0 ? PC : Prt;
   const T r2 = x * x + y * y + z * z;
   T Ar = 0, Ai = 0, Br = 0, Bi = 0;
   pot = 0;
   Dz = 0;
-  T ymr = 1, ymi = 0;  // Y_m^m
-  for (int m = 0; m < P; m++) {
-    if (m > 0) {
-      const T inv = rec[nms(m, m)];
-      const T tr = (ymr * x - ymi * y) * inv, ti = (ymr * y + ymi * x) * inv;
-      ymr = tr;
-      ymi = ti;
-    }
-    T pr = 0, pi = 0, cr = ymr, ci = ymi;
-    for (int n = m; n < P; n++) {
-      if (n > m) {
-        T nr, ni;
-        if (n == m + 1) {
-          nr = -z * cr;
-          ni = -z * ci;
-        } else {
-          const T s = rec[nms(n, m)];
-          const T cz = T(2 * n - 1) * z;
-          nr = (cz * cr + r2 * pr) * s;
-          ni = (cz * ci + r2 * pi) * s;
-        }
-        pr = cr;
-        pi = ci;
-        cr = nr;
-        ci = ni;
+  T ymr = 1, ymi = 0;          // Y_m^m
+  T pr = 0, pi = 0, cr = 0, ci = 0;  // Y_{n-1}^m, Y_n^m
+  auto term = [&](int m, int n) {
+    if (n == m) {
+      if (m > 0) {
+        const T inv = rec[nms(m, m)];
+        const T tr = (ymr * x - ymi * y) * inv, ti = (ymr * y + ymi * x) * inv;
+        ymr = tr;
+        ymi = ti;
       }
-      const cpx<T> l0 = Lh[nms(n, m)];
-      pot += l0.re * cr - l0.im * ci;
-      if (n + 1 < P) {
-        const cpx<T> lz = Lh[nms(n + 1, m)];
-        Dz -= lz.re * cr - lz.im * ci;
-        if (m >= 1) {
-          const cpx<T> la = Lh[nms(n + 1, m - 1)];
-          Ar -= la.re * cr - la.im * ci;
-          Ai -= la.re * ci + la.im * cr;
-        }
-        const cpx<T> lb = Lh[nms(n + 1, m + 1)];
-        Br += lb.re * cr - lb.im * ci;
-        Bi += lb.re * ci + lb.im * cr;
-        if (m == 1) {  // the m = 0 terms' (dx - i dy): -Lh_{n+1}^0 conj(Y_n^1)
-          const cpx<T> l00 = Lh[nms(n + 1, 0)];
-          Br -= l00.re * cr + l00.im * ci;
-          Bi -= l00.im * cr - l00.re * ci;
-        }
+      pr = 0;
+      pi = 0;
+      cr = ymr;
+      ci = ymi;
+    } else {
+      T nr, ni;
+      if (n == m + 1) {
+        nr = -z * cr;
+        ni = -z * ci;
+      } else {
+        const T s = rec[nms(n, m)];
+        const T cz = T(2 * n - 1) * z;
+        nr = (cz * cr + r2 * pr) * s;
+        ni = (cz * ci + r2 * pi) * s;
+      }
+      pr = cr;
+      pi = ci;
+      cr = nr;
+      ci = ni;
+    }
+    const cpx<T> l0 = Lh[nms(n, m)];
+    pot += l0.re * cr - l0.im * ci;
+    if (n + 1 < P) {
+      const cpx<T> lz = Lh[nms(n + 1, m)];
+      Dz -= lz.re * cr - lz.im * ci;
+      if (m >= 1) {
+        const cpx<T> la = Lh[nms(n + 1, m - 1)];
+        Ar -= la.re * cr - la.im * ci;
+        Ai -= la.re * ci + la.im * cr;
+      }
+      const cpx<T> lb = Lh[nms(n + 1, m + 1)];
+      Br += lb.re * cr - lb.im * ci;
+      Bi += lb.re * ci + lb.im * cr;
+      if (m == 1) {  // the m = 0 terms' (dx - i dy): -Lh_{n+1}^0 conj(Y_n^1)
+        const cpx<T> l00 = Lh[nms(n + 1, 0)];
+        Br -= l00.re * cr + l00.im * ci;
+        Bi -= l00.im * cr - l00.re * ci;
       }
     }
+  };
+  if constexpr (PC > 0) {
+#pragma unroll
+    for (int m = 0; m < PC; m++)
+#pragma unroll
+      for (int n = m; n < PC; n++) term(m, n);
+  } else {
+    for (int m = 0; m < P; m++)
+      for (int n = m; n < P; n++) term(m, n);
   }
   Fx2 = Ar + Br;  // 2 dphi/dx
   Fy2 = Ai - Bi;  // 2 dphi/dy
@@ -526,7 +541,7 @@ __device__ __attribute__((always_inline)) inline void l2p_body(int bx, int P, co
 // uniform trees, a few when many small leaves share a wave), and each lane
 // evaluates its own body against its run's slot.
 constexpr int kL2PSlots = 8;
-template <int MAXP>
+template <int MAXP, int PC>
 __global__ void __launch_bounds__(64) k_l2p_flat(int P, int64_t n, const int *__restrict__ body_leaf,
                                                  const double4 *__restrict__ cx, const double *__restrict__ L,
                                                  const double4 *__restrict__ xs, exafmm_body *__restrict__ bodies) {
@@ -569,8 +584,8 @@ __global__ void __launch_bounds__(64) k_l2p_flat(int P, int64_t n, const int *__
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     if (valid && run >= g0 && run < g0 + ns) {
       float pot, fx2, fy2, dz;
-      l2p_eval<float>(P, float((B.x - C.x) * invR), float((B.y - C.y) * invR), float((B.z - C.z) * invR),
-                      Lh[run - g0], rec, pot, fx2, fy2, dz);
+      l2p_eval<float, PC>(P, float((B.x - C.x) * invR), float((B.y - C.y) * invR), float((B.z - C.z) * invR),
+                          Lh[run - g0], rec, pot, fx2, fy2, dz);
       exafmm_body *o = bodies + b;
       o->p += double(pot);
       o->F[0] += 0.5 * double(fx2) * invR;
@@ -676,9 +691,15 @@ void l2p_pass(Ctx &c) {
   if (c.cfg.m2l_precision == EXAFMM_PREC_FP64)
     hipLaunchKernelGGL((k_l2p<kMaxP, double>), dim3(t.nleaves), dim3(64), 0, s, P, t.leaves.p, t.misc.p + 17,
                        t.cbeg.p, t.ccnt.p, t.cx.p, c.L.p, t.xs.p, c.bodies);
-  else
-    hipLaunchKernelGGL((k_l2p_flat<kMaxP>), dim3(ceil_div(t.n, kWave)), dim3(kWave), 0, s, P, t.n, t.body_leaf.p,
-                       t.cx.p, c.L.p, t.xs.p, c.bodies);
+  else {
+    // the common orders unrolled at compile time, any other order generic
+    auto flat = k_l2p_flat<kMaxP, 0>;
+    if (P == 6) flat = k_l2p_flat<kMaxP, 6>;
+    if (P == 8) flat = k_l2p_flat<kMaxP, 8>;
+    if (P == 10) flat = k_l2p_flat<kMaxP, 10>;
+    hipLaunchKernelGGL(flat, dim3(ceil_div(t.n, kWave)), dim3(kWave), 0, s, P, t.n, t.body_leaf.p, t.cx.p, c.L.p,
+                       t.xs.p, c.bodies);
+  }
   XF_LAUNCH();
 }
 
