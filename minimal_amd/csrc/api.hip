@@ -130,8 +130,9 @@ void do_lists(Ctx &tc, const Ctx &sc) {
 
 // P2P on the second stream from the current point of the main stream, so it
 // overlaps whatever the main stream runs next (upward pass, M2L).
-void do_p2p(Ctx &tc, const Ctx &sc) {
-  rec(tc, 14, tc.s0);
+// after_lists: event 14 already marks the end of the lists on s0
+void do_p2p(Ctx &tc, const Ctx &sc, bool after_lists = false) {
+  if (!after_lists) rec(tc, 14, tc.s0);
   XF_HIP(hipStreamWaitEvent(tc.s1, tc.ev[14], 0));
   rec(tc, 5, tc.s1);
   p2p_apply(tc, sc, tc.s1);
@@ -328,12 +329,14 @@ int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_devi
     set_bodies(c, bodies, n, on_device);
     do_build(c);
     do_lists(c, c);
-    // P2P needs only the lists: it starts on s1 at once, and its long
-    // VALU-bound blocks fill the CUs that the latency-bound chains on s0
-    // (upward pass, M2L tiles and operators) leave idle; the keyed GEMM
-    // follows on s0
-    do_p2p(c, c);
+    // P2P needs only the lists (event 14): it runs on s1 beside the upward
+    // pass and the M2L tile and operator build on s0, its long VALU-bound
+    // blocks filling the CUs those latency-bound chains leave idle; the
+    // keyed GEMM follows on s0.  The upward pass is enqueued first so the
+    // P2M workgroups are dispatched ahead of P2P's.
+    rec(c, 14, c.s0);
     do_upward(c);
+    do_p2p(c, c, true);
     do_far(c, c);
     do_downward(c);
     bodies_to_host(c);
