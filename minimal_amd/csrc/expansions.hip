@@ -232,6 +232,10 @@ void shift_ops(Ctx &c) {
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kShiftWaves = 4;  // row tiles per workgroup
 constexpr int kShiftBatch = 4;  // k-steps of 4 columns loaded together
+// L2L runs alone on the device (after the GEMM), so it loads deeper: all
+// 25 k-steps of order 10 in two rounds (M2M runs beside P2P, where more
+// registers cost it more than the latency it saves)
+constexpr int kL2LBatch = 13;
 
 __device__ inline int degree_of(int row) {
   int j = 0;
@@ -369,17 +373,17 @@ __device__ __attribute__((always_inline)) inline void l2l_body(int bx, int by, i
   const int pl = min(p0 + j16, lcount - 1);
   const double *Lp = L + int64_t(lstart + pl) * PP;
   v4d acc = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < ksteps; k0 += kShiftBatch) {
-    double a[kShiftBatch], b[kShiftBatch];
+  for (int k0 = 0; k0 < ksteps; k0 += kL2LBatch) {
+    double a[kL2LBatch], b[kL2LBatch];
 #pragma unroll
-    for (int s = 0; s < kShiftBatch; s++) {
+    for (int s = 0; s < kL2LBatch; s++) {
       const int col = c0 + (k0 + s) * 4 + kq;
       const bool in = col < PP;
       a[s] = in ? To[int64_t(col) * PP + row] : 0.0;
       b[s] = (in && ch >= 0) ? Lp[col] : 0.0;
     }
 #pragma unroll
-    for (int s = 0; s < kShiftBatch; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+    for (int s = 0; s < kL2LBatch; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
   }
   if (ch < 0) return;
 #pragma unroll
