@@ -12,15 +12,21 @@
 // structure (index-based cells, level-ordered passes, explicit work stacks)
 // is this file's own.
 //
-// Parity pins (see tests/test_oracle_*.py):
+// Parity pins (tests/test_oracle_golden.py, tests/test_oracle_pins.py):
 //   tree, lists, p/F at N=1e3/1e4      -> reference fixtures, bit-exact
+//   tree digest, list counts, N=2^20   -> reference fixture
 //   operators at P=4/10/14             -> reference fixtures, <=1e-14 rel
 //   KAT (3d/kernel.cxx)                -> reference error values
-//   P2P/direct                         -> brute force + Newton's 3rd law
+//   P2P                                -> reference fixtures, an independent
+//                                         numpy sum, Newton's third law
 //   M2M/L2L                            -> exact shift identities
-//   M2L full mode                      -> convergence to direct sum
-//   Ewald                              -> convergence in alpha/ksize and
-//                                         periodic direct sum on tiny N
+//   M2L full mode                      -> errors of the reference's kernels
+//                                         filled to degree 2P-1 (config-1
+//                                         set, KAT), convergence to the
+//                                         direct sum in P
+//   periodic, full mode                -> the reference's 3dp run at P=18
+//   Ewald                              -> the reference's Ewald fixture and
+//                                         independence of alpha
 #include "oracle.h"
 
 #include <cmath>

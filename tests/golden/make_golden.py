@@ -5,10 +5,12 @@ oracle/_ref/ by oracle/Makefile); the fixtures it writes are committed so the
 tests can run anywhere.  Every value in a fixture is an output of the
 reference's buildTree / getList / kernels / passes (3d/ and 3dp/ headers).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # every fixture
+    python tests/golden/make_golden.py config2    # one section by name
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import sys
 
@@ -46,9 +48,64 @@ def tree_and_lists(ref, bodies, P, ncrit, theta, with_fmm=True, with_lists=True)
     return out
 
 
+def tree_digest(bodies: np.ndarray, cells: np.ndarray) -> str:
+    """SHA-256 over the tree-ordered X and q (little-endian FP64) and the
+    cell table (nchild, nbody, child, body, X, R per cell, in the
+    reference's numbering); the same digest is computed from the device's
+    outputs by the tests."""
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(bodies["X"], "<f8").tobytes())
+    h.update(np.ascontiguousarray(bodies["q"], "<f8").tobytes())
+    for f in ("nchild", "nbody", "child", "body", "X", "R"):
+        h.update(np.ascontiguousarray(cells[f]).tobytes())
+    return h.hexdigest()
+
+
+def config2(ref):
+    """Config 2 at its stated size: N = 2^20 uniform in [0,1)^3, q = U[-1,1)
+    (drand48 seed 0, the reference's draw order), P = 10, theta = 0.4,
+    ncrit = 64.  Stores the digest of the reference's tree, its list sizes
+    and interaction counts, and the reference's FP64 direct sum on 1,000
+    stride-sampled targets of the tree-ordered bodies (3d/fmm.cxx:57-69)."""
+    n = 1 << 20
+    b = inputs.uniform_cube(n, 0, "sym")
+    ref.set(10, 64, 0.4)
+    cells = ref.build_tree(b)
+    (mo, ms), (po, ps) = ref.lists()
+    nb = cells["nbody"].astype(np.int64)
+    tgt = np.repeat(np.arange(len(cells)), np.diff(po))
+    inter = int(np.sum(nb[tgt] * nb[ps]))
+    idx = inputs.sample_targets(n, 1000)
+    tg = b[idx].copy()
+    tg["p"] = 0
+    tg["F"] = 0
+    ref.direct(tg, b)
+    save("config2_n2p20.npz", n=n, P=10, ncrit=64, theta=0.4, digest=np.array(tree_digest(b, cells)),
+         ncells=len(cells), nleaves=int(np.sum(cells["nchild"] == 0)), m2l_pairs=len(ms), p2p_pairs=len(ps),
+         p2p_interactions=inter, p2p_source_bodies=int(np.sum(nb[ps])), targets=idx,
+         direct_p=tg["p"], direct_F=tg["F"])
+
+
+def periodic_high_order(refp):
+    """The 3dp driver's set evaluated by the reference at P = 18 (other
+    parameters as shipped): the near-converged periodic sum that pins the
+    full-truncation periodic evaluation."""
+    drv = inputs.reference_driver_bodies(1000, 0)
+    b2 = drv.copy()
+    refp.fmm(b2, 18, 64, 0.4, 4, 2 * np.pi)
+    save("periodic_n1000_p18.npz", p=b2["p"], F=b2["F"])
+
+
 def main():
     oracle.build()
     ref = oracle.Reference()
+    only = sys.argv[1:]
+    if only:
+        if "config2" in only:
+            config2(ref)
+        if "periodic_p18" in only:
+            periodic_high_order(oracle.ReferencePeriodic())
+        return
     rng = np.random.default_rng(20261202)
 
     # Config-1 body set (N=1e4, [0,1)^3, q=U[0,1)/N), P=10, theta=0.4, ncrit=64.
@@ -145,6 +202,8 @@ def main():
     ew = b2.copy()
     refp.ewald(ew, 64, 11, 11 / (2 * np.pi), 0.25 / np.pi, np.pi, 2 * np.pi)
     save("periodic_n1000.npz", input=drv, fmm_sorted=b2, ewald=ew)
+    periodic_high_order(refp)
+    config2(ref)
 
 
 if __name__ == "__main__":
