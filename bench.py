@@ -86,6 +86,21 @@ print(json.dumps({{"kind": kind, "times": [float(x) for x in t]}}))
             "stage_s": d["times"]}
 
 
+def upload(dev, host, torch) -> None:
+    """Copy a host body array into a device buffer and wait for the copy:
+    the library's streams do not wait on torch's stream, so device data must
+    be complete before a library call reads it."""
+    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+    torch.cuda.synchronize()
+
+
+def to_device(host, torch):
+    """A new device buffer holding a host body array, complete on return."""
+    d = torch.from_numpy(host.view(np.uint8).copy()).cuda()
+    torch.cuda.synchronize()
+    return d
+
+
 def timed(f, step, steps: int, warmup: int, torch) -> tuple[float, dict]:
     """Warm up, then time `steps` calls of step() between device
     synchronisations (ms per step).  Then the same number of steps again,
@@ -116,7 +131,7 @@ def accuracy(f, dev, host, n, ntargets, torch, fmm_mod):
     """One evaluation of a fresh copy, then the FP64 direct sum on the
     stride-sampled targets (3d/fmm.cxx:57-61).  Returns (err_p, err_F)."""
     from minimal_amd.inputs import BODY_DTYPE, rel_l2, sample_targets
-    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+    upload(dev, host, torch)
     ptr = dev.data_ptr()
     f.evaluate(ptr, n=n, on_device=True)
     f.synchronize()
@@ -125,7 +140,7 @@ def accuracy(f, dev, host, n, ntargets, torch, fmm_mod):
     tg = res[idx].copy()
     tg["p"] = 0
     tg["F"] = 0
-    tdev = torch.from_numpy(tg.view(np.uint8).copy()).cuda()
+    tdev = to_device(tg, torch)
     f.direct(tdev.data_ptr(), ptr, nt=len(tg), ns=n, on_device=True)
     ref = tdev.cpu().numpy().view(BODY_DTYPE)
     return rel_l2(res["p"][idx], ref["p"]), rel_l2(res["F"][idx], ref["F"]), len(idx)
@@ -202,7 +217,7 @@ def plain_learned(args, fmm, torch, dev, host, ptr, n, trunc, prec, p2p, ms_fixe
     f = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=trunc, m2l_precision=prec,
                 p2p_precision=p2p, order_policy=fmm.ORDER_LEARNED)
     f.set_order_model(weights, args.plain_learned_tol)
-    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+    upload(dev, host, torch)
     ms, kern = timed(f, lambda: f.evaluate(ptr, n=n, on_device=True), args.steps, args.warmup, torch)
     hist = f.order_stats()
     err_p, err_F, nt = accuracy(f, dev, host, n, args.targets, torch, fmm)
@@ -221,7 +236,7 @@ def plain_workload(args, fmm, torch):
     trunc = fmm.M2L_FULL if args.truncation == "full" else fmm.M2L_REFERENCE
     n = args.n or (1 << 20)
     host = uniform_cube(n, 0, "sym")
-    dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
+    dev = to_device(host, torch)
     ptr = dev.data_ptr()
     p2p = {"fp32": fmm.PREC_FP32, "fp32_newton": fmm.PREC_FP32_NEWTON}[args.p2p]
     f = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=trunc, m2l_precision=prec,
@@ -245,7 +260,7 @@ def plain_workload(args, fmm, torch):
                      p2p_precision=p2p)
     finally:
         del os.environ["EXAFMM_SERIAL_STREAMS"]
-    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+    upload(dev, host, torch)
     ms_i, kern_i = timed(fi, lambda: fi.evaluate(ptr, n=n, on_device=True), args.steps, args.warmup, torch)
     roof = roofline(wl, kern_i, work, args.P, args.m2l)
     roof["kernel_ms_from"] = "one-stream pass (each kernel alone on the CUs)"
@@ -304,7 +319,7 @@ def learned_workload(args, fmm, torch):
     del tr
     log(f"trained order model in {train_s:.2f} s, loss {loss:.3f}")
     host = plummer_background(n, 6)
-    dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
+    dev = to_device(host, torch)
     ptr = dev.data_ptr()
     res = {}
     for name in ("fixed", "learned"):
@@ -314,7 +329,7 @@ def learned_workload(args, fmm, torch):
             f = fmm.FMM(P=args.P_fixed, ncrit=args.ncrit, theta=args.theta, m2l_precision=prec,
                         order_policy=fmm.ORDER_LEARNED)
             f.set_order_model(weights, args.tol)
-        dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+        upload(dev, host, torch)
         ms, kern = timed(f, lambda: f.evaluate(ptr, n=n, on_device=True), args.steps, args.warmup, torch)
         hist = f.order_stats()
         work = f.work_counts()
@@ -363,7 +378,7 @@ def batched_workload(args, fmm, torch):
     offs = np.concatenate([[0], np.cumsum(ns)])
     N = int(offs[-1])
     host = np.concatenate([b for b, _ in probs])
-    dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
+    dev = to_device(host, torch)
     base = dev.data_ptr()
     log(f"bench batched: {len(probs)} problems, N total {N}, P counts "
         f"{ {P: sum(1 for _, q in probs if q == P) for P in (6, 8, 12)} }")
@@ -464,7 +479,7 @@ def periodic_workload(args, fmm, torch):
     host["q"] = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
     cycle = 1.0
     prec = {"bf16x3": fmm.PREC_BF16X3, "fp32": fmm.PREC_FP32, "fp64": fmm.PREC_FP64}[args.m2l]
-    dev = torch.from_numpy(host.view(np.uint8).copy()).cuda()
+    dev = to_device(host, torch)
     ptr = dev.data_ptr()
     f = fmm.FMM(P=args.P, ncrit=args.ncrit, theta=args.theta, m2l_truncation=fmm.M2L_FULL, m2l_precision=prec,
                 p2p_precision=fmm.PREC_FP32, images=args.images, cycle=cycle)
@@ -477,7 +492,7 @@ def periodic_workload(args, fmm, torch):
     ms, kern = timed(f, step, args.steps, args.warmup, torch)
     work = f.work_counts()
     # accuracy: fresh evaluation without the per-body correction
-    dev.copy_(torch.from_numpy(host.view(np.uint8).copy()).cuda())
+    upload(dev, host, torch)
     f.evaluate(ptr, n=n, on_device=True)
     f.synchronize()
     res = dev.cpu().numpy().view(BODY_DTYPE)
@@ -485,7 +500,7 @@ def periodic_workload(args, fmm, torch):
     tg = res[idx].copy()
     tg["p"] = 0
     tg["F"] = 0
-    tdev = torch.from_numpy(tg.view(np.uint8).copy()).cuda()
+    tdev = to_device(tg, torch)
     ksize, alpha, sigma, cutoff = 11, 10.0, 0.25 / np.pi, 0.5
     f.ewald(tdev.data_ptr(), ptr, ksize, alpha, sigma, cutoff, nt=len(tg), ns=n, on_device=True)
     ew = tdev.cpu().numpy().view(BODY_DTYPE)

@@ -20,6 +20,14 @@
  *    (3d/kernel.h:196-197), cells in the reference's numbering.
  *  - One context per stream; calls on one context are serialised by the
  *    caller, different contexts may be used from different threads.
+ *  - Device-pointer inputs must be complete when the call is made: the
+ *    context's own streams do not wait on any other stream, so a caller
+ *    that filled the bodies with work on its own stream synchronises that
+ *    stream (or the device) first.  Results are complete on device memory
+ *    after exafmm_synchronize(ctx); host-array calls return complete.
+ *  - Host body arrays: a host pointer given to exafmm_build_tree (or
+ *    exafmm_evaluate) is kept by the context until exafmm_downward writes p
+ *    and F back into it; the array must stay valid and unmoved until then.
  */
 #ifndef EXAFMM_MI355X_H
 #define EXAFMM_MI355X_H

@@ -9,17 +9,22 @@
 //   passes       upwardPass, horizontalPass, downwardPass, direct
 //                                                   (3d/traverse_eager.h:140-202)
 //   kernels      initKernel, P2P, P2M, M2M, M2L, L2L, L2P (3d/kernel.h:118-322)
-// The passes run on the GPU through one device context per Cells object;
-// Cell::M and Cell::L are filled from the device after the passes so code
-// that inspects them keeps working.  The single-call kernels run one FP64
-// device call each.  The extra knobs of the C ABI (truncation, precision,
-// device) are set through exafmm::options().
+// The passes run on the GPU through one device context per Cells object:
+// Cells is the reference's std::vector<Cell> with a shared handle on that
+// context, which is destroyed with the last Cells that holds it (so a
+// time-stepping driver that rebuilds its tree every step holds one context at
+// a time).  Cell::M is filled from the device after upwardPass and Cell::L
+// after horizontalPass (the M2L sums, as the reference's M2L writes them) and
+// again after downwardPass, so code that inspects them keeps working.  The
+// single-call kernels run one FP64 device call each.  The extra knobs of the
+// C ABI (truncation, precision, device) are set through exafmm::options().
 #ifndef EXAFMM_MI355X_HPP
 #define EXAFMM_MI355X_HPP
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
-#include <map>
+#include <atomic>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -51,7 +56,29 @@ struct Cell {  // 3d/exafmm.h:23-36
   std::vector<complex_t> M;
   std::vector<complex_t> L;
 };
-typedef std::vector<Cell> Cells;
+namespace detail {
+inline std::atomic<int> &live_contexts() {  // device contexts held by Cells objects
+  static std::atomic<int> n{0};
+  return n;
+}
+// The device context of one tree; destroyed with the last Cells holding it.
+struct Handle {
+  exafmm_ctx *ctx = nullptr;
+  explicit Handle(exafmm_ctx *c) : ctx(c) { live_contexts()++; }
+  Handle(const Handle &) = delete;
+  Handle &operator=(const Handle &) = delete;
+  ~Handle() {
+    exafmm_destroy(ctx);
+    live_contexts()--;
+  }
+};
+}  // namespace detail
+
+// std::vector<Cell> (3d/exafmm.h:37) plus the device context of its tree.
+struct Cells : std::vector<Cell> {
+  using std::vector<Cell>::vector;
+  std::shared_ptr<detail::Handle> device;
+};
 
 inline int P = 10;
 inline int NTERM = 55;
@@ -86,19 +113,9 @@ inline exafmm_config config() {
   c.device = options().device;
   return c;
 }
-struct Bound {
-  exafmm_ctx *ctx = nullptr;
-  Bodies *bodies = nullptr;
-};
-// one device context per Cells array, keyed by its first cell
-inline std::map<const Cell *, Bound> &registry() {
-  static std::map<const Cell *, Bound> r;
-  return r;
-}
-inline Bound &bound(const Cells &cells) {
-  auto it = registry().find(cells.data());
-  if (it == registry().end()) throw std::runtime_error("exafmm: Cells not produced by buildTree");
-  return it->second;
+inline exafmm_ctx *bound(const Cells &cells) {
+  if (!cells.device) throw std::runtime_error("exafmm: Cells not produced by buildTree");
+  return cells.device->ctx;
 }
 inline exafmm_ctx *op_ctx() {  // context for the single-call kernels (FP64)
   static exafmm_ctx *ctx = nullptr;
@@ -141,12 +158,15 @@ inline void unflat(const std::vector<double> &o, std::vector<complex_t> &v) {
 inline void initKernel() { NTERM = P * (P + 1) / 2; }
 
 // 3d/build_tree.h:88-96: bodies' X and q are permuted in place; every
-// Cell::BODY points into `bodies`, which must not reallocate afterwards.
+// Cell::BODY points into `bodies`, which must not reallocate and must stay
+// alive until downwardPass has written p and F back into it.
 inline Cells buildTree(Bodies &bodies) {
   initKernel();
-  exafmm_ctx *ctx = nullptr;
+  exafmm_ctx *raw = nullptr;
   exafmm_config c = detail::config();
-  detail::check(exafmm_create(&c, &ctx), nullptr);
+  detail::check(exafmm_create(&c, &raw), nullptr);
+  auto handle = std::make_shared<detail::Handle>(raw);
+  exafmm_ctx *ctx = handle->ctx;
   detail::check(exafmm_build_tree(ctx, reinterpret_cast<exafmm_body *>(bodies.data()), int64_t(bodies.size()), 0),
                 ctx);
   int64_t n = 0;
@@ -164,29 +184,36 @@ inline Cells buildTree(Bodies &bodies) {
     for (int d = 0; d < 3; d++) o.X[d] = flat[i].X[d];
     o.R = flat[i].R;
   }
-  detail::registry()[cells.data()] = detail::Bound{ctx, &bodies};
+  cells.device = std::move(handle);
   return cells;
 }
 
-// 3d/traverse_eager.h:127-144
+// Releases the device context of a tree before the Cells object goes away
+// (not in the reference, whose Cells hold no device resources).
+inline void releaseTree(Cells &cells) { cells.device.reset(); }
+
+// 3d/traverse_eager.h:127-144: M of every cell; L allocated and zeroed.
 inline void upwardPass(Cells &cells) {
-  auto &b = detail::bound(cells);
-  detail::check(exafmm_upward(b.ctx), b.ctx);
-  detail::get_coefs(b.ctx, cells, false);
+  exafmm_ctx *ctx = detail::bound(cells);
+  detail::check(exafmm_upward(ctx), ctx);
+  detail::get_coefs(ctx, cells, false);
   for (auto &c : cells) c.L.assign(NTERM, complex_t(0, 0));
 }
 
-// 3d/traverse_eager.h:147-172 (list form 3d/traverse_lazy.h:27-56)
+// 3d/traverse_eager.h:147-172 (list form 3d/traverse_lazy.h:27-56): M2L
+// into icells' L (read back into Cell::L) and P2P into icells' bodies.
+// Two distinct trees are supported; their M2L runs per pair in FP64.
 inline void horizontalPass(Cells &icells, Cells &jcells) {
-  auto &bi = detail::bound(icells);
-  auto &bj = detail::bound(jcells);
-  detail::check(exafmm_horizontal(bi.ctx, bj.ctx), bi.ctx);
+  exafmm_ctx *ci = detail::bound(icells);
+  exafmm_ctx *cj = detail::bound(jcells);
+  detail::check(exafmm_horizontal(ci, cj), ci);
+  detail::get_coefs(ci, icells, true);
   if (options().fill_lists) {
     for (int which = 0; which < 2; which++) {
       int64_t nm = 0, np = 0;
-      detail::check(exafmm_list_sizes(bi.ctx, &nm, &np), bi.ctx);
+      detail::check(exafmm_list_sizes(ci, &nm, &np), ci);
       std::vector<int64_t> off(icells.size() + 1), src(std::max<int64_t>(which ? np : nm, 1));
-      detail::check(exafmm_get_lists(bi.ctx, which, off.data(), src.data()), bi.ctx);
+      detail::check(exafmm_get_lists(ci, which, off.data(), src.data()), ci);
       for (size_t i = 0; i < icells.size(); i++) {
         auto &l = which ? icells[i].listP2P : icells[i].listM2L;
         l.clear();
@@ -198,9 +225,9 @@ inline void horizontalPass(Cells &icells, Cells &jcells) {
 
 // 3d/traverse_eager.h:175-190: p and F are accumulated into the bodies.
 inline void downwardPass(Cells &cells) {
-  auto &b = detail::bound(cells);
-  detail::check(exafmm_downward(b.ctx), b.ctx);
-  detail::get_coefs(b.ctx, cells, true);
+  exafmm_ctx *ctx = detail::bound(cells);
+  detail::check(exafmm_downward(ctx), ctx);
+  detail::get_coefs(ctx, cells, true);
 }
 
 // 3d/traverse_eager.h:193-202 (FP64 on the device)

@@ -153,6 +153,30 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
+def _bodies(b, n=None, write=True, what="bodies"):
+    """(pointer, n) of a body array: a device pointer (int, with n given) or
+    a numpy array of the 64-byte Body layout, C-contiguous, writeable when
+    the library writes into it, holding at least n bodies.  The library
+    copies n * 64 bytes in and out through the pointer, so a strided view, a
+    different dtype or a short array would corrupt memory: they raise."""
+    if isinstance(b, int):
+        if n is None:
+            raise ValueError(f"{what}: a device pointer needs an explicit count")
+        return C.c_void_p(b), int(n)
+    if not isinstance(b, np.ndarray):
+        raise TypeError(f"{what}: expected a numpy array of BODY_DTYPE or a device pointer")
+    if b.dtype != BODY_DTYPE:
+        raise TypeError(f"{what}: dtype {b.dtype} is not the 64-byte Body layout")
+    if not b.flags["C_CONTIGUOUS"]:
+        raise ValueError(f"{what}: not C-contiguous")
+    if write and not b.flags["WRITEABLE"]:
+        raise ValueError(f"{what}: read-only array")
+    n = len(b) if n is None else int(n)
+    if n < 0 or n > len(b):
+        raise ValueError(f"{what}: n = {n} exceeds the array's {len(b)} bodies")
+    return b.ctypes.data_as(C.c_void_p), n
+
+
 def default_config(**kw) -> Config:
     cfg = Config()
     load().lib.exafmm_config_default(C.byref(cfg))
@@ -189,9 +213,9 @@ class FMM:
 
     # ---- passes, host numpy arrays or device pointers (int)
     def build_tree(self, bodies, n=None, on_device=False):
-        n = len(bodies) if n is None else n
+        p, n = _bodies(bodies, n)
         self._keep = bodies
-        self._chk(self.L.lib.exafmm_build_tree(self.h, _ptr(bodies), n, int(on_device)))
+        self._chk(self.L.lib.exafmm_build_tree(self.h, p, n, int(on_device)))
 
     def upward(self):
         self._chk(self.L.lib.exafmm_upward(self.h))
@@ -203,14 +227,14 @@ class FMM:
         self._chk(self.L.lib.exafmm_downward(self.h))
 
     def evaluate(self, bodies, n=None, on_device=False):
-        n = len(bodies) if n is None else n
+        p, n = _bodies(bodies, n)
         self._keep = bodies
-        self._chk(self.L.lib.exafmm_evaluate(self.h, _ptr(bodies), n, int(on_device)))
+        self._chk(self.L.lib.exafmm_evaluate(self.h, p, n, int(on_device)))
 
     def direct(self, targets, sources, nt=None, ns=None, on_device=False):
-        nt = len(targets) if nt is None else nt
-        ns = len(sources) if ns is None else ns
-        self._chk(self.L.lib.exafmm_direct(self.h, _ptr(targets), nt, _ptr(sources), ns, int(on_device)))
+        pt, nt = _bodies(targets, nt, what="targets")
+        ps, ns = _bodies(sources, ns, write=False, what="sources")
+        self._chk(self.L.lib.exafmm_direct(self.h, pt, nt, ps, ns, int(on_device)))
 
     def evaluate_batch(self, problems, reuse_geometry=False):
         """Evaluate many problems [(bodies, P) or (device pointer, n, P)] with
@@ -223,8 +247,9 @@ class FMM:
         for k, pr in enumerate(problems):
             if len(pr) == 2:
                 b, P = pr
+                p, n = _bodies(b, what=f"problem {k}")
                 keep.append(b)
-                arr[k] = Problem(_ptr(b).value, len(b), P, 0)
+                arr[k] = Problem(p.value, n, P, 0)
             else:
                 ptr, n, P = pr
                 arr[k] = Problem(ptr, n, P, 1)
@@ -236,10 +261,9 @@ class FMM:
         self._chk(self.L.lib.exafmm_dipole_correction(self.h))
 
     def ewald(self, targets, sources, ksize, alpha, sigma, cutoff, nt=None, ns=None, on_device=False):
-        nt = len(targets) if nt is None else nt
-        ns = len(sources) if ns is None else ns
-        self._chk(self.L.lib.exafmm_ewald(self.h, _ptr(targets), nt, _ptr(sources), ns, ksize, alpha, sigma,
-                                          cutoff, int(on_device)))
+        pt, nt = _bodies(targets, nt, what="targets")
+        ps, ns = _bodies(sources, ns, write=False, what="sources")
+        self._chk(self.L.lib.exafmm_ewald(self.h, pt, nt, ps, ns, ksize, alpha, sigma, cutoff, int(on_device)))
 
     def synchronize(self):
         self._chk(self.L.lib.exafmm_synchronize(self.h))
@@ -342,12 +366,15 @@ class FMM:
 
     # ---- kernel.h operator surface (host arrays)
     def p2p(self, targets, sources):
-        self._chk(self.L.lib.exafmm_p2p(self.h, _ptr(targets), len(targets), _ptr(sources), len(sources)))
+        pt, nt = _bodies(targets, what="targets")
+        ps, ns = _bodies(sources, write=False, what="sources")
+        self._chk(self.L.lib.exafmm_p2p(self.h, pt, nt, ps, ns))
 
     def p2m(self, center, bodies, M=None):
         M = np.zeros(self.nterm, np.complex128) if M is None else np.array(M, np.complex128)
         c = np.ascontiguousarray(center, np.float64)
-        self._chk(self.L.lib.exafmm_p2m(self.h, _ptr(c), _ptr(bodies), len(bodies), _ptr(M)))
+        pb, nb = _bodies(bodies, write=False)
+        self._chk(self.L.lib.exafmm_p2m(self.h, _ptr(c), pb, nb, _ptr(M)))
         return M
 
     def m2m(self, cparent, cchild, Mchild, Mparent=None):
@@ -374,7 +401,8 @@ class FMM:
     def l2p(self, center, L, bodies):
         c = np.ascontiguousarray(center, np.float64)
         Lc = np.ascontiguousarray(L, np.complex128)
-        self._chk(self.L.lib.exafmm_l2p(self.h, _ptr(c), _ptr(Lc), _ptr(bodies), len(bodies)))
+        pb, nb = _bodies(bodies)
+        self._chk(self.L.lib.exafmm_l2p(self.h, _ptr(c), _ptr(Lc), pb, nb))
 
 
 class Graph:

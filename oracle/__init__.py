@@ -256,6 +256,10 @@ class Reference:
         L.ref3d_fmm.restype = _I
         L.ref3d_fmm.argtypes = [_P, _L, _P, _P]
         L.ref3d_get_L.argtypes = [_P]
+        L.ref3d_fmm_two.argtypes = [_P, _L, _P, _L, _P]
+        L.ref3d_get_lists_two.argtypes = [_I, _P, _P]
+        L.ref3d_num_cells.restype = _L
+        L.ref3d_num_cells.argtypes = [_I]
         L.ref3d_direct.argtypes = [_P, _L, _P, _L]
         L.ref3d_p2p.argtypes = [_P, _L, _P, _L]
         L.ref3d_p2m.argtypes = [_P, _P, _L, _P]
@@ -305,6 +309,22 @@ class Reference:
         n = self.lib.ref3d_fmm(_ptr(bodies), len(bodies), _ptr(times), None)
         self.ncells = n
         return times
+
+    def fmm_two(self, targets, sources):
+        """Two-tree evaluation (horizontalPass(icells, jcells)); both arrays
+        are permuted in place, the targets receive p and F.  Returns the
+        (offsets, sources) lists of M2L and P2P, sources numbered in the
+        source tree."""
+        tot = np.zeros(2, np.int64)
+        self.lib.ref3d_fmm_two(_ptr(targets), len(targets), _ptr(sources), len(sources), _ptr(tot))
+        nc = int(self.lib.ref3d_num_cells(0))
+        res = []
+        for w in (0, 1):
+            off = np.zeros(nc + 1, np.int64)
+            src = np.zeros(max(int(tot[w]), 1), np.int64)
+            self.lib.ref3d_get_lists_two(w, _ptr(off), _ptr(src))
+            res.append((off, src[: int(tot[w])]))
+        return res
 
     def L(self):
         nt = self.P * (self.P + 1) // 2

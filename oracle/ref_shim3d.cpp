@@ -24,6 +24,8 @@ using namespace exafmm;
 namespace {
 Bodies g_bodies;
 Cells g_cells;
+Bodies g_jbodies;  // source set of the two-tree evaluation
+Cells g_jcells;
 
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -132,6 +134,46 @@ API int ref3d_fmm(Body *b, int64_t n, double *times, double *coefs_out_M) {
   }
   std::memcpy(b, g_bodies.data(), sizeof(Body) * n);
   return int(g_cells.size());
+}
+
+// Two-tree evaluation: buildTree of the targets and of the sources,
+// upwardPass of both, horizontalPass(icells, jcells) in its lazy form
+// (getList(&icells[0], &jcells[0]) then evaluate, 3d/traverse_lazy.h:59-62)
+// and downwardPass of the targets.  Both body arrays are written back
+// (permuted; targets with p and F).  totals[0..1] = M2L / P2P entries.
+API void ref3d_fmm_two(Body *t, int64_t nt, Body *s, int64_t ns, int64_t *totals) {
+  load(g_bodies, t, nt);
+  load(g_jbodies, s, ns);
+  g_cells = buildTree(g_bodies);
+  g_jcells = buildTree(g_jbodies);
+  upwardPass(g_cells);
+  upwardPass(g_jcells);
+  getList(&g_cells[0], &g_jcells[0]);
+  evaluate(g_cells);
+  downwardPass(g_cells);
+  int64_t a = 0, b = 0;
+  for (auto &c : g_cells) {
+    a += c.listM2L.size();
+    b += c.listP2P.size();
+  }
+  totals[0] = a;
+  totals[1] = b;
+  std::memcpy(t, g_bodies.data(), sizeof(Body) * nt);
+  std::memcpy(s, g_jbodies.data(), sizeof(Body) * ns);
+}
+
+API int64_t ref3d_num_cells(int source_tree) { return int64_t(source_tree ? g_jcells.size() : g_cells.size()); }
+
+// Lists of the two-tree evaluation: CSR by target cell, sources numbered in
+// the source tree.
+API void ref3d_get_lists_two(int which, int64_t *offsets, int64_t *sources) {
+  int64_t o = 0;
+  for (size_t i = 0; i < g_cells.size(); i++) {
+    offsets[i] = o;
+    const auto &l = which == 0 ? g_cells[i].listM2L : g_cells[i].listP2P;
+    for (auto *c : l) sources[o++] = int64_t(c - &g_jcells[0]);
+  }
+  offsets[g_cells.size()] = o;
 }
 
 // L coefficients of the last full evaluation.

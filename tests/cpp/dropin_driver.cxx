@@ -53,6 +53,42 @@ int main(int argc, char **argv) {
     }
     std::printf("cells %zu  M[0][0] %.6e  err_p %.6e  err_F %.6e\n", cells.size(), cells[0].M[0].real(),
                 std::sqrt(pd / pn), std::sqrt(fd / fn));
+    // A time-stepping driver rebuilds its tree every step: each step's Cells
+    // releases its device context, and Cell::L holds the M2L sums as soon as
+    // horizontalPass returns (the reference's M2L writes Ci->L there).
+    double lsum = 0;
+    for (int step = 0; step < 4; step++) {
+      Cells c2 = buildTree(bodies);
+      upwardPass(c2);
+      horizontalPass(c2, c2);
+      for (const Cell &c : c2)
+        for (const auto &v : c.L) lsum += std::abs(v);
+      downwardPass(c2);
+    }
+    std::printf("live_contexts %d  L_after_horizontal %.6e\n", int(detail::live_contexts()), lsum);
+    // Two trees: the first half of the bodies as targets of all the bodies.
+    Bodies ib(bodies.begin(), bodies.begin() + n / 2), jb = bodies;
+    for (auto &b : ib) {
+      b.p = 0;
+      b.F[0] = b.F[1] = b.F[2] = 0;
+    }
+    Cells ic = buildTree(ib), jc = buildTree(jb);
+    upwardPass(ic);
+    upwardPass(jc);
+    horizontalPass(ic, jc);
+    downwardPass(ic);
+    Bodies it(ib.begin(), ib.begin() + 50), id = it;
+    for (auto &b : id) {
+      b.p = 0;
+      b.F[0] = b.F[1] = b.F[2] = 0;
+    }
+    direct(id, jb);
+    double qd = 0, qn = 0;
+    for (int i = 0; i < 50; i++) {
+      qd += (id[i].p - it[i].p) * (id[i].p - it[i].p);
+      qn += id[i].p * id[i].p;
+    }
+    std::printf("two_trees_err_p %.6e\n", std::sqrt(qd / qn));
   } catch (const std::exception &e) {
     std::printf("error: %s\n", e.what());
     return 3;

@@ -96,6 +96,53 @@ def periodic_high_order(refp):
     save("periodic_n1000_p18.npz", p=b2["p"], F=b2["F"])
 
 
+def config5_bodies(n: int) -> np.ndarray:
+    """Config 5's bodies: uniform in the unit periodic cube (drand48 seed 0,
+    the reference's draw order), charges +1 / -1 alternating by index."""
+    b = inputs.uniform_cube(n, 0, "sym")
+    b["q"] = np.where(np.arange(n) % 2 == 0, 1.0, -1.0)
+    return b
+
+
+def config5(refp, n=4096):
+    """Config 5's parameters at a reduced N: P = 10, theta = 0.4, ncrit =
+    64, images = 3, cycle = 1; the reference's periodic FMM (as written,
+    3dp/fmm.cxx:46-61) and its Ewald sum with alpha = 10, ksize = 11,
+    sigma = 0.25/pi, cutoff = 0.5 (3dp/fmm.cxx:77-94) on the tree-ordered
+    bodies."""
+    b = config5_bodies(n)
+    refp.fmm(b, 10, 64, 0.4, 3, 1.0)
+    ew = b.copy()
+    refp.ewald(ew, 64, 11, 10.0, 0.25 / np.pi, 0.5, 1.0)
+    save(f"config5_n{n}.npz", n=n, P=10, ncrit=64, theta=0.4, images=3, cycle=1.0, ksize=11, alpha=10.0,
+         sigma=0.25 / np.pi, cutoff=0.5, fmm_p=b["p"], fmm_F=b["F"], ewald_p=ew["p"], ewald_F=ew["F"])
+
+
+def two_trees(ref):
+    """horizontalPass(icells, jcells) with two distinct trees: 3,000 target
+    bodies uniform in [0,1)^3 and 5,000 source bodies uniform in
+    [-0.1,1.1)^3 (different root boxes), q = U[-1,1), P = 8, theta 0.4,
+    ncrit 32.  The reference's two cell tables, both permutations, the
+    two-tree lists and the targets' p and F."""
+    P, ncrit, theta = 8, 32, 0.4
+    t = inputs.uniform_cube(3000, 11, "sym")
+    s = inputs.uniform_cube(5000, 12, "sym")
+    s["X"] = s["X"] * 1.2 - 0.1
+    ref.set(P, ncrit, theta)
+    tcells = ref.build_tree(t.copy())
+    scells = ref.build_tree(s.copy())
+    t2, s2 = t.copy(), s.copy()
+    (mo, ms), (po, ps) = ref.fmm_two(t2, s2)
+    out = dict(targets=t, sources=s, P=P, ncrit=ncrit, theta=theta, sorted_targets_X=t2["X"],
+               sorted_sources_X=s2["X"], p=t2["p"], F=t2["F"],
+               m2l_off=mo.astype(np.int32), m2l_src=ms.astype(np.int32),
+               p2p_off=po.astype(np.int32), p2p_src=ps.astype(np.int32))
+    for name, c in (("tcells", tcells), ("scells", scells)):
+        for f in ("nchild", "nbody", "child", "body", "X", "R"):
+            out[f"{name}_{f}"] = c[f]
+    save("two_trees_n3000_n5000.npz", **out)
+
+
 def main():
     oracle.build()
     ref = oracle.Reference()
@@ -105,6 +152,10 @@ def main():
             config2(ref)
         if "periodic_p18" in only:
             periodic_high_order(oracle.ReferencePeriodic())
+        if "two_trees" in only:
+            two_trees(ref)
+        if "config5" in only:
+            config5(oracle.ReferencePeriodic())
         return
     rng = np.random.default_rng(20261202)
 
@@ -203,6 +254,8 @@ def main():
     refp.ewald(ew, 64, 11, 11 / (2 * np.pi), 0.25 / np.pi, np.pi, 2 * np.pi)
     save("periodic_n1000.npz", input=drv, fmm_sorted=b2, ewald=ew)
     periodic_high_order(refp)
+    config5(refp)
+    two_trees(ref)
     config2(ref)
 
 

@@ -43,3 +43,9 @@ def test_dropin_driver_on_gpu(tmp_path, gpu):
     err_F = float(parts[parts.index("err_F") + 1])
     # full truncation, FP32 P2P, BF16x3 M2L on the driver set of 3d/fmm.cxx
     assert err_p < 1e-5 and err_F < 1e-4
+    # four rebuilds: only the outer tree's context is alive afterwards; the
+    # locals were filled by horizontalPass
+    assert int(parts[parts.index("live_contexts") + 1]) == 1
+    assert float(parts[parts.index("L_after_horizontal") + 1]) > 0
+    # two trees (targets = half of the bodies, sources = all of them)
+    assert float(parts[parts.index("two_trees_err_p") + 1]) < 1e-5

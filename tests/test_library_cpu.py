@@ -60,3 +60,28 @@ def test_bad_config_is_rejected():
         code = lib.lib.exafmm_create(C.byref(cfg), C.byref(h))
         assert code in (-1, -7), bad
         assert not h.value
+
+
+def test_body_arrays_are_validated():
+    """The library copies n * 64 bytes through a body pointer: arrays that
+    are strided, of another dtype, read-only or shorter than n raise before
+    any call."""
+    import numpy as np
+    from minimal_amd.inputs import BODY_DTYPE
+    b = np.zeros(10, BODY_DTYPE)
+    p, n = fmm._bodies(b)
+    assert n == 10 and p.value == b.ctypes.data
+    with pytest.raises(ValueError):
+        fmm._bodies(b[::2])
+    with pytest.raises(TypeError):
+        fmm._bodies(np.zeros((10, 8)))
+    with pytest.raises(ValueError):
+        fmm._bodies(b, n=11)
+    ro = b.copy()
+    ro.flags.writeable = False
+    with pytest.raises(ValueError):
+        fmm._bodies(ro)
+    assert fmm._bodies(ro, write=False)[1] == 10
+    with pytest.raises(ValueError):
+        fmm._bodies(12345)  # a device pointer needs a count
+    assert fmm._bodies(12345, n=4)[1] == 4
