@@ -53,37 +53,86 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, P: int, ncrit: int, theta: float, threads: int) -> dict | None:
-    """The reference's own CPU evaluation (oracle/_ref, built from the
-    reference sources) -- or the FP64 oracle port when that is absent --
-    timed on this host in a child process with a fixed thread count."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _cpu_run(n: int, P: int, ncrit: int, theta: float, threads: int, reps: int, ntargets: int) -> dict | None:
+    """One child process: `reps` evaluations of the same N-body set on
+    `threads` OpenMP threads by the reference's own 3d/ code (oracle/_ref,
+    built from the reference sources; the FP64 oracle port when it is
+    absent).  Returns the per-run stage times and the sampled targets of the
+    last run (tree order) for the error check."""
     code = f"""
 import json, sys, time
 sys.path.insert(0, {ROOT!r})
 import numpy as np, oracle
-from minimal_amd.inputs import uniform_cube
-b = uniform_cube({n}, 0, "sym")
-if oracle.reference_available():
-    r = oracle.Reference(); r.set({P}, {ncrit}, {theta}); t = r.fmm(b); kind = "reference"
-else:
-    o = oracle.Oracle(); t = o.fmm(b, {P}, {ncrit}, {theta}, full=False); kind = "port"
-print(json.dumps({{"kind": kind, "times": [float(x) for x in t]}}))
+from minimal_amd.inputs import uniform_cube, sample_targets
+b0 = uniform_cube({n}, 0, "sym")
+runs = []
+for _ in range({reps}):
+    b = b0.copy()
+    if oracle.reference_available():
+        r = oracle.Reference(); r.set({P}, {ncrit}, {theta}); t = r.fmm(b); kind = "reference"
+    else:
+        o = oracle.Oracle(); t = o.fmm(b, {P}, {ncrit}, {theta}, full=False); kind = "port"
+    runs.append([float(x) for x in t])
+idx = sample_targets({n}, {ntargets})
+tg = b[idx]
+print(json.dumps({{"kind": kind, "runs": runs, "X": tg["X"].tolist(), "q": tg["q"].tolist(),
+                  "p": tg["p"].tolist(), "F": tg["F"].tolist()}}))
 """
     env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
     try:
-        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
+        r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=900)
     except subprocess.TimeoutExpired:
         return None
     if r.returncode != 0:
         log("cpu baseline failed:", r.stderr[-2000:])
         return None
-    d = json.loads(r.stdout.strip().splitlines()[-1])
-    t_eval = sum(d["times"])
-    return {"value": n / t_eval, "unit": "particles/s", "cores": threads, "kind": d["kind"],
-            "sample": f"N={n} uniform [0,1)^3, q~U[-1,1), P={P}, theta={theta}, ncrit={ncrit}: one evaluation "
-                      f"(tree+upward+lists+M2L/P2P+downward) in {t_eval:.3f} s "
-                      f"({'reference 3d/ lazy traversal, FP64, as written' if d['kind'] == 'reference' else 'FP64 oracle port'})",
-            "stage_s": d["times"]}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(n: int, P: int, ncrit: int, theta: float, threads: int, direct_error=None) -> dict | None:
+    """The reference's CPU evaluation timed on this host, as planned for the
+    baseline: the reference's 3d/ lazy path as written (FP64, M2L truncated
+    at degree j+n <= P-1), the median of 3 evaluations of the full workload
+    on `threads` threads (the host's CPU share), and one evaluation on one
+    thread of an N/8 sample (a bounded single-core figure).  The time is the
+    same interval as the device's: tree + upward + lists + M2L/P2P +
+    downward.  direct_error(X, q, p, F) -> (err_p, err_F) checks the
+    sampled targets against the FP64 direct sum."""
+    d = _cpu_run(n, P, ncrit, theta, threads, 3, 1000)
+    if d is None:
+        return None
+    evals = sorted(sum(t) for t in d["runs"])
+    med = evals[len(evals) // 2]
+    stage = d["runs"][[sum(t) for t in d["runs"]].index(med)]
+    n1 = n // 8
+    d1 = _cpu_run(n1, P, ncrit, theta, 1, 1, 10)
+    one = None
+    if d1 is not None:
+        t1 = sum(d1["runs"][0])
+        one = {"value": n1 / t1, "unit": "particles/s", "cores": 1,
+               "sample": f"N={n1} (N/8) of the same distribution, one evaluation on one thread: {t1:.3f} s"}
+    err = None
+    if direct_error is not None:
+        ep, eF = direct_error(np.array(d["X"]), np.array(d["q"]), np.array(d["p"]), np.array(d["F"]))
+        err = {"p": ep, "F": eF, "targets": len(d["q"]), "vs": "FP64 direct sum"}
+    what = ("reference 3d/ lazy traversal, FP64, M2L truncated at j+n <= P-1 as written"
+            if d["kind"] == "reference" else "FP64 oracle port, M2L truncated at j+n <= P-1 as the reference")
+    return {"value": n / med, "unit": "particles/s", "cores": threads, "kind": d["kind"],
+            "sample": f"N={n} uniform [0,1)^3, q~U[-1,1), P={P}, theta={theta}, ncrit={ncrit}: median of 3 "
+                      f"evaluations (tree+upward+lists+M2L/P2P+downward) on {threads} threads, {med:.3f} s ({what})",
+            "runs_s": evals, "stage_s": stage, "m2l_truncation": "reference (as written)", "error": err,
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(), "one_thread": one}
 
 
 def upload(dev, host, torch) -> None:
@@ -144,6 +193,18 @@ def accuracy(f, dev, host, n, ntargets, torch, fmm_mod):
     f.direct(tdev.data_ptr(), ptr, nt=len(tg), ns=n, on_device=True)
     ref = tdev.cpu().numpy().view(BODY_DTYPE)
     return rel_l2(res["p"][idx], ref["p"]), rel_l2(res["F"][idx], ref["F"]), len(idx)
+
+
+def direct_error(f, ptr, n, X, q, p, F, torch):
+    """Relative L2 of given target values against the device FP64 direct
+    sum over the n bodies at ptr (targets X, q; values p, F)."""
+    from minimal_amd.inputs import BODY_DTYPE, rel_l2
+    tg = np.zeros(len(q), BODY_DTYPE)
+    tg["X"], tg["q"] = X, q
+    tdev = to_device(tg, torch)
+    f.direct(tdev.data_ptr(), ptr, nt=len(tg), ns=n, on_device=True)
+    ref = tdev.cpu().numpy().view(BODY_DTYPE)
+    return rel_l2(p, ref["p"]), rel_l2(F, ref["F"])
 
 
 def traffic_for(workload: str, kernel: str):
@@ -281,7 +342,8 @@ def plain_workload(args, fmm, torch):
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(n, args.P, args.ncrit, args.theta, threads)
+        cpu = cpu_baseline(n, args.P, args.ncrit, args.theta, threads,
+                           lambda X, q, p, F: direct_error(f, ptr, n, X, q, p, F, torch))
     out = {
         "metric": METRIC, "value": n / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,

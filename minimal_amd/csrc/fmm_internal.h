@@ -241,6 +241,7 @@ void traverse(Ctx &tgt, const Ctx &src);
 void upward(Ctx &c);
 void m2l_apply(Ctx &tgt, const Ctx &src);
 void m2l_prepare(Ctx &c, hipStream_t s);
+int m2l_fp64_levels(const Ctx &c);  // levels whose M2L runs per pair in FP64 (m2l.hip)
 // learned order (order.hip)
 constexpr int kMlpIn = 4, kMlpH = 32, kMlpOut = 11, kMlpPmin = 4;
 constexpr int kMlpParams = kMlpH * kMlpIn + kMlpH + kMlpH * kMlpH + kMlpH + kMlpOut * kMlpH + kMlpOut;

@@ -13,6 +13,8 @@
 // Truncation: "reference" keeps the terms the reference evaluates (its
 // harmonic table is zero above degree P-1, so j+n <= P-1); "full" keeps
 // every j+n <= 2P-2.
+#include <algorithm>
+
 #include "fmm_internal.h"
 #include "fmm_ops.h"
 
@@ -76,18 +78,31 @@ __global__ void __launch_bounds__(64) k_m2l_fp64(int P, int NT, int fillP, int n
 
 void m2l_fast(Ctx &tc, const Ctx &sc);  // m2l_mfma.hip
 
+// Levels whose M2L runs per pair in FP64 beside the keyed GEMM.  Periodic:
+// levels 0..2, whose cells collect hundreds of image translations that add
+// up to a small difference of large terms (the neutral periodic potential),
+// where FP32 accumulation would leave an offset of ~1e-5 relative.
+int m2l_fp64_levels(const Ctx &c) { return c.cfg.images > 0 ? std::min(3, c.t.depth + 1) : 0; }
+
 void m2l_apply(Ctx &tc, const Ctx &sc) {
   const int P = tc.P, NT = tc.NT;
   const int fillP = tc.cfg.m2l_truncation == EXAFMM_M2L_FULL ? 2 * P - 1 : P;
   Tree &t = tc.t;
+  const size_t lds = sizeof(double2) * (fillP * (fillP + 1) / 2 + NT);
+  const double cyc = tc.cfg.images > 0 ? tc.cfg.cycle : 0.0;
   if (tc.cfg.m2l_precision != EXAFMM_PREC_FP64 && tc.lists.nkeys > 0) {
+    const int lf = m2l_fp64_levels(tc);
+    if (lf > 0) {  // cells of levels < lf come first in breadth-first order
+      const int nc = t.lstart[lf];
+      hipLaunchKernelGGL(k_m2l_fp64, dim3(nc), dim3(64), lds, tc.s0, P, NT, fillP, nc, t.cx.p, sc.t.cx.p,
+                         tc.lists.m2l_beg.p, tc.lists.m2l_cnt.p, tc.lists.m2l.p, sc.M.p, tc.L.p, cyc);
+      XF_LAUNCH();
+    }
     m2l_fast(tc, sc);
     return;
   }
-  const size_t lds = sizeof(double2) * (fillP * (fillP + 1) / 2 + NT);
   hipLaunchKernelGGL(k_m2l_fp64, dim3(t.ncells), dim3(64), lds, tc.s0, P, NT, fillP, int(t.ncells), t.cx.p,
-                     sc.t.cx.p, tc.lists.m2l_beg.p, tc.lists.m2l_cnt.p, tc.lists.m2l.p, sc.M.p, tc.L.p,
-                     tc.cfg.images > 0 ? tc.cfg.cycle : 0.0);
+                     sc.t.cx.p, tc.lists.m2l_beg.p, tc.lists.m2l_cnt.p, tc.lists.m2l.p, sc.M.p, tc.L.p, cyc);
   XF_LAUNCH();
 }
 

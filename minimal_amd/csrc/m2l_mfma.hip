@@ -71,13 +71,14 @@ __device__ inline unsigned mix32(unsigned h) {
 __global__ void __launch_bounds__(256) k_group_keys(int ncells, const double4 *__restrict__ cx,
                                                     const int *__restrict__ cparent, const int *__restrict__ clevel,
                                                     const int *__restrict__ m2l_beg, const int *__restrict__ m2l_cnt,
-                                                    const int2 *__restrict__ list, int minhash,
+                                                    const int2 *__restrict__ list, int minhash, int lmin,
                                                     unsigned long long *__restrict__ key, int *__restrict__ val) {
   const int lane = threadIdx.x & (kWave - 1);
   const int c = blockIdx.x * (256 / kWave) + int(threadIdx.x / kWave);
   if (c >= ncells) return;  // whole wave
   unsigned g = 255, mh = 0;
-  const int n = m2l_cnt[c];
+  // cells above level lmin take their M2L elsewhere (FP64, per pair)
+  const int n = clevel[c] < lmin ? 0 : m2l_cnt[c];
   if (n > 0) {
     int o = 0;
     const int p = cparent[c];
@@ -1017,8 +1018,8 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   // where BFS neighbours already share key sets, it changes little.
   const int minhash = 1;
   hipLaunchKernelGGL(k_group_keys, dim3(ceil_div(nc, 256 / kWave)), dim3(256), 0, s, nc, c.t.cx.p, c.t.cparent.p,
-                     c.t.clevel.p, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, minhash, c.t.key.p,
-                     c.t.val.p);
+                     c.t.clevel.p, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, minhash,
+                     m2l_fp64_levels(c), c.t.key.p, c.t.val.p);
   XF_LAUNCH();
   unsigned long long *k1 = c.t.key.p, *k2 = c.t.key2.p;
   int *v1 = c.t.val.p, *v2 = c.t.val2.p;

@@ -286,6 +286,22 @@ __global__ void __launch_bounds__(256) k_ewald_target(int nw, const Wave *__rest
   }
 }
 
+static void ewald_launch(Ctx &c, Wave *w, int nw, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns,
+                         double alpha, double sigma, double cutoff, double cyc) {
+  const double scale = 2 * M_PI / cyc;
+  const double coef = 2 / sigma / cyc / cyc / cyc;
+  const double coef2 = 1 / (4 * alpha * alpha);
+  if (nw) {
+    hipLaunchKernelGGL(k_ewald_dft, dim3(nw), dim3(256), 0, c.s0, nw, w, s, ns, scale, coef, coef2);
+    XF_LAUNCH();
+  }
+  if (nt) {
+    hipLaunchKernelGGL(k_ewald_target, dim3(nt), dim3(256), 0, c.s0, nw, w, t, nt, s, ns, scale, alpha, cutoff, cyc);
+    XF_LAUNCH();
+  }
+  XF_HIP(hipStreamSynchronize(c.s0));
+}
+
 void ewald_sum(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns, int ksize, double alpha,
                double sigma, double cutoff, double cyc) {
   std::vector<Wave> w;
@@ -302,16 +318,7 @@ void ewald_sum(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t
   Buf<Wave> dw;
   dw.ensure(std::max(nw, 1));
   if (nw) XF_HIP(hipMemcpyAsync(dw.p, w.data(), sizeof(Wave) * nw, hipMemcpyHostToDevice, c.s0));
-  const double scale = 2 * M_PI / cyc;
-  const double coef = 2 / sigma / cyc / cyc / cyc;
-  const double coef2 = 1 / (4 * alpha * alpha);
-  if (nw) {
-    hipLaunchKernelGGL(k_ewald_dft, dim3(nw), dim3(256), 0, c.s0, nw, dw.p, s, ns, scale, coef, coef2);
-    XF_LAUNCH();
-  }
-  hipLaunchKernelGGL(k_ewald_target, dim3(nt), dim3(256), 0, c.s0, nw, dw.p, t, nt, s, ns, scale, alpha, cutoff, cyc);
-  XF_LAUNCH();
-  XF_HIP(hipStreamSynchronize(c.s0));
+  ewald_launch(c, dw.p, nw, t, nt, s, ns, alpha, sigma, cutoff, cyc);
 }
 
 }  // namespace xf
