@@ -22,6 +22,7 @@
 // v_mfma_f32_16x16x32_bf16 (FP32 accumulate), ~16 significant bits per
 // product.
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "fmm_internal.h"
@@ -39,6 +40,13 @@ struct Geom {
   int K4;        // FP32 k-steps of 4
   int K32;       // BF16 k-steps of 32
 };
+// BF16 multipole rows: P^2 padded to a multiple of 8, plus a tail of zeros
+// after the zero row that covers the last k-step's k-groups.
+constexpr int kBf16Tail = 32;
+static int bf16_row(const Geom &g) { return (g.PP + 7) / 8 * 8; }
+// elements of each of the two BF16 cast arrays (hi, lo) of a tree of nc cells
+static size_t bf16_cast_elems(const Geom &g, int64_t nc) { return size_t(nc + 1) * bf16_row(g) + kBf16Tail; }
+
 static Geom geom(int P) {
   Geom g;
   g.P = P;
@@ -275,14 +283,16 @@ __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirs
 }
 
 // ---------------------------------------------------------------- operands
-// GEMM operands from the scaled FP64 multipoles M~ (P^2 per cell, padded to KP):
-// FP32 copy, or bf16 hi + lo split (hi = bf16(x), lo = bf16(x - hi)).
-__global__ void __launch_bounds__(256) k_mt(int PP, int KP, int ncells, const double *__restrict__ M,
+// GEMM operands from the scaled FP64 multipoles M~ (P^2 per cell, rows of
+// KP): FP32 copy, or bf16 hi + lo split (hi = bf16(x), lo = bf16(x - hi)).
+// Row ncells is the zero row; `tail` more zeros follow it, so a k-group load
+// of the zero row past KP stays inside the array and reads zeros.
+__global__ void __launch_bounds__(256) k_mt(int PP, int KP, int ncells, int tail, const double *__restrict__ M,
                                             float *__restrict__ mt32, __bf16 *__restrict__ mhi,
                                             __bf16 *__restrict__ mlo) {
   const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
   const int c = int(g / KP), r = int(g % KP);
-  if (c > ncells) return;
+  if (g >= int64_t(ncells + 1) * KP + tail) return;
   // row ncells is all zeros: the GEMM's loads for empty slots read it
   const float v = (c < ncells && r < PP) ? float(M[int64_t(c) * PP + r]) : 0.0f;
   if (mt32) mt32[g] = v;
@@ -374,7 +384,7 @@ __device__ inline void write_partial(int PP, int rbase, int nr, const int *cells
   for (int ct = 0; ct < 4; ct++) {
     const int col = cg * 64 + ct * 16 + (lane & 15);
     if (col >= ncol) continue;
-    float *Lc = Lp + int64_t(cells[col]) * PP;
+    float *Lc = Lp + int64_t(cells ? cells[col] : col) * PP;
 #pragma unroll
     for (int r = 0; r < RH; r++) {
       if (r >= nr) break;
@@ -578,8 +588,12 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
   }
 }
 
+// Keys ub .. ue - 1 of tile bx, row block by.  Output: added into the FP64
+// locals L (Lpart null), or stored as FP32 partial locals, by cell
+// (Lpart + cell * PP, part_local false) or by tile column (Lpart + col * PP).
 template <int RH, bool SKIP>
-__device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, int by, int bz, int nz, int PP, int RT, int RTB, int K32, int KP,
+__device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, int by, int ub, int ue, bool part_local,
+                                       int PP, int RT, int RTB, int K32, int KP,
                                        const int *__restrict__ tfirst,
                                        const int *__restrict__ tcount,
                                        const int *__restrict__ tbase,
@@ -605,9 +619,6 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
   // filled tile skips the MFMAs of its empty subtiles, whose B operands are
   // the zero row)
   const int nct = max(0, min(4, (ncol - cg * 64 + 15) / 16));
-  // keys of this workgroup: slice bz of nz of the tile's keys
-  const int Ut = tU[tile], ub = int(int64_t(Ut) * bz / nz),
-            ue = int(int64_t(Ut) * (bz + 1) / nz);
   const int U = ue - ub, base = tbase[tile] + ub;
   const int rt0 = by * RTB;
   const int nrt = min(RTB, RT - rt0);
@@ -765,8 +776,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
   reduce_parities<RH>(acc, kp, cg, lane, Ab);
   if (kp == 0) {
     if (Lpart)
-      write_partial<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane,
-                        Lpart + int64_t(bz) * ncells * PP);
+      write_partial<RH>(PP, rt0, nrt, part_local ? nullptr : sorted + tfirst[tile], ncol, cg, acc, lane, Lpart);
     else
       write_locals<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane, L);
   }
@@ -788,8 +798,77 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
                                                              const unsigned char *__restrict__ tkp, int kr,
                                                              int64_t ncells, float *__restrict__ Lpart,
                                                              double *__restrict__ L) {
-  m2l_bf16x3_body<RH, false>(blockIdx.x, blockIdx.y, blockIdx.z, gridDim.z, PP, RT, RTB, K32, KP, tfirst, tcount, tbase, tU,
-                      sorted, tkeys, tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells, Lpart, L);
+  // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
+  const int Ut = tU[blockIdx.x], ub = int(int64_t(Ut) * blockIdx.z / gridDim.z),
+            ue = int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
+  m2l_bf16x3_body<RH, false>(blockIdx.x, blockIdx.y, ub, ue, false, PP, RT, RTB, K32, KP, tfirst, tcount, tbase,
+                             tU, sorted, tkeys, tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
+                             Lpart ? Lpart + int64_t(blockIdx.z) * ncells * PP : nullptr, L);
+}
+
+// Stream-K schedule: the tiles' key slots, in tile order (slot tbase[t] + u
+// is key u of tile t; S slots in all), are split evenly over the gridDim.x
+// workgroups of one persistent launch (one per CU).  Workgroup w takes slots
+// [w S / G, (w + 1) S / G); each tile it touches gives one FP32 partial of the
+// tile's locals, stored by tile column in segment w + t of Lpart, and
+// k_sk_combine adds a tile's segments in workgroup order.  Every workgroup
+// gets the same number of slots, so none waits on a straggler.
+template <int RH>
+__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, int K32, int KP, int ntiles,
+                                                                int64_t S, const int *__restrict__ tfirst,
+                                                                const int *__restrict__ tcount,
+                                                                const int *__restrict__ tbase,
+                                                                const int *__restrict__ tU,
+                                                                const int *__restrict__ sorted,
+                                                                const int *__restrict__ tkeys,
+                                                                const int *__restrict__ tmat,
+                                                                const v8bf *__restrict__ ophi,
+                                                                const v8bf *__restrict__ oplo,
+                                                                const __bf16 *__restrict__ mhi,
+                                                                const __bf16 *__restrict__ mlo,
+                                                                const unsigned char *__restrict__ tkp, int kr,
+                                                                int64_t ncells, float *__restrict__ Lseg) {
+  const int64_t G = gridDim.x, w = blockIdx.x;
+  const int64_t s0 = w * S / G, s1 = (w + 1) * S / G;
+  if (s0 >= s1) return;
+  // first tile holding slot s0: the last t with tbase[t] <= s0
+  int lo = 0, hi = ntiles - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (tbase[mid] <= s0)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  for (int t = lo; t < ntiles && tbase[t] < s1; t++) {
+    const int64_t b = tbase[t];
+    const int ub = int(max(s0, b) - b), ue = int(min(s1, b + tU[t]) - b);
+    if (ub >= ue) continue;
+    m2l_bf16x3_body<RH, false>(t, 0, ub, ue, true, PP, RT, RT, K32, KP, tfirst, tcount, tbase, tU, sorted, tkeys,
+                               tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
+                               Lseg + (w + t) * int64_t(kTN) * PP, nullptr);
+    __syncthreads();  // the LDS of this segment is free before the next one stages
+  }
+}
+
+// Sum of a tile's stream-K segments (workgroups wf .. wl in order) into the
+// FP64 locals.  One thread per (tile column, row).
+__global__ void __launch_bounds__(256) k_sk_combine(int PP, int ntiles, int64_t S, int G,
+                                                    const int *__restrict__ tfirst, const int *__restrict__ tcount,
+                                                    const int *__restrict__ tbase, const int *__restrict__ tU,
+                                                    const int *__restrict__ sorted, const float *__restrict__ Lseg,
+                                                    double *__restrict__ L) {
+  const int t = blockIdx.y;
+  const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int col = int(e / PP), row = int(e % PP);
+  if (t >= ntiles || col >= tcount[t]) return;
+  // workgroup of slot s: the largest w with w S / G <= s
+  auto wg_of = [&](int64_t sl) { return int(((sl + 1) * G - 1) / S); };
+  const int64_t b = tbase[t];
+  const int wf = wg_of(b), wl = wg_of(b + tU[t] - 1);
+  double acc = 0.0;
+  for (int w = wf; w <= wl; w++) acc += double(Lseg[(int64_t(w + t) * kTN + col) * PP + row]);
+  L[int64_t(sorted[tfirst[t] + col]) * PP + row] += acc;
 }
 
 // One problem of a grouped keyed-GEMM launch (all problems of one order in
@@ -812,7 +891,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_grouped(const Group
                                                                      int kr) {
   const int b = blockIdx.x, i = group_of(start, nprob, b), lb = b - start[i];
   const GroupGemm &q = g[i];
-  m2l_bf16x3_body<RH, true>(lb % q.ntiles, lb / q.ntiles, 0, 1, PP, RT, RTB, K32, KP, q.tfirst, q.tcount, q.tbase,
+  m2l_bf16x3_body<RH, true>(lb % q.ntiles, lb / q.ntiles, 0, q.tU[lb % q.ntiles], false, PP, RT, RTB, K32, KP,
+                            q.tfirst, q.tcount, q.tbase,
                             q.tU, q.sorted, q.tkeys, q.tmat, q.ophi, q.oplo, q.mhi, q.mlo, nullptr, kr, q.ncells,
                             nullptr, q.L);
 }
@@ -858,7 +938,7 @@ static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool 
     const v8bf *oplo = reinterpret_cast<const v8bf *>(reinterpret_cast<const __bf16 *>(c.ops32.p) +
                                                       per * c.lists.nkeys);
     const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
-    const __bf16 *mlo = mhi + size_t(sc.t.ncells + 1) * KPb;
+    const __bf16 *mlo = mhi + bf16_cast_elems(g, sc.t.ncells);
     const size_t lds = gemm_lds(g, true, RTB, kr);
     XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3<RH>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
@@ -880,10 +960,61 @@ static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool 
 
 struct GemmShape {
   int rtb, kr, ksplit;
+  int sk = 0;  // stream-K: one persistent workgroup per CU over the tiles' key slots (BF16x3)
 };
+
+static int cu_count(int device) {
+  int n = 0;
+  XF_HIP(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device));
+  return std::max(n, 1);
+}
+
+// Stream-K launch: all rows per workgroup (R <= kGemmMaxRT), then the
+// fixed-order combine of each tile's segments.
+template <int RH>
+static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int KPb, int kr, double *Lout) {
+  const int ts = c.m2l_tstride;
+  const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ts, *tbase = c.tiles.p + 2 * ts, *tU = c.tiles.p + 3 * ts;
+  const int *sorted = c.tiles.p + 4 * ts;
+  const int G = cu_count(c.cfg.device);
+  const int64_t S = c.m2l_totalU;
+  c.m2l_part.ensure(size_t(G + ntiles) * kTN * g.PP);
+  const v8bf *ophi = reinterpret_cast<const v8bf *>(c.ops32.p);
+  const size_t per = size_t(g.K32) * g.R * 64 * 8;
+  const v8bf *oplo =
+      reinterpret_cast<const v8bf *>(reinterpret_cast<const __bf16 *>(c.ops32.p) + per * c.lists.nkeys);
+  const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
+  const __bf16 *mlo = mhi + bf16_cast_elems(g, sc.t.ncells);
+  const size_t lds = gemm_lds(g, true, g.R, kr);
+  XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3_sk<RH>),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+  hipLaunchKernelGGL((k_m2l_bf16x3_sk<RH>), dim3(G), dim3(kGemmThreads), lds, c.s0, g.PP, g.R, g.K32, KPb, ntiles, S,
+                     tfirst, tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi, mlo,
+                     c.learned ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), c.m2l_part.p);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_sk_combine, dim3(ceil_div(int64_t(kTN) * g.PP, 256), ntiles), dim3(256), 0, c.s0, g.PP,
+                     ntiles, S, G, tfirst, tcount, tbase, tU, sorted, c.m2l_part.p, Lout);
+  XF_LAUNCH();
+}
 
 static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, GemmShape sh,
                      double *Lout) {
+  if (sh.sk) {
+    switch (g.R) {
+#define XF_GEMM_SK(n) \
+  case n: launch_gemm_sk<n>(c, sc, g, ntiles, KPb, sh.kr, Lout); return;
+      XF_GEMM_SK(1)
+      XF_GEMM_SK(2)
+      XF_GEMM_SK(3)
+      XF_GEMM_SK(4)
+      XF_GEMM_SK(5)
+      XF_GEMM_SK(6)
+      XF_GEMM_SK(7)
+      XF_GEMM_SK(8)
+#undef XF_GEMM_SK
+      default: fail(EXAFMM_ERR_UNSUPPORTED, "stream-K GEMM needs all rows in one workgroup");
+    }
+  }
   float *part = nullptr;
   if (sh.ksplit > 1) {
     c.m2l_part.ensure(size_t(sh.ksplit) * c.t.ncells * g.PP);
@@ -928,7 +1059,8 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
   const long long key = ((long long)g.P << 24) | ((long long)bf16 << 20) | ((long long)c.learned << 16) | tclass;
   auto it = c.gemm_tuned.find(key);
   if (it != c.gemm_tuned.end())
-    return GemmShape{int(it->second & 255), int((it->second >> 8) & 255), int(it->second >> 16)};
+    return GemmShape{int(it->second & 255), int((it->second >> 8) & 255), int((it->second >> 16) & 255),
+                     int(it->second >> 24)};
   std::vector<GemmShape> cand;
   for (int kr = 1; kr <= 3; kr++) {
     const int rmax = rows_per_block(g, bf16, c.cfg.device, kr);
@@ -946,15 +1078,29 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
       last = r;
     }
   }
+  // stream-K (BF16x3, all rows per workgroup)
+  if (bf16 && g.R <= kGemmMaxRT && std::getenv("EXAFMM_GEMM_NO_SK") == nullptr)
+    for (int kr = 1; kr <= 3; kr++)
+      if (rows_per_block(g, bf16, c.cfg.device, kr) >= g.R) cand.push_back(GemmShape{g.R, kr, 1, 1});
   GemmShape best = cand[0];
+  if (std::getenv("EXAFMM_GEMM_SK_ONLY"))  // the stream-K candidates alone (tests, A/B timing)
+    for (const GemmShape &sh : cand)
+      if (sh.sk) {
+        best = sh;
+        break;
+      }
   if (c.capturing) fail(EXAFMM_ERR_STATE, "GEMM shape not tuned before recording a graph");
+  if (std::getenv("EXAFMM_GEMM_SK_ONLY")) cand.erase(std::remove_if(cand.begin(), cand.end(), [](const GemmShape &x) { return !x.sk; }), cand.end());
+  if (cand.empty()) fail(EXAFMM_ERR_UNSUPPORTED, "no GEMM shape");
   if (cand.size() > 1 && c.cfg.autotune) {
     hipStream_t s = c.s0;
     XF_HIP(hipStreamSynchronize(c.s1));  // tune alone on the device
     c.L_tune.ensure(size_t(c.t.ncells) * c.P * c.P);
-    int kmax = 1;
-    for (const GemmShape &sh : cand) kmax = std::max(kmax, sh.ksplit);
-    if (kmax > 1) c.m2l_part.ensure(size_t(kmax) * c.t.ncells * g.PP);  // no allocation inside the timings
+    size_t need = 1;
+    for (const GemmShape &sh : cand)
+      need = std::max(need, sh.sk ? size_t(cu_count(c.cfg.device) + ntiles) * kTN * g.PP
+                                  : size_t(sh.ksplit > 1 ? sh.ksplit : 0) * c.t.ncells * g.PP);
+    c.m2l_part.ensure(need);  // no allocation inside the timings
     hipEvent_t e0, e1;
     XF_HIP(hipEventCreate(&e0));
     XF_HIP(hipEventCreate(&e1));
@@ -976,7 +1122,7 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
     XF_HIP(hipEventDestroy(e0));
     XF_HIP(hipEventDestroy(e1));
   }
-  c.gemm_tuned[key] = best.rtb | (best.kr << 8) | (best.ksplit << 16);
+  c.gemm_tuned[key] = best.rtb | (best.kr << 8) | (best.ksplit << 16) | (best.sk << 24);
   return best;
 }
 
@@ -1111,18 +1257,22 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   c.m2l_ntiles_last = ntiles;
   if (!c.graph_locks) c.m2l_ready = false;  // tiles belong to this horizontal pass (or to the graph)
   if (ntiles == 0) return;
-  const int KP = g.K4 * 4, KPb = g.K32 * 32;
+  // BF16 rows hold P^2 coefficients padded to a multiple of 8 (one k-group
+  // load), not to the MFMA's K of 32 per step: the k-groups past the row
+  // read the zero row, which stays in cache
+  const int KP = g.K4 * 4, KPb = bf16_row(g);
   if (bf16) {
-    c.mt32.ensure((size_t(nc + 1) * KPb * 2 + 1) / 2 + 1);  // two bf16 arrays of nc + 1 rows
+    const size_t rows = bf16_cast_elems(g, nc);  // per part
+    c.mt32.ensure(rows + 1);                     // two bf16 arrays in a float buffer
     __bf16 *mhi = reinterpret_cast<__bf16 *>(c.mt32.p);
-    __bf16 *mlo = mhi + size_t(nc + 1) * KPb;
-    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc + 1) * KPb, 256)), dim3(256), 0, s, c.P * c.P, KPb, nc, c.M.p,
-                       (float *)nullptr, mhi, mlo);
+    __bf16 *mlo = mhi + rows;
+    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(rows), 256)), dim3(256), 0, s, c.P * c.P, KPb, nc, kBf16Tail,
+                       c.M.p, (float *)nullptr, mhi, mlo);
     XF_LAUNCH();
   } else {
     c.mt32.ensure(size_t(nc + 1) * KP);
-    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc + 1) * KP, 256)), dim3(256), 0, s, c.P * c.P, KP, nc, c.M.p,
-                       c.mt32.p, (__bf16 *)nullptr, (__bf16 *)nullptr);
+    hipLaunchKernelGGL(k_mt, dim3(ceil_div(int64_t(nc + 1) * KP, 256)), dim3(256), 0, s, c.P * c.P, KP, nc, 0,
+                       c.M.p, c.mt32.p, (__bf16 *)nullptr, (__bf16 *)nullptr);
     XF_LAUNCH();
   }
 
@@ -1145,6 +1295,7 @@ struct GroupCast {
   __bf16 *mhi, *mlo;
   int64_t ncells;
   int PP, KP;
+  int64_t elems;  // elements of each cast array: rows of KP, the zero row, the zero tail
 };
 
 // BF16 hi/lo cast of the multipoles of every problem of one order (row
@@ -1154,9 +1305,9 @@ __global__ void __launch_bounds__(256) k_mt_grouped(const GroupCast *__restrict_
   const int b = blockIdx.x, i = group_of(start, nprob, b);
   const GroupCast q = g[i];
   const int64_t e = int64_t(b - start[i]) * 256 + threadIdx.x;
+  if (e >= q.elems) return;
   const int64_t c = e / q.KP;
   const int r = int(e % q.KP);
-  if (c > q.ncells) return;
   const float v = (c < q.ncells && r < q.PP) ? float(q.M[c * q.PP + r]) : 0.0f;
   const __bf16 h = __bf16(v);
   q.mhi[e] = h;
@@ -1179,7 +1330,7 @@ static void group_tables(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, i
   if (probs.empty()) return;
   const int P = probs[0]->P;
   const Geom g = geom(P);
-  const int KPb = g.K32 * 32;
+  const int KPb = bf16_row(g);  // the cast layout of m2l_fast: rows of KPb, zero row, zero tail
   plan.P = P;
   plan.rtb = rtb;
   plan.kr = kr;
@@ -1198,6 +1349,8 @@ static void group_tables(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, i
     const int64_t nc = c.t.ncells;
     const size_t per = size_t(g.K32) * g.R * 64 * 8;
     const __bf16 *ophi = reinterpret_cast<const __bf16 *>(c.ops32.p);
+    const size_t elems = bf16_cast_elems(g, nc);
+    c.mt32.ensure(elems + 1);  // two bf16 arrays in a float buffer (set-up time, not under capture)
     __bf16 *mhi = reinterpret_cast<__bf16 *>(c.mt32.p);
     GroupGemm q;
     const int ts = c.m2l_tstride;
@@ -1211,17 +1364,17 @@ static void group_tables(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, i
     q.ophi = reinterpret_cast<const v8bf *>(ophi);
     q.oplo = reinterpret_cast<const v8bf *>(ophi + per * c.lists.nkeys);
     q.mhi = mhi;
-    q.mlo = mhi + size_t(nc + 1) * KPb;
+    q.mlo = mhi + elems;
     q.L = c.L.p;
     q.ncells = nc;
     q.ntiles = nt;
     d.push_back(q);
     st.push_back(int(acc));
     acc += int64_t(nt) * nrb;
-    GroupCast cq{c.M.p, mhi, mhi + size_t(nc + 1) * KPb, nc, g.PP, KPb};
+    GroupCast cq{c.M.p, mhi, mhi + elems, nc, g.PP, KPb, int64_t(elems)};
     cd.push_back(cq);
     cst.push_back(int(cacc));
-    cacc += ceil_div((nc + 1) * KPb, 256);
+    cacc += ceil_div(int64_t(elems), 256);
   }
   if (acc > (int64_t(1) << 31) - 1 || cacc > (int64_t(1) << 31) - 1) fail(EXAFMM_ERR_CAPACITY, "batch too large");
   plan.nprob = int(d.size());
@@ -1305,7 +1458,7 @@ void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s) {
   hipLaunchKernelGGL(k_mt_grouped, dim3(plan.cast_blocks), dim3(256), 0, s, cd, plan.cast_start.p, plan.nprob);
   XF_LAUNCH();
   const GroupGemm *d = reinterpret_cast<const GroupGemm *>(plan.desc.p);
-  const int KPb = g.K32 * 32;
+  const int KPb = bf16_row(g);
   switch (plan.rtb) {
 #define XF_GROUP_CASE(n)                                                                                         \
   case n:                                                                                                         \
