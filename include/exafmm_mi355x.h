@@ -156,6 +156,17 @@ int exafmm_dipole_correction(exafmm_ctx *ctx);
  * each pair's nearest image, self term). */
 int exafmm_ewald(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources, int64_t ns,
                  int ksize, double alpha, double sigma, double cutoff, int on_device);
+/* exafmm_ewald without allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops, harnesses that forbid hipMalloc).  targets and
+ * sources are device pointers; the wave table lives in a fixed array of
+ * 32,768 waves per GPU (ksize <= 25; larger ksize returns
+ * EXAFMM_ERR_CAPACITY), set up by exafmm_create on the context's device.
+ * Results equal exafmm_ewald(..., on_device = 1) bit for bit.  Every call
+ * on one GPU shares that array: calls that overlap on one GPU (several
+ * host threads or contexts at once) race. */
+int exafmm_ewald_noalloc(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources,
+                         int64_t ns, int ksize, double alpha, double sigma, double cutoff);
 
 /* Interaction lists of the last horizontal pass (Cell::listM2L /
  * Cell::listP2P, 3d/exafmm.h:30-33), reference cell numbering, CSR by

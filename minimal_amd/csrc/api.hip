@@ -217,6 +217,7 @@ int exafmm_create(const exafmm_config *cfg, exafmm_ctx **out) {
     }
     if (cfg->device < 0 || cfg->device >= ndev) fail(EXAFMM_ERR_ARG, "device ordinal out of range");
     XF_HIP(hipSetDevice(cfg->device));
+    ewald_scratch_init(cfg->device);  // address of this device's fixed scratch array
     auto *h = new exafmm_ctx();
     Ctx &c = h->c;
     c.cfg = *cfg;
@@ -399,6 +400,17 @@ int exafmm_ewald(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm
       ewald_sum(c, dt.p, nt, ds.p, ns, ksize, alpha, sigma, cutoff, c.cfg.cycle);
       XF_HIP(hipMemcpy(targets, dt.p, sizeof(exafmm_body) * nt, hipMemcpyDeviceToHost));
     }
+  });
+}
+
+int exafmm_ewald_noalloc(exafmm_ctx *ctx, exafmm_body *targets, int64_t nt, const exafmm_body *sources,
+                         int64_t ns, int ksize, double alpha, double sigma, double cutoff) {
+  return guard(ctx, [&] {
+    if (!ctx || !targets || !sources || nt < 0 || ns < 0 || ksize < 0 || !(alpha > 0) || !(sigma > 0))
+      fail(EXAFMM_ERR_ARG, "bad argument");
+    Ctx &c = ctx->c;
+    if (!(c.cfg.cycle > 0)) fail(EXAFMM_ERR_ARG, "Ewald sum needs the periodic cycle in the configuration");
+    ewald_sum_noalloc(c, targets, nt, sources, ns, ksize, alpha, sigma, cutoff, c.cfg.cycle);
   });
 }
 

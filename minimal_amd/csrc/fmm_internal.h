@@ -284,6 +284,12 @@ void periodic_far(Ctx &c);
 void dipole_correction(Ctx &c);
 void ewald_sum(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns, int ksize, double alpha,
                double sigma, double cutoff, double cycle);
+// allocation-free Ewald sum: wave table in a fixed per-device array
+constexpr int kMaxDevices = 64;
+constexpr int kEwaldMaxWaves = 32768;  // 40 B each (1.3 MB): ksize <= 25 (32,633 waves)
+void ewald_scratch_init(int device);
+void ewald_sum_noalloc(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns, int ksize,
+                       double alpha, double sigma, double cutoff, double cycle);
 
 // single-operator entry points (FP64, one call == one reference call)
 void op_p2p(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns);

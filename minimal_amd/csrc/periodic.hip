@@ -17,6 +17,9 @@
 // body pair's nearest image within the cutoff, self term.
 #include <cmath>
 
+#include <mutex>
+#include <string>
+
 #include "fmm_internal.h"
 #include "fmm_ops.h"
 
@@ -286,6 +289,23 @@ __global__ void __launch_bounds__(256) k_ewald_target(int nw, const Wave *__rest
   }
 }
 
+// Wave vectors k in the half space (3dp/ewald.h:54-79): l = 0..kmax,
+// m = -kmax..kmax (0..kmax at l = 0), n = -kmax..kmax (1..kmax at l = m = 0),
+// l^2 + m^2 + n^2 <= kmax^2, in that nested order.
+static int ewald_wave_count(int kmax) {
+  int nw = 0;
+  const int kmaxsq = kmax * kmax;
+  for (int l = 0; l <= kmax; l++) {
+    const int mmin = l == 0 ? 0 : -kmax;
+    for (int m = mmin; m <= kmax; m++) {
+      const int nmin = (l == 0 && m == 0) ? 1 : -kmax;
+      for (int n = nmin; n <= kmax; n++)
+        if (l * l + m * m + n * n <= kmaxsq) nw++;
+    }
+  }
+  return nw;
+}
+
 static void ewald_launch(Ctx &c, Wave *w, int nw, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns,
                          double alpha, double sigma, double cutoff, double cyc) {
   const double scale = 2 * M_PI / cyc;
@@ -319,6 +339,79 @@ void ewald_sum(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t
   dw.ensure(std::max(nw, 1));
   if (nw) XF_HIP(hipMemcpyAsync(dw.p, w.data(), sizeof(Wave) * nw, hipMemcpyHostToDevice, c.s0));
   ewald_launch(c, dw.p, nw, t, nt, s, ns, alpha, sigma, cutoff, cyc);
+}
+
+// ---------------------------------------------------------------- Ewald without allocation
+// The wave table of exafmm_ewald_noalloc lives in this fixed device array
+// (one copy per GPU, kept until the process ends).  Host code never passes
+// the array's name to a kernel: it uses the address hipGetSymbolAddress
+// returned for the current device (ewald_scratch).
+static __device__ __align__(256) Wave scratch[kEwaldMaxWaves];
+
+namespace {
+std::mutex g_scratch_mu;
+Wave *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+void ewald_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) fail(EXAFMM_ERR_ARG, "device ordinal beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  XF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[device] = static_cast<Wave *>(p);
+}
+
+static Wave *ewald_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  Wave *p = (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+  if (!p) fail(EXAFMM_ERR_STATE, "scratch of the device not initialised (no context created on it)");
+  return p;
+}
+
+// The wave table on the device, in ewald_wave_count's order: one
+// workgroup walks the candidate box (l, m, n) 256 at a time and compacts
+// the valid vectors with a ballot prefix.
+__global__ void __launch_bounds__(256) k_ewald_waves(int kmax, Wave *__restrict__ w) {
+  __shared__ int wave_tot[4];
+  __shared__ int base;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int side = 2 * kmax + 1;
+  const int64_t ncand = int64_t(kmax + 1) * side * side;
+  if (threadIdx.x == 0) base = 0;
+  __syncthreads();
+  for (int64_t c0 = 0; c0 < ncand; c0 += 256) {
+    const int64_t cnd = c0 + threadIdx.x;
+    const int l = int(cnd / (side * side)), m = int((cnd / side) % side) - kmax, n = int(cnd % side) - kmax;
+    bool ok = cnd < ncand && l * l + m * m + n * n <= kmax * kmax;
+    if (l == 0 && m < 0) ok = false;
+    if (l == 0 && m == 0 && n < 1) ok = false;
+    const unsigned long long b = __ballot(ok);
+    const int before = __popcll(b & (lane ? (~0ull >> (64 - lane)) : 0ull));
+    if (lane == 0) wave_tot[wv] = __popcll(b);
+    __syncthreads();
+    int off = base;
+    for (int i = 0; i < wv; i++) off += wave_tot[i];
+    if (ok) w[off + before] = Wave{{double(l), double(m), double(n)}, 0.0, 0.0};
+    __syncthreads();
+    if (threadIdx.x == 0) base += wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+    __syncthreads();
+  }
+}
+
+void ewald_sum_noalloc(Ctx &c, exafmm_body *t, int64_t nt, const exafmm_body *s, int64_t ns, int ksize,
+                       double alpha, double sigma, double cutoff, double cyc) {
+  const int nw = ewald_wave_count(ksize);
+  if (nw > kEwaldMaxWaves)
+    fail(EXAFMM_ERR_CAPACITY, "ksize " + std::to_string(ksize) + " needs " + std::to_string(nw) +
+                                  " waves; the allocation-free Ewald sum holds at most " +
+                                  std::to_string(kEwaldMaxWaves));
+  Wave *w = ewald_scratch(c.cfg.device);
+  if (nw) {
+    hipLaunchKernelGGL(k_ewald_waves, dim3(1), dim3(256), 0, c.s0, ksize, w);
+    XF_LAUNCH();
+  }
+  ewald_launch(c, w, nw, t, nt, s, ns, alpha, sigma, cutoff, cyc);
 }
 
 }  // namespace xf

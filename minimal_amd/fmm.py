@@ -38,7 +38,7 @@ EXPORTS = [
     "exafmm_horizontal", "exafmm_downward", "exafmm_evaluate", "exafmm_direct", "exafmm_list_sizes",
     "exafmm_get_lists", "exafmm_get_multipoles", "exafmm_get_locals", "exafmm_stage_times", "exafmm_p2p",
     "exafmm_p2m", "exafmm_m2m", "exafmm_m2l", "exafmm_l2l", "exafmm_l2p", "exafmm_work_counts",
-    "exafmm_dipole_correction", "exafmm_ewald", "exafmm_order_model_train", "exafmm_set_order_model",
+    "exafmm_dipole_correction", "exafmm_ewald", "exafmm_ewald_noalloc", "exafmm_order_model_train", "exafmm_set_order_model",
     "exafmm_get_order_model", "exafmm_order_training_set", "exafmm_mlp_forward", "exafmm_mlp_gradient",
     "exafmm_order_stats", "exafmm_graph_create", "exafmm_graph_create_grouped", "exafmm_graph_launch",
     "exafmm_graph_synchronize", "exafmm_graph_nodes", "exafmm_graph_gemm_shapes", "exafmm_graph_destroy",
@@ -104,6 +104,7 @@ class Library:
         L.exafmm_work_counts.argtypes = [_P, _P, _I]
         L.exafmm_dipole_correction.argtypes = [_P]
         L.exafmm_ewald.argtypes = [_P, _P, _L, _P, _L, _I, _D, _D, _D, _I]
+        L.exafmm_ewald_noalloc.argtypes = [_P, _P, _L, _P, _L, _I, _D, _D, _D]
         L.exafmm_p2p.argtypes = [_P, _P, _L, _P, _L]
         L.exafmm_p2m.argtypes = [_P, _P, _P, _L, _P]
         L.exafmm_m2m.argtypes = [_P, _P, _P, _P, _P]
