@@ -211,6 +211,9 @@ struct Ctx {
   std::map<long long, int> gemm_tuned;  // autotuned GEMM shape (rows, keys per round, slices) per class
   Buf<double> L_tune;                   // scratch locals of the autotuning runs
   Buf<unsigned char> tile_p;
+  // BF16x3 learned order: each tile's slots grouped by order (tile_perm:
+  // permuted slot -> slot; tile_cls: [first | count] x order x tile stride)
+  Buf<int> tile_perm, tile_cls;
   // learned order model (order.hip): FP32 master weights, tolerance
   bool have_model = false;
   float order_log_tol = -5.0f;

@@ -282,6 +282,54 @@ __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirs
   }
 }
 
+// Learned orders: each tile's slots grouped by order (ascending order,
+// key order within an order), so that one GEMM launch per order covers a
+// contiguous run of every tile's slots at a uniform order.  One wave per
+// tile; per 64 slots, one ballot per order gives every slot its rank.
+// perm[tbase[t] + j] = slot of the j-th slot of tile t in that grouping;
+// cls[o * ts + t] = first grouped slot of order pmin + o in tile t,
+// cls[(ncls + o) * ts + t] = their count.
+__global__ void __launch_bounds__(256) k_order_partition(int ntiles, int ts, const int *__restrict__ tbase,
+                                                         const int *__restrict__ tU,
+                                                         const unsigned char *__restrict__ tkp, int pmin, int ncls,
+                                                         int *__restrict__ perm, int *__restrict__ cls) {
+  const int lane = threadIdx.x & 63;
+  const int t = blockIdx.x * 4 + int(threadIdx.x >> 6);
+  if (t >= ntiles) return;  // whole wave
+  const int b = tbase[t], U = tU[t];
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  auto cls_of = [&](int u) { return u < U ? min(max(int(tkp[b + u]) - pmin, 0), ncls - 1) : -1; };
+  int cnt = 0;  // lane o < ncls: slots of order pmin + o
+  for (int u0 = 0; u0 < U; u0 += 64) {
+    const int o = cls_of(u0 + lane);
+    for (int k = 0; k < ncls; k++) {
+      const unsigned long long m = __ballot(o == k);
+      if (lane == k) cnt += __popcll(m);
+    }
+  }
+  // exclusive prefix of the counts over the orders
+  int incl = cnt;
+  for (int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d);
+    if (lane >= d) incl += v;
+  }
+  int run = incl - cnt;
+  if (lane < ncls) {
+    cls[int64_t(lane) * ts + t] = run;
+    cls[int64_t(ncls + lane) * ts + t] = cnt;
+  }
+  for (int u0 = 0; u0 < U; u0 += 64) {
+    const int o = cls_of(u0 + lane);
+    for (int k = 0; k < ncls; k++) {
+      const unsigned long long m = __ballot(o == k);
+      if (!m) continue;
+      const int at = __shfl(run, k);
+      if (o == k) perm[b + at + __popcll(m & below)] = b + u0 + lane;
+      if (lane == k) run += __popcll(m);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- operands
 // GEMM operands from the scaled FP64 multipoles M~ (P^2 per cell, rows of
 // KP): FP32 copy, or bf16 hi + lo split (hi = bf16(x), lo = bf16(x - hi)).
@@ -591,9 +639,19 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
 // Keys ub .. ue - 1 of tile bx, row block by.  Output: added into the FP64
 // locals L (Lpart null), or stored as FP32 partial locals, by cell
 // (Lpart + cell * PP, part_local false) or by tile column (Lpart + col * PP).
+// Order of one GEMM launch: pl (k bound pl^2 of every key, unless the
+// per-slot orders tkp are given), the row tiles RTo and k-steps K32o it
+// covers (of the RT x K32 of the operator layout, which holds order
+// sqrt(PP)), and an optional slot permutation (slot u of tile t is
+// perm[tbase[t] + u]).
+struct OrderPass {
+  int pl, RTo, K32o;
+  const int *perm;
+};
+
 template <int RH, bool SKIP>
 __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, int by, int ub, int ue, bool part_local,
-                                       int PP, int RT, int RTB, int K32, int KP,
+                                       int PP, int RT, int RTB, int K32, int KP, OrderPass op,
                                        const int *__restrict__ tfirst,
                                        const int *__restrict__ tcount,
                                        const int *__restrict__ tbase,
@@ -620,8 +678,10 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
   // the zero row)
   const int nct = max(0, min(4, (ncol - cg * 64 + 15) / 16));
   const int U = ue - ub, base = tbase[tile] + ub;
+  // a tile without keys in this launch adds nothing to the locals
+  if (U <= 0 && !Lpart) return;
   const int rt0 = by * RTB;
-  const int nrt = min(RTB, RT - rt0);
+  const int nrt = min(RTB, op.RTo - rt0);
   const int kq = lane >> 4;
   v4f acc[RH][4];
 #pragma unroll
@@ -629,11 +689,14 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) acc[r][ct] = v4f{0, 0, 0, 0};
   const int64_t per = int64_t(K32) * RT * 64;  // v8bf per key and part
-  const int NC = K32 * nrt;                    // runs of 64 v8bf per staged part
+  const int NC = op.K32o * nrt;                // runs of 64 v8bf per staged part
   const int E = NC * 64;                       // v8bf per staged part
+  auto slot = [&](int u) { return op.perm ? op.perm[base + u] : base + u; };
   // order of key u (tkp: per-key orders of a variable-order evaluation):
   // k-steps and row tiles of this block that it needs, and its K bound
-  auto kcols = [&](int u) { const int p = tkp ? tkp[base + u] : (int)__builtin_sqrtf(float(PP) + 0.5f); return p * p; };
+  auto kcols = [&](int u) { const int p = tkp ? tkp[slot(u)] : op.pl; return p * p; };
+  // a K bound below the operator's order can fall inside a k-group
+  const bool varK = tkp || op.pl * op.pl < PP;
   auto ksteps = [&](int kc) { return (kc + 31) / 32; };
   auto rtiles = [&](int kc) { return max(0, min(nrt, (kc + 15) / 16 - rt0)); };
   if (U > 0) {
@@ -649,7 +712,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
       for (int u = r * kr; u < min(r * kr + kr, U); u++) {
         v8bf *dst = Ab + size_t((r & 1) * kr + u - r * kr) * 2 * E;
         const int kc = kcols(u), ks = ksteps(kc), nr = rtiles(kc);
-        const int64_t k = tkeys[base + u];
+        const int64_t k = tkeys[slot(u)];
         const int n1 = ks * nr;
         for (int j = w; j < 2 * n1; j += kGemmThreads / 64) {
           const int part = j >= n1, jj = part ? j - n1 : j;
@@ -664,7 +727,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
       const int nu = min(kr, U - r * kr);
       for (int j = w; j < 4 * nu; j += kGemmThreads / 64) {
         const int ui = j >> 2, q = j & 3;
-        __builtin_amdgcn_global_load_lds(tmat + (int64_t(base) + r * kr + ui) * kTN + q * 64 + lane,
+        __builtin_amdgcn_global_load_lds(tmat + int64_t(slot(r * kr + ui)) * kTN + q * 64 + lane,
                                          Ts + ((r % 3) * kr + ui) * kTN + q * 64, 4, 0, 0);
       }
     };
@@ -729,8 +792,8 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
             bh[ct] = nbh[ct];
             bl[ct] = nbl[ct];
           }
-          // only a variable order has a bound inside the padded K range
-          if (tkp && kk * 32 + 32 > kc) mask_b(kk, kc, bh, bl);
+          // only an order below the operator's has a bound inside the padded K range
+          if (varK && kk * 32 + 32 > kc) mask_b(kk, kc, bh, bl);
           if (kk + 2 < ks) {
             load_b(src, kk + 2, kc, nbh, nbl);
             b_after = true;
@@ -783,7 +846,8 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
 }
 
 template <int RH>
-__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int RTB, int K32, int KP,
+__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int RTB, int K32, int KP, OrderPass op,
+                                                             const int *__restrict__ tcb,
                                                              const int *__restrict__ tfirst,
                                                              const int *__restrict__ tcount,
                                                              const int *__restrict__ tbase,
@@ -799,9 +863,11 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
                                                              int64_t ncells, float *__restrict__ Lpart,
                                                              double *__restrict__ L) {
   // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
-  const int Ut = tU[blockIdx.x], ub = int(int64_t(Ut) * blockIdx.z / gridDim.z),
-            ue = int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
-  m2l_bf16x3_body<RH, false>(blockIdx.x, blockIdx.y, ub, ue, false, PP, RT, RTB, K32, KP, tfirst, tcount, tbase,
+  // (with tcb: of the tU[t] keys from tcb[t] on, one order's share)
+  const int Ut = tU[blockIdx.x], c0 = tcb ? tcb[blockIdx.x] : 0;
+  const int ub = c0 + int(int64_t(Ut) * blockIdx.z / gridDim.z),
+            ue = c0 + int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
+  m2l_bf16x3_body<RH, false>(blockIdx.x, blockIdx.y, ub, ue, false, PP, RT, RTB, K32, KP, op, tfirst, tcount, tbase,
                              tU, sorted, tkeys, tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
                              Lpart ? Lpart + int64_t(blockIdx.z) * ncells * PP : nullptr, L);
 }
@@ -814,7 +880,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
 // k_sk_combine adds a tile's segments in workgroup order.  Every workgroup
 // gets the same number of slots, so none waits on a straggler.
 template <int RH>
-__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, int K32, int KP, int ntiles,
+__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, int K32, int KP, int pl, int ntiles,
                                                                 int64_t S, const int *__restrict__ tfirst,
                                                                 const int *__restrict__ tcount,
                                                                 const int *__restrict__ tbase,
@@ -844,7 +910,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, 
     const int64_t b = tbase[t];
     const int ub = int(max(s0, b) - b), ue = int(min(s1, b + tU[t]) - b);
     if (ub >= ue) continue;
-    m2l_bf16x3_body<RH, false>(t, 0, ub, ue, true, PP, RT, RT, K32, KP, tfirst, tcount, tbase, tU, sorted, tkeys,
+    m2l_bf16x3_body<RH, false>(t, 0, ub, ue, true, PP, RT, RT, K32, KP, OrderPass{pl, RT, K32, nullptr}, tfirst,
+                               tcount, tbase, tU, sorted, tkeys,
                                tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
                                Lseg + (w + t) * int64_t(kTN) * PP, nullptr);
     __syncthreads();  // the LDS of this segment is free before the next one stages
@@ -888,10 +955,11 @@ template <int RH>
 __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_grouped(const GroupGemm *__restrict__ g,
                                                                      const int *__restrict__ start, int nprob,
                                                                      int PP, int RT, int RTB, int K32, int KP,
-                                                                     int kr) {
+                                                                     int pl, int kr) {
   const int b = blockIdx.x, i = group_of(start, nprob, b), lb = b - start[i];
   const GroupGemm &q = g[i];
   m2l_bf16x3_body<RH, true>(lb % q.ntiles, lb / q.ntiles, 0, q.tU[lb % q.ntiles], false, PP, RT, RTB, K32, KP,
+                            OrderPass{pl, RT, K32, nullptr},
                             q.tfirst, q.tcount, q.tbase,
                             q.tU, q.sorted, q.tkeys, q.tmat, q.ophi, q.oplo, q.mhi, q.mlo, nullptr, kr, q.ncells,
                             nullptr, q.L);
@@ -924,14 +992,23 @@ static size_t gemm_lds(const Geom &g, bool bf16, int RTB, int kr) {
   return std::max(ops, red);
 }
 
+// One order's launch of a learned-order BF16x3 GEMM: order p over the
+// slots of that order (tile t: grouped slots cb[t] .. cb[t] + cnt[t] - 1,
+// slot permutation perm).
+struct ClassPass {
+  int p;
+  const int *perm, *cb, *cnt;
+};
+
 template <int RH>
 static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, int RTB,
-                        int kr, int ksplit, float *Lpart, double *Lout) {
+                        int kr, int ksplit, float *Lpart, double *Lout, const ClassPass *cp) {
   Ctx &c = tc;
   const int ts = c.m2l_tstride;
   const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ts, *tbase = c.tiles.p + 2 * ts, *tU = c.tiles.p + 3 * ts;
   const int *sorted = c.tiles.p + 4 * ts;
-  const dim3 grid(ntiles, ceil_div(g.R, RTB), ksplit);
+  const Geom go = cp ? geom(cp->p) : g;  // rows and k-steps this launch covers
+  const dim3 grid(ntiles, ceil_div(go.R, RTB), ksplit);
   if (bf16) {
     const v8bf *ophi = reinterpret_cast<const v8bf *>(c.ops32.p);
     const size_t per = size_t(g.K32) * g.R * 64 * 8;
@@ -939,13 +1016,14 @@ static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool 
                                                       per * c.lists.nkeys);
     const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
     const __bf16 *mlo = mhi + bf16_cast_elems(g, sc.t.ncells);
-    const size_t lds = gemm_lds(g, true, RTB, kr);
+    const size_t lds = gemm_lds(go, true, RTB, kr);
+    const OrderPass op{go.P, go.R, go.K32, cp ? cp->perm : nullptr};
     XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3<RH>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL((k_m2l_bf16x3<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, tfirst,
-                       tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi, mlo,
-                       c.learned ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), ksplit > 1 ? Lpart : nullptr,
-                       Lout);
+    hipLaunchKernelGGL((k_m2l_bf16x3<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, op,
+                       cp ? cp->cb : nullptr, tfirst, tcount, tbase, cp ? cp->cnt : tU, sorted, c.tile_keys.p,
+                       c.tile_mat.p, ophi, oplo, mhi, mlo, (c.learned && !cp) ? c.tile_p.p : nullptr, kr,
+                       int64_t(c.t.ncells), ksplit > 1 ? Lpart : nullptr, Lout);
   } else {
     const size_t lds = gemm_lds(g, false, RTB, kr);
     XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_f32<RH>),
@@ -988,7 +1066,7 @@ static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int
   const size_t lds = gemm_lds(g, true, g.R, kr);
   XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3_sk<RH>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-  hipLaunchKernelGGL((k_m2l_bf16x3_sk<RH>), dim3(G), dim3(kGemmThreads), lds, c.s0, g.PP, g.R, g.K32, KPb, ntiles, S,
+  hipLaunchKernelGGL((k_m2l_bf16x3_sk<RH>), dim3(G), dim3(kGemmThreads), lds, c.s0, g.PP, g.R, g.K32, KPb, g.P, ntiles, S,
                      tfirst, tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi, mlo,
                      c.learned ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), c.m2l_part.p);
   XF_LAUNCH();
@@ -998,8 +1076,9 @@ static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int
 }
 
 static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, GemmShape sh,
-                     double *Lout) {
+                     double *Lout, const ClassPass *cp = nullptr) {
   if (sh.sk) {
+    if (cp) fail(EXAFMM_ERR_UNSUPPORTED, "stream-K GEMM over one order's slots");
     switch (g.R) {
 #define XF_GEMM_SK(n) \
   case n: launch_gemm_sk<n>(c, sc, g, ntiles, KPb, sh.kr, Lout); return;
@@ -1022,7 +1101,7 @@ static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16
   }
   switch (sh.rtb) {
 #define XF_GEMM_CASE(n) \
-  case n: launch_gemm<n>(c, sc, g, ntiles, bf16, KP, KPb, sh.rtb, sh.kr, sh.ksplit, part, Lout); break;
+  case n: launch_gemm<n>(c, sc, g, ntiles, bf16, KP, KPb, sh.rtb, sh.kr, sh.ksplit, part, Lout, cp); break;
     XF_GEMM_CASE(1)
     XF_GEMM_CASE(2)
     XF_GEMM_CASE(3)
@@ -1053,21 +1132,27 @@ static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16
 // combining kernel) runs twice into a scratch copy of the locals, alone on
 // the device, timed with HIP events; the fastest run's shape is kept for
 // the context's lifetime.
-static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb) {
+static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb,
+                            const ClassPass *cp = nullptr) {
   int tclass = 0;
   while ((1 << tclass) < ntiles) tclass++;
-  const long long key = ((long long)g.P << 24) | ((long long)bf16 << 20) | ((long long)c.learned << 16) | tclass;
+  // a learned-order launch of one order is tuned per order (its rows, k-steps and LDS)
+  const Geom go = cp ? geom(cp->p) : g;
+  const long long key = ((long long)g.P << 24) | ((long long)bf16 << 20) | ((long long)c.learned << 16) |
+                        ((long long)(cp ? cp->p : 0) << 32) | tclass;
   auto it = c.gemm_tuned.find(key);
   if (it != c.gemm_tuned.end())
     return GemmShape{int(it->second & 255), int((it->second >> 8) & 255), int((it->second >> 16) & 255),
                      int(it->second >> 24)};
   std::vector<GemmShape> cand;
-  for (int kr = 1; kr <= 3; kr++) {
-    const int rmax = rows_per_block(g, bf16, c.cfg.device, kr);
+  // low orders: short keys, so more of them per LDS round
+  const int krmax = go.P <= 10 ? 5 : 3;
+  for (int kr = 1; kr <= krmax; kr++) {
+    const int rmax = rows_per_block(go, bf16, c.cfg.device, kr);
     if (rmax < 1) continue;
     int last = -1;
-    for (int nb = ceil_div(g.R, rmax); nb <= ceil_div(g.R, rmax) + 2; nb++) {
-      const int r = ceil_div(g.R, nb);
+    for (int nb = ceil_div(go.R, rmax); nb <= ceil_div(go.R, rmax) + 2; nb++) {
+      const int r = ceil_div(go.R, nb);
       if (r != last && r <= rmax) {
         cand.push_back(GemmShape{r, kr, 1});
         // key slices only where the workgroups would not fill the device
@@ -1079,7 +1164,7 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
     }
   }
   // stream-K (BF16x3, all rows per workgroup)
-  if (bf16 && g.R <= kGemmMaxRT && std::getenv("EXAFMM_GEMM_NO_SK") == nullptr)
+  if (bf16 && !cp && g.R <= kGemmMaxRT && std::getenv("EXAFMM_GEMM_NO_SK") == nullptr)
     for (int kr = 1; kr <= 3; kr++)
       if (rows_per_block(g, bf16, c.cfg.device, kr) >= g.R) cand.push_back(GemmShape{g.R, kr, 1, 1});
   GemmShape best = cand[0];
@@ -1108,7 +1193,7 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
     for (const GemmShape &sh : cand) {
       for (int rep = 0; rep < 2; rep++) {  // best of two runs per candidate
         XF_HIP(hipEventRecord(e0, s));
-        run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L_tune.p);
+        run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L_tune.p, cp);
         XF_HIP(hipEventRecord(e1, s));
         XF_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -1283,6 +1368,26 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
                  c.tiles.p + 3 * c.m2l_tstride, c.tiles.p + 4 * c.m2l_tstride, ntiles, c.m2l_totalU, kTN, s);
 
   // 3. GEMM over tiles, shape from the autotuner
+  if (c.learned && bf16) {
+    // one launch per order over that order's slots of every tile (grouped
+    // by k_order_partition), each with its own tuned shape: a slot's cost
+    // then follows its order, and the launches add into the locals in
+    // order sequence
+    const int pmin = kMlpPmin, ncls = std::max(1, c.P - kMlpPmin + 1);
+    const int ts = c.m2l_tstride;
+    c.tile_perm.ensure(std::max(c.m2l_totalU, 1));
+    c.tile_cls.ensure(size_t(2) * ncls * ts);
+    hipLaunchKernelGGL(k_order_partition, dim3(ceil_div(ntiles, 4)), dim3(256), 0, s, ntiles, ts, c.tiles.p + 2 * ts,
+                       c.tiles.p + 3 * ts, c.tile_p.p, pmin, ncls, c.tile_perm.p, c.tile_cls.p);
+    XF_LAUNCH();
+    if (!c.capturing) XF_HIP(hipEventRecord(c.ev[13], s));
+    for (int o = ncls - 1; o >= 0; o--) {
+      const ClassPass cp{pmin + o, c.tile_perm.p, c.tile_cls.p + size_t(o) * ts, c.tile_cls.p + size_t(ncls + o) * ts};
+      const GemmShape sh = gemm_shape(c, sc, g, ntiles, bf16, KP, KPb, &cp);
+      run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L.p, &cp);
+    }
+    return;
+  }
   const GemmShape sh = gemm_shape(c, sc, g, ntiles, bf16, KP, KPb);
   if (!c.capturing) XF_HIP(hipEventRecord(c.ev[13], s));
   run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L.p);
@@ -1463,7 +1568,7 @@ void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s) {
 #define XF_GROUP_CASE(n)                                                                                         \
   case n:                                                                                                         \
     hipLaunchKernelGGL((k_m2l_bf16x3_grouped<n>), dim3(plan.nblocks), dim3(kGemmThreads), plan.lds, s, d,         \
-                       plan.start.p, plan.nprob, g.PP, g.R, plan.rtb, g.K32, KPb, plan.kr);                       \
+                       plan.start.p, plan.nprob, g.PP, g.R, plan.rtb, g.K32, KPb, g.P, plan.kr);                  \
     break;
     XF_GROUP_CASE(1)
     XF_GROUP_CASE(2)

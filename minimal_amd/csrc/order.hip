@@ -80,67 +80,123 @@ __global__ void __launch_bounds__(256) k_mlp_forward(const float *__restrict__ w
 }
 
 // Loss = sum over samples and valid outputs of (y - t)^2 / count, with
-// t = NaN marking an output without data.  Gradients with respect to the
-// BF16-valued weights are accumulated per workgroup in LDS (FP32) and
-// written as one partial row per workgroup.
+// t = NaN marking an output without data.  Every gradient entry is a sum
+// over samples of a product of two per-sample quantities (weight grads:
+// the backpropagated error of a unit times an input of its layer; biases:
+// the error times 1).  A workgroup takes chunks of kGradChunk samples
+// (chunks b, b + gridDim.x, ...): one thread per sample runs the forward
+// and backward pass and stores those quantities in LDS; then each thread
+// adds, for the parameters it owns, the chunk's products in sample order.
+// No atomics: the gradient is the same bit for bit on every run.
+constexpr int kGradChunk = 128;
+// per-sample row: x | h1 | h2 | dy | g2 | g1 | 1
+constexpr int sX = 0, sH1 = sX + kMlpIn, sH2 = sH1 + kMlpH, sDY = sH2 + kMlpH, sG2 = sDY + kMlpOut,
+              sG1 = sG2 + kMlpH, sOne = sG1 + kMlpH, kGradRow = sOne + 1;
+constexpr int kGradOwn = (kMlpParams + 255) / 256;  // parameters per thread
+
+// LDS columns (a, b) whose per-sample product sums to parameter q's gradient.
+__device__ inline void grad_cols(int q, int &a, int &b) {
+  if (q < oB1) {
+    a = sG1 + (q - oW1) / kMlpIn;
+    b = sX + (q - oW1) % kMlpIn;
+  } else if (q < oW2) {
+    a = sG1 + (q - oB1);
+    b = sOne;
+  } else if (q < oB2) {
+    a = sG2 + (q - oW2) / kMlpH;
+    b = sH1 + (q - oW2) % kMlpH;
+  } else if (q < oW3) {
+    a = sG2 + (q - oB2);
+    b = sOne;
+  } else if (q < oB3) {
+    a = sDY + (q - oW3) / kMlpH;
+    b = sH2 + (q - oW3) % kMlpH;
+  } else {
+    a = sDY + (q - oB3);
+    b = sOne;
+  }
+}
+
 __global__ void __launch_bounds__(256) k_mlp_grad(const float *__restrict__ w, const float *__restrict__ X,
                                                   const float *__restrict__ T, int64_t n, float inv_count,
                                                   float *__restrict__ gpart, float *__restrict__ lpart) {
   __shared__ float W[kMlpParams];
-  __shared__ float G[kMlpParams];
-  __shared__ float Ls;
-  for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) G[i] = 0.0f;
-  if (threadIdx.x == 0) Ls = 0.0f;
+  __shared__ float S[kGradChunk * kGradRow];
+  __shared__ float Lsum[256];
   load_weights(w, W);
-  float lsum = 0.0f;
-  for (int64_t s = int64_t(blockIdx.x) * 256 + threadIdx.x; s < n; s += int64_t(gridDim.x) * 256) {
-    float x[kMlpIn], h1[kMlpH], h2[kMlpH], y[kMlpOut], dy[kMlpOut];
+  float g[kGradOwn];
+  int ca[kGradOwn], cb[kGradOwn];
 #pragma unroll
-    for (int k = 0; k < kMlpIn; k++) x[k] = X[s * kMlpIn + k];
-    mlp_forward(W, x, h1, h2, y);
-    for (int o = 0; o < kMlpOut; o++) {
-      const float t = T[s * kMlpOut + o];
-      const bool ok = t == t;
-      const float e = ok ? y[o] - t : 0.0f;
-      lsum += e * e;
-      dy[o] = 2.0f * e * inv_count;
-    }
-    // layer 3
-    float dh2[kMlpH];
-    for (int k = 0; k < kMlpH; k++) dh2[k] = 0.0f;
-    for (int o = 0; o < kMlpOut; o++) {
-      if (dy[o] == 0.0f) continue;
-      atomicAdd(&G[oB3 + o], dy[o]);
-      for (int k = 0; k < kMlpH; k++) {
-        atomicAdd(&G[oW3 + o * kMlpH + k], dy[o] * h2[k]);
-        dh2[k] += W[oW3 + o * kMlpH + k] * dy[o];
-      }
-    }
-    // layer 2
-    float dh1[kMlpH];
-    for (int k = 0; k < kMlpH; k++) dh1[k] = 0.0f;
-    for (int i = 0; i < kMlpH; i++) {
-      const float g = h2[i] > 0.0f ? dh2[i] : 0.0f;
-      if (g == 0.0f) continue;
-      atomicAdd(&G[oB2 + i], g);
-      for (int k = 0; k < kMlpH; k++) {
-        atomicAdd(&G[oW2 + i * kMlpH + k], g * h1[k]);
-        dh1[k] += W[oW2 + i * kMlpH + k] * g;
-      }
-    }
-    // layer 1
-    for (int i = 0; i < kMlpH; i++) {
-      const float g = h1[i] > 0.0f ? dh1[i] : 0.0f;
-      if (g == 0.0f) continue;
-      atomicAdd(&G[oB1 + i], g);
-#pragma unroll
-      for (int k = 0; k < kMlpIn; k++) atomicAdd(&G[oW1 + i * kMlpIn + k], g * x[k]);
-    }
+  for (int j = 0; j < kGradOwn; j++) {
+    g[j] = 0.0f;
+    const int q = threadIdx.x + 256 * j;
+    ca[j] = cb[j] = sOne;
+    if (q < kMlpParams) grad_cols(q, ca[j], cb[j]);
   }
-  atomicAdd(&Ls, lsum * inv_count);
+  float lsum = 0.0f;
+  for (int64_t c0 = int64_t(blockIdx.x) * kGradChunk; c0 < n; c0 += int64_t(gridDim.x) * kGradChunk) {
+    const int ns = int(min<int64_t>(kGradChunk, n - c0));
+    if (int(threadIdx.x) < kGradChunk) {
+      float *R = S + threadIdx.x * kGradRow;
+      if (int(threadIdx.x) < ns) {
+        const int64_t s = c0 + threadIdx.x;
+        float x[kMlpIn], h1[kMlpH], h2[kMlpH], y[kMlpOut], dy[kMlpOut];
+#pragma unroll
+        for (int k = 0; k < kMlpIn; k++) x[k] = X[s * kMlpIn + k];
+        mlp_forward(W, x, h1, h2, y);
+        for (int o = 0; o < kMlpOut; o++) {
+          const float t = T[s * kMlpOut + o];
+          const bool ok = t == t;
+          const float e = ok ? y[o] - t : 0.0f;
+          lsum += e * e;
+          dy[o] = 2.0f * e * inv_count;
+        }
+        float g2[kMlpH], g1[kMlpH];
+        for (int k = 0; k < kMlpH; k++) {
+          float a = 0.0f;
+          for (int o = 0; o < kMlpOut; o++) a += W[oW3 + o * kMlpH + k] * dy[o];
+          g2[k] = h2[k] > 0.0f ? a : 0.0f;
+        }
+        for (int k = 0; k < kMlpH; k++) {
+          float a = 0.0f;
+          for (int i = 0; i < kMlpH; i++) a += W[oW2 + i * kMlpH + k] * g2[i];
+          g1[k] = h1[k] > 0.0f ? a : 0.0f;
+        }
+#pragma unroll
+        for (int k = 0; k < kMlpIn; k++) R[sX + k] = x[k];
+        for (int k = 0; k < kMlpH; k++) {
+          R[sH1 + k] = h1[k];
+          R[sH2 + k] = h2[k];
+          R[sG2 + k] = g2[k];
+          R[sG1 + k] = g1[k];
+        }
+        for (int o = 0; o < kMlpOut; o++) R[sDY + o] = dy[o];
+        R[sOne] = 1.0f;
+      } else {
+        for (int k = 0; k < kGradRow; k++) R[k] = 0.0f;  // past the last sample: adds zero
+      }
+    }
+    __syncthreads();
+    for (int s = 0; s < kGradChunk; s++) {
+      const float *R = S + s * kGradRow;
+#pragma unroll
+      for (int j = 0; j < kGradOwn; j++) g[j] += R[ca[j]] * R[cb[j]];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int j = 0; j < kGradOwn; j++) {
+    const int q = threadIdx.x + 256 * j;
+    if (q < kMlpParams) gpart[int64_t(blockIdx.x) * kMlpParams + q] = g[j];
+  }
+  // loss: the threads' sums added in thread order
+  Lsum[threadIdx.x] = lsum * inv_count;
   __syncthreads();
-  for (int i = threadIdx.x; i < kMlpParams; i += blockDim.x) gpart[int64_t(blockIdx.x) * kMlpParams + i] = G[i];
-  if (threadIdx.x == 0) lpart[blockIdx.x] = Ls;
+  if (threadIdx.x == 0) {
+    float l = 0.0f;
+    for (int i = 0; i < 256; i++) l += Lsum[i];
+    lpart[blockIdx.x] = l;
+  }
 }
 
 // Sum of the partial rows (fixed order) -> gradient and loss.
@@ -403,11 +459,12 @@ __global__ void __launch_bounds__(64) k_pair_errors(int npairs, const int *__res
 // ---------------------------------------------------------------- host side
 static void mlp_grad(Ctx &c, const float *w, const float *X, const float *T, int64_t n, float *grad, float *loss,
                      hipStream_t s) {
-  const int nb = int(std::min<int64_t>(64, std::max<int64_t>(1, ceil_div(n, 256))));
+  // one chunk of samples per workgroup up to 256 workgroups (a fixed
+  // function of n: the partial rows, and so the gradient, do not depend on
+  // the run)
+  const int nb = int(std::min<int64_t>(256, std::max<int64_t>(1, ceil_div(n, kGradChunk))));
   c.mlp_part.ensure(size_t(nb) * (kMlpParams + 1));
   float *gp = c.mlp_part.p, *lp = c.mlp_part.p + size_t(nb) * kMlpParams;
-  int64_t cnt = 0;
-  (void)cnt;
   hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(256), 0, s, w, X, T, n, 1.0f / float(std::max<int64_t>(n, 1)), gp,
                      lp);
   XF_LAUNCH();

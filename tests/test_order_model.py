@@ -7,6 +7,7 @@ Bars:
   oracle gradient vs central differences (FP64, unrounded weights)   2e-7 relative
   device forward vs oracle (BF16 weights, FP32 vs FP64 accumulation)  1e-4 (1 + |y|)
   device gradient vs oracle                                           1e-4 of max |g|
+  device gradient and training, run twice                             bit for bit
   training errors vs oracle P2M/M2L/L2P + direct (FP64 both sides)    0.02 in log10
 """
 import numpy as np
@@ -70,6 +71,31 @@ def test_device_gradient_matches_oracle(gpu):
     assert np.max(np.abs(g - gref)) <= 1e-4 * np.max(np.abs(gref))
 
 
+@pytest.mark.gpu
+def test_device_gradient_is_deterministic(gpu):
+    """The gradient is summed without atomics, in a fixed order: two runs
+    over 40,000 samples (313 chunks over 256 workgroups) agree bit for bit,
+    and so do two trainings from the same seed."""
+    w, X, T = _random_problem(7, 40000)
+    f = fmm.FMM()
+    g1, l1 = f.mlp_gradient(w, X, T)
+    g2, l2 = f.mlp_gradient(w, X, T)
+    assert np.array_equal(g1, g2) and l1 == l2
+    lref, gref = om.loss_and_grad(w, X, T)
+    assert np.max(np.abs(g1 - gref)) <= 1e-4 * np.max(np.abs(gref))
+    b = _plummer_bodies(20000, 8)
+    ws = []
+    for _ in range(2):
+        t = fmm.FMM(P=14, ncrit=64, theta=0.4)
+        t.build_tree(b.copy())
+        t.upward()
+        t.horizontal()
+        t.train_order_model(nsamples=2048, epochs=20)
+        ws.append(t.order_model())
+        del t
+    assert np.array_equal(ws[0], ws[1])
+
+
 def _plummer_bodies(n, seed):
     rng = np.random.default_rng(seed)
     b = np.zeros(n, BODY_DTYPE)
@@ -119,6 +145,50 @@ def test_training_errors_match_oracle(gpu, orc):
                 assert abs(np.log10(e) - t[o]) <= 0.02, (ct, cs, p, np.log10(e), t[o])
                 checked += 1
     assert checked > 100
+
+
+@pytest.mark.gpu
+def test_order_grouped_gemm_matches_per_slot_orders(gpu):
+    """The BF16x3 learned-order GEMM groups each tile's slots by order and
+    runs one launch per order; the FP32 GEMM reads each slot's order inside
+    one launch.  With the same model both pick the same slot orders, and
+    the two evaluations may differ only as much as BF16x3 and FP32 differ
+    at a fixed order.  With a tolerance no order meets, every slot runs at
+    the context's order P and the grouped launches reproduce the
+    fixed-order evaluation bit for bit."""
+    n = 1 << 15
+    tr = _plummer_bodies(n, 9)
+    t = fmm.FMM(P=14, ncrit=64, theta=0.4)
+    t.build_tree(tr)
+    t.upward()
+    t.horizontal()
+    t.train_order_model(nsamples=4096, epochs=150)
+    w = t.order_model()
+    del t
+    b = _plummer_bodies(n, 10)
+
+    def run(policy, prec, tol=None):
+        f = fmm.FMM(P=12, ncrit=64, theta=0.4, m2l_precision=prec, order_policy=policy, autotune=0)
+        if tol is not None:
+            f.set_order_model(w, tol)
+        x = b.copy()
+        f.evaluate(x)
+        return x, f.order_stats()
+
+    lb, hb = run(fmm.ORDER_LEARNED, fmm.PREC_BF16X3, -7.5)
+    lf, hf = run(fmm.ORDER_LEARNED, fmm.PREC_FP32, -7.5)
+    fb, _ = run(fmm.ORDER_FIXED, fmm.PREC_BF16X3)
+    ff, _ = run(fmm.ORDER_FIXED, fmm.PREC_FP32)
+    assert np.array_equal(hb, hf)
+    assert np.count_nonzero(hb) >= 3  # several orders in play
+    dl = rel_l2(lb["p"], lf["p"]), rel_l2(lb["F"], lf["F"])
+    df = rel_l2(fb["p"], ff["p"]), rel_l2(fb["F"], ff["F"])
+    print("BF16x3 vs FP32, learned", dl, "fixed", df)
+    assert dl[0] <= 2 * df[0] + 1e-7 and dl[1] <= 2 * df[1] + 1e-7, (dl, df)
+    # no order meets the tolerance: all slots at P = 12, one launch
+    la, ha = run(fmm.ORDER_LEARNED, fmm.PREC_BF16X3, -30.0)
+    assert ha[12] == ha.sum()
+    assert np.array_equal(la["p"], fb["p"]) and np.array_equal(la["F"], fb["F"])
 
 
 @pytest.mark.gpu
