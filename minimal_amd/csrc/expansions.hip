@@ -333,6 +333,129 @@ __global__ void __launch_bounds__(kM2MWaves *kWave) k_m2m(int P, int lstart, int
   m2m_body(blockIdx.x, blockIdx.y, P, lstart, lcount, cchild, cnch, cx, T, M);
 }
 
+// Row-tile form of M2M and L2L (single tree): one workgroup per 16 parents
+// of a level, wave w = row tile w of the P^2 rows (ceil(P^2/16) waves), each
+// wave looping over the 8 octants.  All 8 x R waves that read a parent
+// tile's operands (the children's multipoles, the parent's locals) run in
+// one workgroup, so those rows come from HBM once and then from that CU's
+// caches (the one-octant-per-workgroup layout fetched them once per XCD
+// and row block).  Operator columns and the B operand are loaded
+// kShiftBatch2 k-steps at a time.
+constexpr int kShiftBatch2 = 8;
+constexpr int kRowTileMaxWaves = 16;  // P <= 16
+
+// waves of a row-tile workgroup: one per row tile, at least two (filing the
+// children of 16 parents takes 128 threads)
+static int row_tile_waves(int PP) { return std::max(2, ceil_div(PP, 16)); }
+
+__device__ inline void file_children_all(int p0, int lstart, int lcount, const int *__restrict__ cchild,
+                                         const int *__restrict__ cnch, const double4 *__restrict__ cx,
+                                         int (*chl)[16]) {
+  if (threadIdx.x < 128) chl[threadIdx.x / 16][threadIdx.x % 16] = -1;
+  __syncthreads();
+  file_children(threadIdx.x, p0, lstart, lcount, cchild, cnch, cx, -1, chl);
+  __syncthreads();
+}
+
+// M~_parent = sum over octants of T_M2M[octant] M~_child (3d/kernel.h:206),
+// octants added in order into one accumulator per row tile.
+__global__ void __launch_bounds__(kRowTileMaxWaves *kWave) k_m2m_rows(int P, int lstart, int lcount,
+                                                                      const int *__restrict__ cchild,
+                                                                      const int *__restrict__ cnch,
+                                                                      const double4 *__restrict__ cx,
+                                                                      const double *__restrict__ T,
+                                                                      double *__restrict__ M) {
+  __shared__ int chl[8][16];
+  const int PP = P * P;
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int j16 = lane & 15, kq = lane >> 4;
+  const int p0 = blockIdx.x * 16;
+  file_children_all(p0, lstart, lcount, cchild, cnch, cx, chl);
+  const int r0 = w * 16;  // blockDim.x = 64 max(2, ceil(PP / 16))
+  if (r0 >= PP) return;
+  const int jmax = degree_of(min(r0 + 16, PP) - 1);
+  const int ncol = (jmax + 1) * (jmax + 1);
+  const int ksteps = (ncol + 3) / 4;
+  const int row = min(r0 + j16, PP - 1);
+  v4d acc = {0, 0, 0, 0};
+  for (int o = 0; o < 8; o++) {
+    const int ch = chl[o][j16];
+    if (!__any(ch >= 0)) continue;
+    const double *To = T + int64_t(o) * PP * PP;
+    const double *Mc = M + int64_t(ch < 0 ? 0 : ch) * PP;
+    for (int k0 = 0; k0 < ksteps; k0 += kShiftBatch2) {
+      double a[kShiftBatch2], b[kShiftBatch2];
+#pragma unroll
+      for (int s = 0; s < kShiftBatch2; s++) {
+        const int col = (k0 + s) * 4 + kq;
+        const bool in = col < ncol;
+        a[s] = in ? To[int64_t(col) * PP + row] : 0.0;
+        b[s] = (in && ch >= 0) ? Mc[col] : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < kShiftBatch2; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+    }
+  }
+  const int pl = p0 + j16;
+  if (pl >= lcount) return;
+  const int p = lstart + pl;
+  if (cnch[p] == 0) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int r = r0 + kq + 4 * i;
+    if (r < PP) M[int64_t(p) * PP + r] = acc[i];
+  }
+}
+
+// L~_child += T_L2L[octant] L~_parent (3d/kernel.h:265-280) for every child
+// of the 16 parents, octant by octant.
+__global__ void __launch_bounds__(kRowTileMaxWaves *kWave) k_l2l_rows(int P, int lstart, int lcount,
+                                                                      const int *__restrict__ cchild,
+                                                                      const int *__restrict__ cnch,
+                                                                      const double4 *__restrict__ cx,
+                                                                      const double *__restrict__ T,
+                                                                      double *__restrict__ L) {
+  __shared__ int chl[8][16];
+  const int PP = P * P;
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int j16 = lane & 15, kq = lane >> 4;
+  const int p0 = blockIdx.x * 16;
+  file_children_all(p0, lstart, lcount, cchild, cnch, cx, chl);
+  const int r0 = w * 16;
+  if (r0 >= PP) return;
+  const int jmin = degree_of(r0);
+  const int c0 = jmin * jmin;
+  const int ksteps = (PP - c0 + 3) / 4;
+  const int row = min(r0 + j16, PP - 1);
+  const int pl = min(p0 + j16, lcount - 1);
+  const double *Lp = L + int64_t(lstart + pl) * PP;
+  for (int o = 0; o < 8; o++) {
+    const int ch = chl[o][j16];
+    if (!__any(ch >= 0)) continue;
+    const double *To = T + (int64_t(8) + o) * PP * PP;
+    v4d acc = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < ksteps; k0 += kShiftBatch2) {
+      double a[kShiftBatch2], b[kShiftBatch2];
+#pragma unroll
+      for (int s = 0; s < kShiftBatch2; s++) {
+        const int col = c0 + (k0 + s) * 4 + kq;
+        const bool in = col < PP;
+        a[s] = in ? To[int64_t(col) * PP + row] : 0.0;
+        b[s] = (in && ch >= 0) ? Lp[col] : 0.0;
+      }
+#pragma unroll
+      for (int s = 0; s < kShiftBatch2; s++) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+    }
+    if (ch >= 0) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int r = r0 + kq + 4 * i;
+        if (r < PP) L[int64_t(ch) * PP + r] += acc[i];
+      }
+    }
+  }
+}
+
 // One level of every problem of a batch in one launch: lv[i] describes
 // problem lv[i].prob's cells of this level as nbx x nby blocks.
 __global__ void __launch_bounds__(kM2MWaves *kWave) k_m2m_grouped(const GroupTree *__restrict__ g,
@@ -667,8 +790,8 @@ void upward(Ctx &c) {
   XF_LAUNCH();
   for (int l = t.depth - 1; l >= 0; l--) {
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_m2m, dim3(ceil_div(cnt, 16), m2m_group_rows(P)), dim3(kM2MWaves * kWave), 0,
-                       s, c.P, t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
+    hipLaunchKernelGGL(k_m2m_rows, dim3(ceil_div(cnt, 16)), dim3(row_tile_waves(PP) * kWave), 0, s, c.P,
+                       t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
     XF_LAUNCH();
   }
   c.have_M = true;
@@ -681,8 +804,8 @@ void l2l_pass(Ctx &c) {
   shift_ops(c);
   for (int l = 0; l < t.depth; l++) {  // parents at level l
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_l2l, dim3(ceil_div(cnt, 16), ceil_div(PP, 16 * kShiftWaves), 8), dim3(kShiftWaves * kWave),
-                       0, s, c.P, t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.L.p);
+    hipLaunchKernelGGL(k_l2l_rows, dim3(ceil_div(cnt, 16)), dim3(row_tile_waves(PP) * kWave), 0, s, c.P,
+                       t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.L.p);
     XF_LAUNCH();
   }
 }
