@@ -287,8 +287,8 @@ __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirs
 // contiguous run of every tile's slots at a uniform order.  One wave per
 // tile; per 64 slots, one ballot per order gives every slot its rank.
 // perm[tbase[t] + j] = slot of the j-th slot of tile t in that grouping;
-// cls[o * ts + t] = first grouped slot of order pmin + o in tile t,
-// cls[(ncls + o) * ts + t] = their count.
+// cls[o * ts + t] = tbase[t] + first grouped slot of order pmin + o in tile
+// t, cls[(ncls + o) * ts + t] = their count.
 __global__ void __launch_bounds__(256) k_order_partition(int ntiles, int ts, const int *__restrict__ tbase,
                                                          const int *__restrict__ tU,
                                                          const unsigned char *__restrict__ tkp, int pmin, int ncls,
@@ -315,7 +315,7 @@ __global__ void __launch_bounds__(256) k_order_partition(int ntiles, int ts, con
   }
   int run = incl - cnt;
   if (lane < ncls) {
-    cls[int64_t(lane) * ts + t] = run;
+    cls[int64_t(lane) * ts + t] = b + run;
     cls[int64_t(ncls + lane) * ts + t] = cnt;
   }
   for (int u0 = 0; u0 < U; u0 += 64) {
@@ -328,6 +328,28 @@ __global__ void __launch_bounds__(256) k_order_partition(int ntiles, int ts, con
       if (lane == k) run += __popcll(m);
     }
   }
+}
+
+// Per order (wave o), the exclusive prefix over the tiles of that order's
+// slot counts, and their total: the stream-K schedule of the order's launch.
+__global__ void __launch_bounds__(1024) k_order_prefix(int ntiles, int ts, int ncls, int *__restrict__ cls) {
+  const int lane = threadIdx.x & 63, o = int(threadIdx.x >> 6);
+  if (o >= ncls) return;
+  const int *cnt = cls + int64_t(ncls + o) * ts;
+  int *pre = cls + int64_t(2 * ncls + o) * ts;
+  int carry = 0;
+  for (int t0 = 0; t0 < ntiles; t0 += 64) {
+    const int t = t0 + lane;
+    const int v = t < ntiles ? cnt[t] : 0;
+    int incl = v;
+    for (int d = 1; d < 64; d <<= 1) {
+      const int u = __shfl_up(incl, d);
+      if (lane >= d) incl += u;
+    }
+    if (t < ntiles) pre[t] = carry + incl - v;
+    carry += __shfl(incl, 63);
+  }
+  if (lane == 0) cls[int64_t(3 * ncls) * ts + o] = carry;
 }
 
 // ---------------------------------------------------------------- operands
@@ -847,7 +869,6 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
 
 template <int RH>
 __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int RTB, int K32, int KP, OrderPass op,
-                                                             const int *__restrict__ tcb,
                                                              const int *__restrict__ tfirst,
                                                              const int *__restrict__ tcount,
                                                              const int *__restrict__ tbase,
@@ -863,10 +884,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
                                                              int64_t ncells, float *__restrict__ Lpart,
                                                              double *__restrict__ L) {
   // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
-  // (with tcb: of the tU[t] keys from tcb[t] on, one order's share)
-  const int Ut = tU[blockIdx.x], c0 = tcb ? tcb[blockIdx.x] : 0;
-  const int ub = c0 + int(int64_t(Ut) * blockIdx.z / gridDim.z),
-            ue = c0 + int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
+  const int Ut = tU[blockIdx.x];
+  const int ub = int(int64_t(Ut) * blockIdx.z / gridDim.z), ue = int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
   m2l_bf16x3_body<RH, false>(blockIdx.x, blockIdx.y, ub, ue, false, PP, RT, RTB, K32, KP, op, tfirst, tcount, tbase,
                              tU, sorted, tkeys, tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
                              Lpart ? Lpart + int64_t(blockIdx.z) * ncells * PP : nullptr, L);
@@ -878,12 +897,18 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
 // [w S / G, (w + 1) S / G); each tile it touches gives one FP32 partial of the
 // tile's locals, stored by tile column in segment w + t of Lpart, and
 // k_sk_combine adds a tile's segments in workgroup order.  Every workgroup
-// gets the same number of slots, so none waits on a straggler.
+// gets the same number of slots, so none waits on a straggler.  A launch of
+// one order of a learned-order evaluation schedules that order's slots:
+// tbase is then the exclusive prefix of their per-tile counts tU, tstart[t]
+// where tile t's run of them starts in the order-grouped slots, and Sdev
+// their total.
 template <int RH>
-__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, int K32, int KP, int pl, int ntiles,
-                                                                int64_t S, const int *__restrict__ tfirst,
+__global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, int K32, int KP, OrderPass op,
+                                                                int ntiles, int64_t S, const int *__restrict__ Sdev,
+                                                                const int *__restrict__ tfirst,
                                                                 const int *__restrict__ tcount,
                                                                 const int *__restrict__ tbase,
+                                                                const int *__restrict__ tstart,
                                                                 const int *__restrict__ tU,
                                                                 const int *__restrict__ sorted,
                                                                 const int *__restrict__ tkeys,
@@ -895,6 +920,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, 
                                                                 const unsigned char *__restrict__ tkp, int kr,
                                                                 int64_t ncells, float *__restrict__ Lseg) {
   const int64_t G = gridDim.x, w = blockIdx.x;
+  if (Sdev) S = *Sdev;  // one order's slot count, summed on the device
   const int64_t s0 = w * S / G, s1 = (w + 1) * S / G;
   if (s0 >= s1) return;
   // first tile holding slot s0: the last t with tbase[t] <= s0
@@ -910,25 +936,26 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, 
     const int64_t b = tbase[t];
     const int ub = int(max(s0, b) - b), ue = int(min(s1, b + tU[t]) - b);
     if (ub >= ue) continue;
-    m2l_bf16x3_body<RH, false>(t, 0, ub, ue, true, PP, RT, RT, K32, KP, OrderPass{pl, RT, K32, nullptr}, tfirst,
-                               tcount, tbase, tU, sorted, tkeys,
-                               tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
+    m2l_bf16x3_body<RH, false>(t, 0, ub, ue, true, PP, RT, op.RTo, K32, KP, op, tfirst, tcount, tstart, tU, sorted,
+                               tkeys, tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
                                Lseg + (w + t) * int64_t(kTN) * PP, nullptr);
     __syncthreads();  // the LDS of this segment is free before the next one stages
   }
 }
 
 // Sum of a tile's stream-K segments (workgroups wf .. wl in order) into the
-// FP64 locals.  One thread per (tile column, row).
-__global__ void __launch_bounds__(256) k_sk_combine(int PP, int ntiles, int64_t S, int G,
-                                                    const int *__restrict__ tfirst, const int *__restrict__ tcount,
-                                                    const int *__restrict__ tbase, const int *__restrict__ tU,
-                                                    const int *__restrict__ sorted, const float *__restrict__ Lseg,
-                                                    double *__restrict__ L) {
+// FP64 locals, rows 0 .. rows - 1 (those the launch computed).  One thread
+// per (tile column, row).
+__global__ void __launch_bounds__(256) k_sk_combine(int PP, int rows, int ntiles, int64_t S, const int *__restrict__ Sdev,
+                                                    int G, const int *__restrict__ tfirst,
+                                                    const int *__restrict__ tcount, const int *__restrict__ tbase,
+                                                    const int *__restrict__ tU, const int *__restrict__ sorted,
+                                                    const float *__restrict__ Lseg, double *__restrict__ L) {
   const int t = blockIdx.y;
   const int64_t e = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  const int col = int(e / PP), row = int(e % PP);
-  if (t >= ntiles || col >= tcount[t]) return;
+  const int col = int(e / rows), row = int(e % rows);
+  if (t >= ntiles || col >= tcount[t] || tU[t] == 0) return;
+  if (Sdev) S = *Sdev;
   // workgroup of slot s: the largest w with w S / G <= s
   auto wg_of = [&](int64_t sl) { return int(((sl + 1) * G - 1) / S); };
   const int64_t b = tbase[t];
@@ -997,7 +1024,9 @@ static size_t gemm_lds(const Geom &g, bool bf16, int RTB, int kr) {
 // slot permutation perm).
 struct ClassPass {
   int p;
-  const int *perm, *cb, *cnt;
+  const int *perm;
+  const int *start, *cnt;  // per tile: first order-grouped slot (tbase included) and count
+  const int *pre, *tot;    // per tile: exclusive prefix of the counts; their total (stream-K)
 };
 
 template <int RH>
@@ -1021,7 +1050,7 @@ static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool 
     XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3<RH>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL((k_m2l_bf16x3<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, op,
-                       cp ? cp->cb : nullptr, tfirst, tcount, tbase, cp ? cp->cnt : tU, sorted, c.tile_keys.p,
+                       tfirst, tcount, cp ? cp->start : tbase, cp ? cp->cnt : tU, sorted, c.tile_keys.p,
                        c.tile_mat.p, ophi, oplo, mhi, mlo, (c.learned && !cp) ? c.tile_p.p : nullptr, kr,
                        int64_t(c.t.ncells), ksplit > 1 ? Lpart : nullptr, Lout);
   } else {
@@ -1050,12 +1079,15 @@ static int cu_count(int device) {
 // Stream-K launch: all rows per workgroup (R <= kGemmMaxRT), then the
 // fixed-order combine of each tile's segments.
 template <int RH>
-static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int KPb, int kr, double *Lout) {
+static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int KPb, int kr, double *Lout,
+                           const ClassPass *cp) {
   const int ts = c.m2l_tstride;
   const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ts, *tbase = c.tiles.p + 2 * ts, *tU = c.tiles.p + 3 * ts;
   const int *sorted = c.tiles.p + 4 * ts;
   const int G = cu_count(c.cfg.device);
   const int64_t S = c.m2l_totalU;
+  const Geom go = cp ? geom(cp->p) : g;
+  const OrderPass op{go.P, go.R, go.K32, cp ? cp->perm : nullptr};
   c.m2l_part.ensure(size_t(G + ntiles) * kTN * g.PP);
   const v8bf *ophi = reinterpret_cast<const v8bf *>(c.ops32.p);
   const size_t per = size_t(g.K32) * g.R * 64 * 8;
@@ -1063,25 +1095,27 @@ static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int
       reinterpret_cast<const v8bf *>(reinterpret_cast<const __bf16 *>(c.ops32.p) + per * c.lists.nkeys);
   const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
   const __bf16 *mlo = mhi + bf16_cast_elems(g, sc.t.ncells);
-  const size_t lds = gemm_lds(g, true, g.R, kr);
+  const size_t lds = gemm_lds(go, true, go.R, kr);
+  const int *pre = cp ? cp->pre : tbase, *start = cp ? cp->start : tbase, *cnt = cp ? cp->cnt : tU;
+  const int *Sdev = cp ? cp->tot : nullptr;
   XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3_sk<RH>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-  hipLaunchKernelGGL((k_m2l_bf16x3_sk<RH>), dim3(G), dim3(kGemmThreads), lds, c.s0, g.PP, g.R, g.K32, KPb, g.P, ntiles, S,
-                     tfirst, tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi, mlo,
-                     c.learned ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), c.m2l_part.p);
+  hipLaunchKernelGGL((k_m2l_bf16x3_sk<RH>), dim3(G), dim3(kGemmThreads), lds, c.s0, g.PP, g.R, g.K32, KPb, op, ntiles,
+                     S, Sdev, tfirst, tcount, pre, start, cnt, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi,
+                     mlo, (c.learned && !cp) ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), c.m2l_part.p);
   XF_LAUNCH();
-  hipLaunchKernelGGL(k_sk_combine, dim3(ceil_div(int64_t(kTN) * g.PP, 256), ntiles), dim3(256), 0, c.s0, g.PP,
-                     ntiles, S, G, tfirst, tcount, tbase, tU, sorted, c.m2l_part.p, Lout);
+  const int rows = std::min(g.PP, 16 * go.R);
+  hipLaunchKernelGGL(k_sk_combine, dim3(ceil_div(int64_t(kTN) * rows, 256), ntiles), dim3(256), 0, c.s0, g.PP, rows,
+                     ntiles, S, Sdev, G, tfirst, tcount, pre, cnt, sorted, c.m2l_part.p, Lout);
   XF_LAUNCH();
 }
 
 static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, GemmShape sh,
                      double *Lout, const ClassPass *cp = nullptr) {
   if (sh.sk) {
-    if (cp) fail(EXAFMM_ERR_UNSUPPORTED, "stream-K GEMM over one order's slots");
-    switch (g.R) {
+    switch (cp ? geom(cp->p).R : g.R) {
 #define XF_GEMM_SK(n) \
-  case n: launch_gemm_sk<n>(c, sc, g, ntiles, KPb, sh.kr, Lout); return;
+  case n: launch_gemm_sk<n>(c, sc, g, ntiles, KPb, sh.kr, Lout, cp); return;
       XF_GEMM_SK(1)
       XF_GEMM_SK(2)
       XF_GEMM_SK(3)
@@ -1164,9 +1198,9 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
     }
   }
   // stream-K (BF16x3, all rows per workgroup)
-  if (bf16 && !cp && g.R <= kGemmMaxRT && std::getenv("EXAFMM_GEMM_NO_SK") == nullptr)
+  if (bf16 && go.R <= kGemmMaxRT && std::getenv("EXAFMM_GEMM_NO_SK") == nullptr)
     for (int kr = 1; kr <= 3; kr++)
-      if (rows_per_block(g, bf16, c.cfg.device, kr) >= g.R) cand.push_back(GemmShape{g.R, kr, 1, 1});
+      if (rows_per_block(go, bf16, c.cfg.device, kr) >= go.R) cand.push_back(GemmShape{go.R, kr, 1, 1});
   GemmShape best = cand[0];
   if (std::getenv("EXAFMM_GEMM_SK_ONLY"))  // the stream-K candidates alone (tests, A/B timing)
     for (const GemmShape &sh : cand)
@@ -1376,13 +1410,18 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
     const int pmin = kMlpPmin, ncls = std::max(1, c.P - kMlpPmin + 1);
     const int ts = c.m2l_tstride;
     c.tile_perm.ensure(std::max(c.m2l_totalU, 1));
-    c.tile_cls.ensure(size_t(2) * ncls * ts);
+    // tile_cls: [start | count | prefix] x order x tile stride, then the totals
+    c.tile_cls.ensure(size_t(3) * ncls * ts + ncls);
     hipLaunchKernelGGL(k_order_partition, dim3(ceil_div(ntiles, 4)), dim3(256), 0, s, ntiles, ts, c.tiles.p + 2 * ts,
                        c.tiles.p + 3 * ts, c.tile_p.p, pmin, ncls, c.tile_perm.p, c.tile_cls.p);
     XF_LAUNCH();
+    hipLaunchKernelGGL(k_order_prefix, dim3(1), dim3(64 * ncls), 0, s, ntiles, ts, ncls, c.tile_cls.p);
+    XF_LAUNCH();
     if (!c.capturing) XF_HIP(hipEventRecord(c.ev[13], s));
     for (int o = ncls - 1; o >= 0; o--) {
-      const ClassPass cp{pmin + o, c.tile_perm.p, c.tile_cls.p + size_t(o) * ts, c.tile_cls.p + size_t(ncls + o) * ts};
+      const int *cls = c.tile_cls.p;
+      const ClassPass cp{pmin + o, c.tile_perm.p, cls + size_t(o) * ts, cls + size_t(ncls + o) * ts,
+                         cls + size_t(2 * ncls + o) * ts, cls + size_t(3 * ncls) * ts + o};
       const GemmShape sh = gemm_shape(c, sc, g, ntiles, bf16, KP, KPb, &cp);
       run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L.p, &cp);
     }
