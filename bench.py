@@ -429,7 +429,7 @@ def learned_workload(args, fmm, torch):
         if name == "fixed":
             f = fmm.FMM(P=args.P_fixed, ncrit=args.ncrit, theta=args.theta, m2l_precision=prec)
         else:
-            f = fmm.FMM(P=args.P_fixed, ncrit=args.ncrit, theta=args.theta, m2l_precision=prec,
+            f = fmm.FMM(P=args.P_learned, ncrit=args.ncrit, theta=args.theta, m2l_precision=prec,
                         order_policy=fmm.ORDER_LEARNED)
             f.set_order_model(weights, args.tol)
         upload(dev, host, torch)
@@ -450,7 +450,8 @@ def learned_workload(args, fmm, torch):
         "warmup": args.warmup, "ms_per_step": L["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "fp32", "data": "synthetic",
         "config": {"workload": f"learned per-pair M2L order, N={n}: 90% Plummer (a=0.05, r<=0.5) + 10% uniform, "
-                               f"q~U[-1,1), orders 4..{args.P_fixed}, fallback P={args.P_fixed}, theta={args.theta}, "
+                               f"q~U[-1,1), learned orders 4..{args.P_learned} (expansions held to order "
+                               f"{args.P_learned}) against fixed order {args.P_fixed}, theta={args.theta}, "
                                f"ncrit={args.ncrit}",
                    "m2l_precision": args.m2l, "p2p_precision": "fp32", "order_policy": "learned",
                    "model": "MLP 4-32-32-11, ReLU, BF16 weights, FP32 accumulate", "log10_tolerance": args.tol},
@@ -467,14 +468,14 @@ def learned_workload(args, fmm, torch):
 def batched_workload(args, fmm, torch):
     """256 independent problems (N log-uniform in [10^3, 10^5], even-indexed
     uniform cubes, odd-indexed Plummer spheres, q = U[0,1)/N, P drawn from {6, 8, 12},
-    theta = 0.4, ncrit = 64), bodies resident in one device buffer.  Each
-    problem is planned once by an ordinary evaluation (tree, lists, keyed-M2L
-    tiles and operators, autotuned GEMM shape); one step is then one
-    hipGraph launch that re-evaluates all 256 problems (every charge-dependent
-    pass, p and F overwritten) with one grouped launch per stage over all
-    problems.  The planning time, one eager pass over all problems (tree
-    build included) and the same graph built from per-problem launches are
-    reported beside it."""
+    theta = 0.4, ncrit = 64), bodies resident in one device buffer.  One step
+    is one exafmm_evaluate_batch call with EXAFMM_BATCH_FOREST: the problems of
+    each order are gathered and evaluated as one forest -- tree builds,
+    traversals and every pass of all problems, with one launch per pass --
+    and copied back.  The first (cold) call, with context creation and the
+    GEMM autotuner, is reported beside the warm steps; so is the grouped
+    hipGraph that re-evaluates the charge-dependent passes of per-problem
+    contexts whose trees were built beforehand (positions frozen)."""
     from minimal_amd.inputs import BODY_DTYPE, batched_problems, rel_l2, sample_targets
     probs = batched_problems(256, 3)
     ns = [len(b) for b, _ in probs]
@@ -483,8 +484,49 @@ def batched_workload(args, fmm, torch):
     host = np.concatenate([b for b, _ in probs])
     dev = to_device(host, torch)
     base = dev.data_ptr()
-    log(f"bench batched: {len(probs)} problems, N total {N}, P counts "
-        f"{ {P: sum(1 for _, q in probs if q == P) for P in (6, 8, 12)} }")
+    counts = {P: sum(1 for _, q in probs if q == P) for P in (6, 8, 12)}
+    log(f"bench batched: {len(probs)} problems, N total {N}, P counts {counts}")
+    spec = [(base + int(o) * 64, nb, P) for o, nb, (_, P) in zip(offs[:-1], ns, probs)]
+    fb = fmm.FMM(ncrit=args.ncrit, theta=args.theta)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    fb.evaluate_batch(spec, forest=True)
+    torch.cuda.synchronize()
+    cold_ms = (time.perf_counter() - t0) * 1e3
+    for _ in range(args.warmup):
+        fb.evaluate_batch(spec, forest=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fb.evaluate_batch(spec, forest=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    log(f"forest batch: cold {cold_ms:.1f} ms, step {ms:.3f} ms ({N / (ms * 1e-3):.3e} particles/s)")
+    # accuracy of one call on fresh bodies (p and F are accumulated, so the
+    # timed calls' sums are not checked) on 24 problems (both kinds, every
+    # P), FP64 direct sum on sampled targets
+    upload(dev, host, torch)
+    fb.evaluate_batch(spec, forest=True)
+    torch.cuda.synchronize()
+    res = dev.cpu().numpy().view(BODY_DTYPE)
+    errs = {}
+    chk = fmm.FMM(P=6)
+    for i in range(0, 256, 11):  # both kinds (even uniform, odd Plummer), every order
+        b = res[offs[i]:offs[i + 1]]
+        idx = sample_targets(len(b), 100)
+        ref = b[idx].copy()
+        ref["p"] = 0
+        ref["F"] = 0
+        chk.direct(ref, b.copy())
+        P = probs[i][1]
+        e = errs.setdefault(P, [0.0, 0.0, 0])
+        e[0] = max(e[0], rel_l2(b["p"][idx], ref["p"]))
+        e[1] = max(e[1], rel_l2(b["F"][idx], ref["F"]))
+        e[2] += 1
+    log("max rel L2 per order:", errs)
+    # the grouped graph over per-problem contexts (charges only; trees and
+    # lists built once, outside the timed replays)
+    upload(dev, host, torch)
     t0 = time.perf_counter()
     ctxs = []
     for (b, P), o in zip(probs, offs[:-1]):
@@ -495,75 +537,40 @@ def batched_workload(args, fmm, torch):
         f.synchronize()
     plan_s = time.perf_counter() - t0
     t0 = time.perf_counter()
-    for f, nb, o in zip(ctxs, ns, offs[:-1]):
-        f.evaluate(base + int(o) * 64, n=nb, on_device=True)
-    for f in ctxs:
-        f.synchronize()
-    eager_ms = (time.perf_counter() - t0) * 1e3
-    def graph_steps(grouped):
-        t0 = time.perf_counter()
-        g = fmm.Graph(ctxs, streams=args.streams, grouped=grouped)
-        create_s = time.perf_counter() - t0
-        for _ in range(args.warmup):
-            g.launch()
-        g.synchronize()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            g.launch()
-        g.synchronize()
-        torch.cuda.synchronize()
-        return g, create_s, (time.perf_counter() - t0) * 1e3 / args.steps
-
-    # per-problem launches on parallel branches (reported), then one launch
-    # per stage over all problems (the measured form; its results are checked)
-    g, create_pp, ms_pp = graph_steps(False)
-    nodes_pp = g.nodes()
-    g.close()
-    g, create_s, ms = graph_steps(True)
+    g = fmm.Graph(ctxs, streams=args.streams, grouped=True)
+    create_s = time.perf_counter() - t0
+    for _ in range(args.warmup):
+        g.launch()
+    g.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.launch()
+    g.synchronize()
+    torch.cuda.synchronize()
+    ms_g = (time.perf_counter() - t0) * 1e3 / args.steps
     nodes = g.nodes()
-    shapes = g.gemm_shapes()
-    log("autotuned GEMM shapes (P, rows, keys per round):", shapes)
-    log(f"plan {plan_s:.2f} s, eager pass {eager_ms:.2f} ms, per-problem graph {ms_pp:.3f} ms ({nodes_pp} nodes), "
-        f"grouped graph create {create_s:.2f} s ({nodes} nodes), step {ms:.3f} ms")
-    # accuracy on 24 problems (both kinds, every P), FP64 direct sum on sampled targets
-    res = dev.cpu().numpy().view(BODY_DTYPE)
-    errs = {}
-    for i in range(0, 256, 11):  # both kinds (even uniform, odd Plummer), every order
-        b = res[offs[i]:offs[i + 1]]
-        idx = sample_targets(len(b), 100)
-        ref = b[idx].copy()
-        ref["p"] = 0
-        ref["F"] = 0
-        ctxs[i].direct(ref, b.copy())
-        P = probs[i][1]
-        e = errs.setdefault(P, [0.0, 0.0, 0])
-        e[0] = max(e[0], rel_l2(b["p"][idx], ref["p"]))
-        e[1] = max(e[1], rel_l2(b["F"][idx], ref["F"]))
-        e[2] += 1
-    log("max rel L2 per order:", errs)
+    log(f"grouped graph: plan {plan_s:.2f} s, create {create_s:.2f} s ({nodes} nodes), replay {ms_g:.3f} ms")
+    g.close()
     out = {
-        "metric": "batched 3D Laplace FMM particles/s, 256 problems in one hipGraph",
+        "metric": "batched 3D Laplace FMM particles/s, 256 problems, tree and lists included",
         "value": N / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic",
         "config": {"workload": f"batched FMM: 256 problems, N log-uniform in [1e3,1e5] (total {N}), even-indexed "
-                               f"uniform cubes, odd-indexed Plummer (a=0.1, r<=1), q=U[0,1)/N, P in {{6,8,12}}, "
-                               f"theta={args.theta}, "
-                               f"ncrit={args.ncrit}; one step = one hipGraph launch re-evaluating all problems "
-                               f"with one grouped launch per stage",
-                   "m2l_precision": "bf16x3", "p2p_precision": "fp32", "graph_nodes": nodes,
-                   "gemm_shapes_autotuned": [{"P": a, "rows": b, "keys_per_round": c} for a, b, c in shapes]},
-        "plan_s": plan_s, "graph_create_s": create_s, "eager_pass_ms": eager_ms,
-        "eager_value": N / (eager_ms * 1e-3),
-        "per_problem_graph": {"ms_per_step": ms_pp, "value": N / (ms_pp * 1e-3), "nodes": nodes_pp,
-                              "streams": args.streams},
+                               f"uniform cubes, odd-indexed Plummer (a=0.1, r<=1), q=U[0,1)/N, P in {{6,8,12}} "
+                               f"({counts}), theta={args.theta}, ncrit={args.ncrit}; one step = one "
+                               f"exafmm_evaluate_batch(EXAFMM_BATCH_FOREST) call: per order one forest (tree "
+                               f"build, traversal, every pass, one launch per pass over its problems)",
+                   "m2l_precision": "bf16x3", "p2p_precision": "fp32"},
+        "cold_ms": cold_ms,
         "error": {"max_rel_l2_by_P": {str(k): {"p": v[0], "F": v[1], "problems": v[2]} for k, v in errs.items()},
                   "targets_per_problem": 100, "vs": "FP64 direct sum"},
+        "grouped_graph_charges_only": {"ms_per_step": ms_g, "value": N / (ms_g * 1e-3), "nodes": nodes,
+                                       "plan_s": plan_s, "create_s": create_s},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
-    g.close()
 
 
 def periodic_workload(args, fmm, torch):
@@ -639,13 +646,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=0, help="bodies (default: the workload's size)")
     ap.add_argument("--P", type=int, default=10)
-    ap.add_argument("--P-fixed", dest="P_fixed", type=int, default=12, help="learned workload: fallback order")
-    ap.add_argument("--tol", type=float, default=-7.5, help="learned workload: log10 pair-error tolerance")
+    ap.add_argument("--P-fixed", dest="P_fixed", type=int, default=12, help="learned workload: the fixed order compared")
+    ap.add_argument("--P-learned", dest="P_learned", type=int, default=14,
+                    help="learned workload: highest learned order (the learned context's P)")
+    ap.add_argument("--tol", type=float, default=-10.0,
+                    help="learned workload: log10 of the allowed M2L error per pair, relative to sum|q| / R0")
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--m2l", choices=["bf16x3", "fp32", "fp64"], default="bf16x3")
     ap.add_argument("--no-learned", action="store_true", help="plain workload: skip the learned-order comparison")
-    ap.add_argument("--plain-learned-tol", type=float, default=-7.0,
+    ap.add_argument("--plain-learned-tol", type=float, default=-9.5,
                     help="plain workload: log10 tolerance of the learned-order comparison")
     ap.add_argument("--p2p", choices=["fp32", "fp32_newton"], default="fp32_newton",
                     help="plain workload: P2P precision (fp32_newton, the stated arithmetic of the N=2^20 "

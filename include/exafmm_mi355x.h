@@ -138,6 +138,19 @@ int exafmm_downward(exafmm_ctx *ctx);
 /* buildTree + upwardPass + horizontalPass(self) + downwardPass in one call
  * (3d/fmm.cxx:40-53). */
 int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_device);
+/* exafmm_evaluate of nprob (<= 512) independent free-space problems in one
+ * call: problem p is bodies[offsets[p] .. offsets[p+1]-1] (offsets[0] = 0,
+ * offsets[nprob] = total), with its own root cube (3d/build_tree.h:7-21)
+ * and tree (3d/build_tree.h:88-96) and no interaction with the others.
+ * The trees hang under three levels of cells indexed by the problem number
+ * (never paired by the traversal), so one tree build, one traversal and one
+ * launch per pass serve all problems, and the keyed M2L packs the cells of
+ * every problem into shared tiles with one operator per translation key
+ * (keys are scale-free).  Each problem's X and q are permuted within its
+ * own range as exafmm_evaluate permutes them.  No counterpart in the
+ * reference (one problem per run, 3d/fmm.cxx:40-53). */
+int exafmm_evaluate_forest(exafmm_ctx *ctx, exafmm_body *bodies, const int64_t *offsets, int32_t nprob,
+                           int on_device);
 
 /* direct (3d/traverse_eager.h:193-202): FP64 direct sum of every source on
  * every target, accumulated into the targets' p and F. */
@@ -293,6 +306,11 @@ typedef struct exafmm_problem {
   int32_t on_device;
 } exafmm_problem;
 #define EXAFMM_BATCH_REUSE_GEOMETRY 1
+/* EXAFMM_BATCH_FOREST: the problems of each order are gathered into one
+ * array and evaluated as one forest (exafmm_evaluate_forest), then copied
+ * back: every call builds the trees and lists of all problems afresh, with
+ * one launch per pass over all problems of an order. */
+#define EXAFMM_BATCH_FOREST 2
 int exafmm_evaluate_batch(exafmm_ctx *ctx, const exafmm_problem *problems, int count, int flags);
 
 #ifdef __cplusplus

@@ -17,6 +17,9 @@ struct exafmm_ctx {
   std::vector<exafmm_ctx *> batch;
   exafmm_graph *batch_graph = nullptr;
   std::vector<std::pair<const void *, int64_t>> batch_sig;
+  // EXAFMM_BATCH_FOREST: one forest context per order and its gathered bodies
+  std::vector<exafmm_ctx *> forests;
+  std::vector<std::unique_ptr<xf::Buf<exafmm_body>>> forest_bodies;
 };
 
 namespace xf {
@@ -77,6 +80,7 @@ void set_bodies(Ctx &c, exafmm_body *b, int64_t n, int on_device) {
   if (c.graph_locks) fail(EXAFMM_ERR_STATE, "the context is held by a graph (exafmm_graph_destroy it first)");
   if (!b || n < 1) fail(EXAFMM_ERR_ARG, "need at least one body");
   c.t.n = n;
+  c.forest_n = 0;  // one problem unless exafmm_evaluate_forest says otherwise
   if (on_device) {
     c.bodies = b;
     c.host_mode = false;
@@ -251,6 +255,9 @@ int exafmm_destroy(exafmm_ctx *ctx) {
   ctx->batch_graph = nullptr;
   for (exafmm_ctx *b : ctx->batch) exafmm_destroy(b);
   ctx->batch.clear();
+  for (exafmm_ctx *f : ctx->forests) exafmm_destroy(f);
+  ctx->forests.clear();
+  ctx->forest_bodies.clear();
   hipSetDevice(ctx->c.cfg.device);
   if (ctx->c.s0) hipStreamSynchronize(ctx->c.s0);
   if (ctx->c.s1) hipStreamSynchronize(ctx->c.s1);
@@ -335,6 +342,33 @@ int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_devi
     // blocks filling the CUs those latency-bound chains leave idle; the
     // keyed GEMM follows on s0.  The upward pass is enqueued first so the
     // P2M workgroups are dispatched ahead of P2P's.
+    rec(c, 14, c.s0);
+    do_upward(c);
+    do_p2p(c, c, true);
+    do_far(c, c);
+    do_downward(c);
+    bodies_to_host(c);
+  });
+}
+
+int exafmm_evaluate_forest(exafmm_ctx *ctx, exafmm_body *bodies, const int64_t *offsets, int32_t nprob,
+                           int on_device) {
+  return guard(ctx, [&] {
+    if (!ctx || !offsets) fail(EXAFMM_ERR_ARG, "null argument");
+    if (nprob < 1 || nprob > kForestMax) fail(EXAFMM_ERR_ARG, "number of problems must be in [1, 512]");
+    if (offsets[0] != 0) fail(EXAFMM_ERR_ARG, "offsets[0] must be 0");
+    for (int p = 0; p < nprob; p++)
+      if (offsets[p + 1] < offsets[p]) fail(EXAFMM_ERR_ARG, "offsets must not decrease");
+    Ctx &c = ctx->c;
+    if (c.cfg.images > 0) fail(EXAFMM_ERR_UNSUPPORTED, "a forest evaluation is free-space only");
+    set_bodies(c, bodies, offsets[nprob], on_device);
+    c.forest_n = nprob;
+    c.forest_off.ensure(nprob + 1);
+    c.forest_roots.ensure(nprob);
+    // synchronous upload: the caller's offsets need not outlive the call
+    XF_HIP(hipMemcpy(c.forest_off.p, offsets, sizeof(int64_t) * (nprob + 1), hipMemcpyHostToDevice));
+    do_build(c);
+    do_lists(c, c);
     rec(c, 14, c.s0);
     do_upward(c);
     do_p2p(c, c, true);
@@ -1119,6 +1153,66 @@ int exafmm_graph_destroy(exafmm_graph *g) {
 }
 
 
+// EXAFMM_BATCH_FOREST: the problems of each order, in their given order and
+// at most kForestMax per forest, are gathered into one device array and
+// evaluated as one forest (exafmm_evaluate_forest: one tree build, one
+// traversal and one launch per pass over all of them), then copied back.
+static void batch_forests(exafmm_ctx *ctx, const exafmm_problem *problems, int count) {
+  using namespace xf;
+  std::vector<std::vector<int>> groups;  // problem indices, by order, chunks of kForestMax
+  {
+    std::vector<int> idx(count);
+    for (int i = 0; i < count; i++) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return problems[a].P < problems[b].P; });
+    for (int i : idx) {
+      if (groups.empty() || problems[groups.back()[0]].P != problems[i].P ||
+          int(groups.back().size()) == kForestMax)
+        groups.emplace_back();
+      groups.back().push_back(i);
+    }
+  }
+  while (ctx->forests.size() < groups.size()) {
+    exafmm_config cfg = ctx->c.cfg;
+    exafmm_ctx *sub = nullptr;
+    if (exafmm_create(&cfg, &sub) != EXAFMM_OK) fail(EXAFMM_ERR_ALLOC, g_error);
+    ctx->forests.push_back(sub);
+  }
+  while (ctx->forest_bodies.size() < ctx->forests.size())
+    ctx->forest_bodies.push_back(std::make_unique<Buf<exafmm_body>>());
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    const std::vector<int> &g = groups[gi];
+    const int P = problems[g[0]].P;
+    if (ctx->forests[gi]->c.P != P) {  // another order in this slot: a new context
+      exafmm_destroy(ctx->forests[gi]);
+      exafmm_config cfg = ctx->c.cfg;
+      cfg.P = P;
+      ctx->forests[gi] = nullptr;
+      if (exafmm_create(&cfg, &ctx->forests[gi]) != EXAFMM_OK) fail(EXAFMM_ERR_ALLOC, g_error);
+    }
+    Ctx &fc = ctx->forests[gi]->c;
+    std::vector<int64_t> off(g.size() + 1, 0);
+    for (size_t k = 0; k < g.size(); k++) off[k + 1] = off[k] + problems[g[k]].n;
+    Buf<exafmm_body> &buf = *ctx->forest_bodies[gi];
+    buf.ensure(off.back());
+    for (size_t k = 0; k < g.size(); k++) {
+      const exafmm_problem &q = problems[g[k]];
+      XF_HIP(hipMemcpyAsync(buf.p + off[k], q.bodies, sizeof(exafmm_body) * q.n,
+                            q.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, fc.s0));
+    }
+    if (exafmm_evaluate_forest(ctx->forests[gi], buf.p, off.data(), int(g.size()), 1) != EXAFMM_OK)
+      fail(EXAFMM_ERR_HIP, std::string("forest of order ") + std::to_string(P) + ": " + fc.last_error);
+    for (size_t k = 0; k < g.size(); k++) {
+      const exafmm_problem &q = problems[g[k]];
+      XF_HIP(hipMemcpyAsync(q.bodies, buf.p + off[k], sizeof(exafmm_body) * q.n,
+                            q.on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, fc.s0));
+    }
+  }
+  for (exafmm_ctx *f : ctx->forests) {
+    XF_HIP(hipStreamSynchronize(f->c.s1));
+    XF_HIP(hipStreamSynchronize(f->c.s0));
+  }
+}
+
 int exafmm_evaluate_batch(exafmm_ctx *ctx, const exafmm_problem *problems, int count, int flags) {
   return guard(ctx, [&] {
     if (!ctx || !problems || count < 1) fail(EXAFMM_ERR_ARG, "bad argument");
@@ -1129,6 +1223,11 @@ int exafmm_evaluate_batch(exafmm_ctx *ctx, const exafmm_problem *problems, int c
       if (!q.bodies || q.n < 1) fail(EXAFMM_ERR_ARG, "a problem without bodies");
       if (q.P < 2 || q.P > kMaxP) fail(EXAFMM_ERR_ARG, "P must be in [2, 16]");
       sig[i] = {q.bodies, (q.n << 8) | (int64_t(q.P) << 2) | (q.on_device ? 1 : 0)};
+    }
+    if (flags & EXAFMM_BATCH_FOREST) {
+      if (reuse) fail(EXAFMM_ERR_ARG, "EXAFMM_BATCH_FOREST and EXAFMM_BATCH_REUSE_GEOMETRY exclude each other");
+      batch_forests(ctx, problems, count);
+      return;
     }
     if (reuse && ctx->batch_graph && sig == ctx->batch_sig) {  // same problems, positions kept
       XF_HIP(hipSetDevice(ctx->c.cfg.device));

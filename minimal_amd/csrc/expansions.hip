@@ -57,7 +57,7 @@ __device__ __attribute__((always_inline)) inline void p2m_body(int bx, int P, co
   const int c = leaves[li];
   const int beg = cbeg[c], cnt = ccnt[c];
   const double4 C = cx[c];
-  const double invR = 1.0 / C.w;
+  const double invR = C.w > 0.0 ? 1.0 / C.w : 0.0;  // R = 0: every body at the centre
   const int m = lane & 15, g = lane >> 4;
   // per-lane constants (no divisions per body): 1/(2^m m!) and the
   // recurrence factors -1/((n+m)(n-m)), n = m+2..P-1
@@ -645,7 +645,7 @@ __device__ __attribute__((always_inline)) inline void l2p_body(int bx, int P, co
   if (li >= *nleaves) return;
   const int c = leaves[li];
   const double4 C = cx[c];
-  const double invR = 1.0 / C.w;
+  const double invR = C.w > 0.0 ? 1.0 / C.w : 0.0;  // R = 0: every body at the centre
   l2p_stage<T>(P, threadIdx.x, 64, L + int64_t(c) * P * P, invR, Lh, rec);
   __syncthreads();
   const int beg = cbeg[c], cnt = ccnt[c];
@@ -695,7 +695,7 @@ __global__ void __launch_bounds__(64) k_l2p_flat(int P, int64_t n, const int *__
     C = cx[leaf];
     B = xs[b];
   }
-  const double invR = 1.0 / C.w;
+  const double invR = C.w > 0.0 ? 1.0 / C.w : 0.0;  // R = 0: every body at the centre
   for (int g0 = 0; g0 < nruns; g0 += kL2PSlots) {
     const int ns = min(kL2PSlots, nruns - g0);
     if (valid && run >= g0 && run < g0 + ns && (lane == 0 || prev != leaf)) sleaf[run - g0] = leaf;
@@ -704,7 +704,8 @@ __global__ void __launch_bounds__(64) k_l2p_flat(int P, int64_t n, const int *__
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     for (int e = lane; e < ns * NT; e += kWave) {
       const int sl = e / NT, k = e - sl * NT, c = sleaf[sl];
-      l2p_stage<float>(P, k, NT, L + int64_t(c) * P * P, 1.0 / cx[c].w, Lh[sl], nullptr);
+      const double Rc = cx[c].w;
+      l2p_stage<float>(P, k, NT, L + int64_t(c) * P * P, Rc > 0.0 ? 1.0 / Rc : 0.0, Lh[sl], nullptr);
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -788,7 +789,7 @@ void upward(Ctx &c) {
     default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
   }
   XF_LAUNCH();
-  for (int l = t.depth - 1; l >= 0; l--) {
+  for (int l = t.depth - 1; l >= t.lroot; l--) {  // (a forest stops at its problems' roots)
     const int cnt = t.lstart[l + 1] - t.lstart[l];
     hipLaunchKernelGGL(k_m2m_rows, dim3(ceil_div(cnt, 16)), dim3(row_tile_waves(PP) * kWave), 0, s, c.P,
                        t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
@@ -802,7 +803,7 @@ void l2l_pass(Ctx &c) {
   hipStream_t s = c.s0;
   const int PP = c.P * c.P;
   shift_ops(c);
-  for (int l = 0; l < t.depth; l++) {  // parents at level l
+  for (int l = t.lroot; l < t.depth; l++) {  // parents at level l
     const int cnt = t.lstart[l + 1] - t.lstart[l];
     hipLaunchKernelGGL(k_l2l_rows, dim3(ceil_div(cnt, 16)), dim3(row_tile_waves(PP) * kWave), 0, s, c.P,
                        t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.L.p);

@@ -13,6 +13,11 @@
 namespace xf {
 
 constexpr int kMaxLevel = 21;     // 63-bit octant keys
+// forest of independent problems in one tree: levels 0 .. kForestLevel - 1
+// hold the problem index (up to 8^kForestLevel problems), the problems'
+// roots are the cells of level kForestLevel
+constexpr int kForestLevel = 3;
+constexpr int kForestMax = 1 << (3 * kForestLevel);
 constexpr int kMaxP = 16;         // expansion order cap (NTERM <= 136)
 constexpr int kWave = 64;
 
@@ -119,6 +124,7 @@ struct Tree {
   int64_t ncells = 0, nleaves = 0;
   std::vector<int> lstart;       // host copy: level start, size depth+2
   double X0[3] = {0, 0, 0}, R0 = 0;
+  int lroot = 0;                 // level of the (problem) roots: kForestLevel in a forest
 
   Buf<double4> cx;     // centre xyz, radius R
   Buf<int> cbeg, ccnt; // body range in tree order
@@ -178,6 +184,11 @@ struct Ctx {
   Tree t;
   Lists lists;
   const Ctx *src = nullptr;        // source tree of the last horizontal pass
+  // forest evaluation (exafmm_evaluate_forest): forest_n problems, bodies
+  // forest_off[p] .. forest_off[p+1] - 1, their roots; 0 = one problem
+  int forest_n = 0;
+  Buf<int64_t> forest_off;
+  Buf<double4> forest_roots;
   // expansions, radius-scaled real form [ncells][P^2]:
   //   M~ = (Re, Im) M_n^m / R^n,  L~ = (Re, Im) L_j^k R^(j+1)
   Buf<double> M, L;
@@ -218,6 +229,7 @@ struct Ctx {
   bool have_model = false;
   float order_log_tol = -5.0f;
   Buf<float> mlp_w, mlp_part, mshape;
+  Buf<double> absq;  // sum |q| per cell (order selection)
   int train_n = 0;
   Buf<float> train_X, train_T;
   std::vector<int> train_pairs;  // (target cell, source cell, image) per sample, internal numbering  // scratch of m2l_prepare (may run on the side stream)

@@ -52,6 +52,7 @@ struct TravArgs {
   unsigned long long *hslots;
   int hmask;
   int want_keys;
+  int self_start;  // first level: each cell starts from (itself, no shift) -- a forest's problem roots
 };
 
 __device__ inline unsigned long long mix64(unsigned long long x) {
@@ -131,6 +132,9 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
       return;
     }
     for (int i = lane; i < tail; i += kWave) q[i] = a.in[ib + i];
+  } else if (a.self_start) {
+    tail = 1;
+    if (lane == 0) q[0] = (c << 5) | 13;
   } else if (a.periodic) {
     tail = 27;
     if (lane < 27) q[lane] = lane;  // (root, image) for the 27 near images
@@ -344,10 +348,20 @@ void traverse(Ctx &tc, const Ctx &sc) {
     a.hslots = Lr.hslots.p;
     a.hmask = Lr.hcap - 1;
     a.want_keys = keys;
-    for (int l = 0; l <= T.depth; l++) {
+    // a forest starts at its problems' roots (level lroot): the cells above
+    // are never paired, their lists stay empty
+    const int l0 = T.lroot;
+    if (l0 > 0 && &tc != &sc) fail(EXAFMM_ERR_UNSUPPORTED, "a forest's horizontal pass needs source == target");
+    a.self_start = l0 > 0;
+    if (l0 > 0) {
+      const size_t nv = size_t(T.lstart[l0]);
+      for (int *z : {Lr.m2l_cnt.p, Lr.m2l_beg.p, Lr.p2p_cnt.p, Lr.p2p_beg.p, Lr.out_cnt.p, Lr.out_beg.p})
+        XF_HIP(hipMemsetAsync(z, 0, sizeof(int) * nv, s));
+    }
+    for (int l = l0; l <= T.depth; l++) {
       a.lstart = T.lstart[l];
       a.lcount = T.lstart[l + 1] - T.lstart[l];
-      a.in = l ? Lr.out.p : nullptr;
+      a.in = l > l0 ? Lr.out.p : nullptr;
       a.in_beg = Lr.out_beg.p;
       a.in_cnt = Lr.out_cnt.p;
       const int g = ceil_div(a.lcount, kTravWaves);

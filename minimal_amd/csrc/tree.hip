@@ -97,23 +97,94 @@ __global__ void __launch_bounds__(256) k_bounds_final(const double *__restrict__
   root[3] = R0;
 }
 
+// Forest: one workgroup per problem (bodies off[p] .. off[p+1] - 1), the
+// root of k_bounds_final per problem (min / max are exact, so any reduction
+// order gives the same root).  Empty problems get a unit root.
+__global__ void __launch_bounds__(256) k_forest_bounds(const exafmm_body *__restrict__ b,
+                                                       const int64_t *__restrict__ off,
+                                                       double4 *__restrict__ roots, int *__restrict__ flags) {
+  __shared__ double s[6][256];
+  const int p = blockIdx.x, t = threadIdx.x;
+  const int64_t i0 = off[p], i1 = off[p + 1];
+  if (i1 <= i0) {
+    if (t == 0) roots[p] = make_double4(0.0, 0.0, 0.0, 1.0);
+    return;
+  }
+  double lo[3] = {b[i0].X[0], b[i0].X[1], b[i0].X[2]};
+  double hi[3] = {lo[0], lo[1], lo[2]};
+  int bad = 0;
+  for (int64_t i = i0 + t; i < i1; i += 256) {
+    for (int d = 0; d < 3; d++) {
+      const double x = b[i].X[d];
+      if (!isfinite(x)) bad = 1;
+      lo[d] = fmin(x, lo[d]);
+      hi[d] = fmax(x, hi[d]);
+    }
+    if (!isfinite(b[i].q)) bad = 1;
+  }
+  if (bad) atomicOr(flags, 1);
+  for (int d = 0; d < 3; d++) {
+    s[d][t] = lo[d];
+    s[3 + d][t] = hi[d];
+  }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (t < o)
+      for (int d = 0; d < 3; d++) {
+        s[d][t] = fmin(s[d][t], s[d][t + o]);
+        s[3 + d][t] = fmax(s[3 + d][t], s[3 + d][t + o]);
+      }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  double X0[3], R0 = 0;
+  for (int d = 0; d < 3; d++) X0[d] = (s[3 + d][0] + s[d][0]) / 2;
+  for (int d = 0; d < 3; d++) {
+    R0 = fmax(X0[d] - s[d][0], R0);
+    R0 = fmax(s[3 + d][0] - X0[d], R0);
+  }
+  R0 *= 1.00001;
+  roots[p] = make_double4(X0[0], X0[1], X0[2], R0);
+}
+
 // ---------------------------------------------------------------- keys
 // Octant path of each body through the reference's centre sequence, and
 // the population of its level-7 prefix (dense, 8^7 bins: spread-out atomics).
 __global__ void __launch_bounds__(256) k_keys(const exafmm_body *__restrict__ b, int64_t n,
                                               const double *__restrict__ root, unsigned long long *__restrict__ key,
-                                              int *__restrict__ val, int *__restrict__ hist7) {
-  const double R0 = root[3];
+                                              int *__restrict__ val, int *__restrict__ hist7,
+                                              const double4 *__restrict__ froots, const int64_t *__restrict__ off,
+                                              int nprob) {
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
   const double x = b[i].X[0], y = b[i].X[1], z = b[i].X[2];
-  double cx = root[0], cy = root[1], cz = root[2];
+  double R0 = root[3], cx = root[0], cy = root[1], cz = root[2];
   unsigned long long k = 0;
-  for (int l = 0; l < kMaxLevel; l++) {
+  int l0 = 0;
+  if (froots) {
+    // forest: the problem's index fills the digits of levels 0 .. kForestLevel - 1,
+    // its own root's octant path the rest
+    int lo = 0, hi = nprob - 1;  // last problem with off[p] <= i
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= i)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const double4 Rt = froots[lo];
+    cx = Rt.x;
+    cy = Rt.y;
+    cz = Rt.z;
+    R0 = Rt.w;
+    k = (unsigned long long)lo;
+    l0 = kForestLevel;
+  }
+  for (int l = l0; l < kMaxLevel; l++) {
     const int bx = x > cx, by = y > cy, bz = z > cz;
     const int oct = bx + (by << 1) + (bz << 2);
     k = (k << 3) | unsigned(oct);
-    const double r = R0 / (1 << (l + 1));
+    const double r = R0 / (1 << (l - l0 + 1));
     cx += bx ? r : -r;
     cy += by ? r : -r;
     cz += bz ? r : -r;
@@ -227,8 +298,9 @@ __global__ void __launch_bounds__(512) k_hist_top(int *__restrict__ hist, const 
 // sorted keys: b[o] .. b[o + 1] are octant o's bodies.  Returns the number of
 // non-empty children (0: the cell is a leaf, b untouched).
 __device__ inline int split_bounds(int l, int ncrit, int beg, int cnt, const unsigned long long *__restrict__ key,
-                                   const int *__restrict__ hist, int b[9]) {
-  if (cnt <= ncrit || l >= kMaxLevel) return 0;
+                                   const int *__restrict__ hist, int b[9], int lforce) {
+  // levels above lforce (a forest's problem-index levels) always split
+  if ((cnt <= ncrit && l >= lforce) || cnt == 0 || l >= kMaxLevel) return 0;
   const int sh = 3 * (kMaxLevel - 1 - l);
   b[0] = beg;
   b[8] = beg + cnt;
@@ -256,7 +328,7 @@ __device__ inline int split_bounds(int l, int ncrit, int beg, int cnt, const uns
   return k;
 }
 
-__global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__restrict__ lcount,
+__global__ void __launch_bounds__(256) k_split(int l, int ncrit, int lforce, const int *__restrict__ lcount,
                                                const int *__restrict__ lstart, const int *__restrict__ cbeg,
                                                const int *__restrict__ ccnt,
                                                const unsigned long long *__restrict__ key,
@@ -266,7 +338,7 @@ __global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__re
   if (t >= lcount[l]) return;
   const int c = lstart[l] + t;
   int b[9];
-  const int k = split_bounds(l, ncrit, cbeg[c], ccnt[c], key, hist, b);
+  const int k = split_bounds(l, ncrit, cbeg[c], ccnt[c], key, hist, b, lforce);
   nch[t] = k;
   if (k == 0) return;
   for (int o = 0; o < 9; o++) bnd[t * 9 + o] = b[o];
@@ -274,12 +346,16 @@ __global__ void __launch_bounds__(256) k_split(int l, int ncrit, const int *__re
 
 // Writes the non-empty children of level-l cell c (boundaries b) to cells
 // first, first + 1, ... in octant order.
-__device__ inline void emit_children(int l, int c, int first, const int b[9], const double *__restrict__ root,
-                                     double4 *__restrict__ cx, int *__restrict__ cbeg, int *__restrict__ ccnt,
-                                     int *__restrict__ cchild, int *__restrict__ cparent, int *__restrict__ clevel) {
+// r = R_parent / 2 is R0 / 2^(l+1) exactly.  Forest: the cells of level
+// kForestLevel are the problems' roots (froots, by the problem index in the
+// top key digits).
+__device__ inline void emit_children(int l, int c, int first, const int b[9], const double4 *__restrict__ froots,
+                                     const unsigned long long *__restrict__ key, double4 *__restrict__ cx,
+                                     int *__restrict__ cbeg, int *__restrict__ ccnt, int *__restrict__ cchild,
+                                     int *__restrict__ cparent, int *__restrict__ clevel) {
   cchild[c] = first;
   const double4 P = cx[c];
-  const double r = root[3] / (1 << (l + 1));
+  const double r = P.w * 0.5;
   int j = 0;
   for (int o = 0; o < 8; o++) {
     const int b0 = b[o], b1 = b[o + 1];
@@ -290,6 +366,7 @@ __device__ inline void emit_children(int l, int c, int first, const int b[9], co
     X.y = P.y + ((o & 2) ? r : -r);
     X.z = P.z + ((o & 4) ? r : -r);
     X.w = r;
+    if (froots && l + 1 == kForestLevel) X = froots[int(key[b0] >> (3 * (kMaxLevel - kForestLevel)))];
     cx[ch] = X;
     cbeg[ch] = b0;
     ccnt[ch] = b1 - b0;
@@ -309,7 +386,8 @@ __global__ void k_level_next(int l, int *__restrict__ lcount, int *__restrict__ 
 // centre = parent centre + r*(+/-1), r = R0/2^(l+1); R = R0/2^(l+1).
 __global__ void __launch_bounds__(256) k_emit(int l, const int *__restrict__ lcount, const int *__restrict__ lstart,
                                               const int *__restrict__ nch, const int *__restrict__ off,
-                                              const int *__restrict__ bnd, const double *__restrict__ root,
+                                              const int *__restrict__ bnd, const double4 *__restrict__ froots,
+                                              const unsigned long long *__restrict__ key,
                                               double4 *__restrict__ cx, int *__restrict__ cbeg,
                                               int *__restrict__ ccnt, int *__restrict__ cchild,
                                               int *__restrict__ cnch, int *__restrict__ cparent,
@@ -329,7 +407,7 @@ __global__ void __launch_bounds__(256) k_emit(int l, const int *__restrict__ lco
   }
   int b[9];
   for (int o = 0; o < 9; o++) b[o] = bnd[t * 9 + o];
-  emit_children(l, c, lstart[l + 1] + off[t], b, root, cx, cbeg, ccnt, cchild, cparent, clevel);
+  emit_children(l, c, lstart[l + 1] + off[t], b, froots, key, cx, cbeg, ccnt, cchild, cparent, clevel);
 }
 
 // The top levels, 0 .. lend - 1, split by one workgroup: per level the
@@ -343,9 +421,9 @@ __global__ void __launch_bounds__(256) k_emit(int l, const int *__restrict__ lco
 constexpr int kTopThreads = 1024, kTopPer = 1, kTopLevels = 4;
 static_assert(1 << (3 * (kTopLevels - 1)) <= kTopThreads * kTopPer, "top levels exceed the workgroup");
 __global__ void __launch_bounds__(kTopThreads)
-    k_top_levels(int lend, int ncrit, int *__restrict__ lcount, int *__restrict__ lstart,
+    k_top_levels(int lend, int ncrit, int lforce, int *__restrict__ lcount, int *__restrict__ lstart,
                  const unsigned long long *__restrict__ key, const int *__restrict__ hist,
-                 const double *__restrict__ root, double4 *__restrict__ cx, int *__restrict__ cbeg,
+                 const double4 *__restrict__ froots, double4 *__restrict__ cx, int *__restrict__ cbeg,
                  int *__restrict__ ccnt, int *__restrict__ cchild, int *__restrict__ cnch,
                  int *__restrict__ cparent, int *__restrict__ clevel, int *__restrict__ nsplit) {
   __shared__ int wsum[kTopThreads / kWave];
@@ -360,7 +438,7 @@ __global__ void __launch_bounds__(kTopThreads)
       k[j] = 0;
       if (t < cnt) {
         const int c = start + t;
-        k[j] = split_bounds(l, ncrit, cbeg[c], ccnt[c], key, hist, b[j]);
+        k[j] = split_bounds(l, ncrit, cbeg[c], ccnt[c], key, hist, b[j], lforce);
       }
       mine += k[j];
       nspl += k[j] > 0;
@@ -390,7 +468,7 @@ __global__ void __launch_bounds__(kTopThreads)
         if (k[j] == 0)
           cchild[c] = -1;
         else
-          emit_children(l, c, next + off, b[j], root, cx, cbeg, ccnt, cchild, cparent, clevel);
+          emit_children(l, c, next + off, b[j], froots, key, cx, cbeg, ccnt, cchild, cparent, clevel);
         off += k[j];
       }
     }
@@ -496,10 +574,24 @@ void tree_build(Ctx &c) {
   double *root = t.bounds_part.p + 256 * 6;
   XF_HIP(hipMemsetAsync(t.misc.p, 0, 64 * sizeof(int), s));
 
-  hipLaunchKernelGGL(k_bounds_part, dim3(256), dim3(256), 0, s, c.bodies, n, t.bounds_part.p, t.misc.p);
-  XF_LAUNCH();
-  hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, t.bounds_part.p, 256, root);
-  XF_LAUNCH();
+  // forest: per-problem roots; the root of levels 0 .. kForestLevel - 1
+  // (problem-index cells, never paired by the traversal) is a unit cube
+  const bool forest = c.forest_n > 0;
+  const double4 *froots = forest ? c.forest_roots.p : nullptr;
+  const int lforce = forest ? kForestLevel : 0;
+  t.lroot = lforce;
+  if (forest) {
+    hipLaunchKernelGGL(k_forest_bounds, dim3(c.forest_n), dim3(256), 0, s, c.bodies, c.forest_off.p,
+                       c.forest_roots.p, t.misc.p);
+    XF_LAUNCH();
+    const double unit[4] = {0.0, 0.0, 0.0, 1.0};
+    XF_HIP(hipMemcpyAsync(root, unit, sizeof(unit), hipMemcpyHostToDevice, s));
+  } else {
+    hipLaunchKernelGGL(k_bounds_part, dim3(256), dim3(256), 0, s, c.bodies, n, t.bounds_part.p, t.misc.p);
+    XF_LAUNCH();
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, s, t.bounds_part.p, 256, root);
+    XF_LAUNCH();
+  }
 
   t.key.ensure(n);
   t.key2.ensure(n);
@@ -508,7 +600,7 @@ void tree_build(Ctx &c) {
   t.hist.ensure(kHistTotal);
   XF_HIP(hipMemsetAsync(t.hist.p + hist_off(kHistLevels), 0, sizeof(int) * (1 << (3 * kHistLevels)), s));
   hipLaunchKernelGGL(k_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, c.bodies, n, root, t.key.p, t.val.p,
-                     t.hist.p + hist_off(kHistLevels));
+                     t.hist.p + hist_off(kHistLevels), froots, forest ? c.forest_off.p : nullptr, c.forest_n);
   XF_LAUNCH();
   // one block per level-3 prefix (4096 level-7 bins each)
   t.hist_bmax.ensure(8 << (3 * 3));
@@ -527,10 +619,10 @@ void tree_build(Ctx &c) {
   for (int d = 0; d < 3; d++) t.X0[d] = rootv[d];
   t.R0 = rootv[3];
   int Ds = kMaxLevel;
-  if (n <= ncrit) {
+  if (n <= ncrit && !forest) {
     Ds = 0;
   } else {
-    for (int l = 1; l <= kHistLevels; l++)
+    for (int l = std::max(1, lforce); l <= kHistLevels; l++)
       if (hm[l] <= ncrit) {
         Ds = l;
         break;
@@ -549,7 +641,7 @@ void tree_build(Ctx &c) {
   const int64_t ub_split = 8 * (n / (ncrit + 1)) + 8;
   for (int l = 0; l <= Ds; l++) {
     ub[l] = std::min<int64_t>(l >= 10 ? n : (int64_t(1) << (3 * l)), n);
-    if (l > 0) ub[l] = std::min(ub[l], ub_split);
+    if (l > lforce) ub[l] = std::min(ub[l], ub_split);  // (a forest's top levels split unconditionally)
     ubtot += ub[l];
   }
   t.cx.ensure(ubtot);
@@ -572,21 +664,21 @@ void tree_build(Ctx &c) {
   XF_LAUNCH();
   const int lfuse = std::min(Ds, kTopLevels);
   if (lfuse > 0) {
-    hipLaunchKernelGGL(k_top_levels, dim3(1), dim3(kTopThreads), 0, s, lfuse, ncrit, t.lcount.p, t.lstart_d.p, k1,
-                       t.hist.p, root, t.cx.p, t.cbeg.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.cparent.p, t.clevel.p,
-                       t.misc.p + 18);
+    hipLaunchKernelGGL(k_top_levels, dim3(1), dim3(kTopThreads), 0, s, lfuse, ncrit, lforce, t.lcount.p,
+                       t.lstart_d.p, k1, t.hist.p, froots, t.cx.p, t.cbeg.p, t.ccnt.p, t.cchild.p, t.cnch.p,
+                       t.cparent.p, t.clevel.p, t.misc.p + 18);
     XF_LAUNCH();
   }
   for (int l = lfuse; l < Ds; l++) {
     const int g = ceil_div(ub[l], 256);
-    hipLaunchKernelGGL(k_split, dim3(g), dim3(256), 0, s, l, ncrit, t.lcount.p, t.lstart_d.p, t.cbeg.p, t.ccnt.p, k1,
-                       t.hist.p, t.tmp_nch.p, t.tmp_bnd.p);
+    hipLaunchKernelGGL(k_split, dim3(g), dim3(256), 0, s, l, ncrit, lforce, t.lcount.p, t.lstart_d.p, t.cbeg.p,
+                       t.ccnt.p, k1, t.hist.p, t.tmp_nch.p, t.tmp_bnd.p);
     XF_LAUNCH();
     scan_exclusive_dev(t.tmp_nch.p, t.tmp_off.p, t.lcount.p + l, int(ub[l]), t.misc.p + 16, t.scan_tmp, s);
     hipLaunchKernelGGL(k_level_next, dim3(1), dim3(1), 0, s, l, t.lcount.p, t.lstart_d.p, t.misc.p + 16);
     XF_LAUNCH();
     hipLaunchKernelGGL(k_emit, dim3(g), dim3(256), 0, s, l, t.lcount.p, t.lstart_d.p, t.tmp_nch.p, t.tmp_off.p,
-                       t.tmp_bnd.p, root, t.cx.p, t.cbeg.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.cparent.p,
+                       t.tmp_bnd.p, froots, k1, t.cx.p, t.cbeg.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.cparent.p,
                        t.clevel.p, t.misc.p + 18);
     XF_LAUNCH();
   }

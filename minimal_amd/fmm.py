@@ -42,7 +42,7 @@ EXPORTS = [
     "exafmm_get_order_model", "exafmm_order_training_set", "exafmm_mlp_forward", "exafmm_mlp_gradient",
     "exafmm_order_stats", "exafmm_graph_create", "exafmm_graph_create_grouped", "exafmm_graph_launch",
     "exafmm_graph_synchronize", "exafmm_graph_nodes", "exafmm_graph_gemm_shapes", "exafmm_graph_destroy",
-    "exafmm_evaluate_batch",
+    "exafmm_evaluate_batch", "exafmm_evaluate_forest",
 ]
 
 MLP_PARAMS = 1579  # EXAFMM_MLP_PARAMS
@@ -56,6 +56,8 @@ class Problem(C.Structure):
 
 
 BATCH_REUSE_GEOMETRY = 1
+BATCH_FOREST = 2
+FOREST_MAX = 512  # problems per forest (8^3 cells above the problems' roots)
 
 
 class Config(C.Structure):
@@ -125,6 +127,7 @@ class Library:
         L.exafmm_graph_nodes.argtypes = [_P, C.POINTER(C.c_int64)]
         L.exafmm_graph_gemm_shapes.argtypes = [_P, _P, _I, C.POINTER(C.c_int)]
         L.exafmm_evaluate_batch.argtypes = [_P, _P, _I, _I]
+        L.exafmm_evaluate_forest.argtypes = [_P, _P, _P, _I, _I]
         L.exafmm_graph_destroy.argtypes = [_P]
 
     def device_count(self) -> int:
@@ -237,12 +240,26 @@ class FMM:
         ps, ns = _bodies(sources, ns, write=False, what="sources")
         self._chk(self.L.lib.exafmm_direct(self.h, pt, nt, ps, ns, int(on_device)))
 
-    def evaluate_batch(self, problems, reuse_geometry=False):
+    def evaluate_forest(self, bodies, offsets, on_device=False):
+        """Independent problems in one call (exafmm_evaluate_forest): problem
+        p is bodies[offsets[p]:offsets[p+1]]; a host array or a device
+        pointer (int)."""
+        off = np.ascontiguousarray(offsets, np.int64)
+        if off.ndim != 1 or len(off) < 2:
+            raise ValueError("offsets: need nprob + 1 >= 2 entries")
+        p, n = _bodies(bodies, int(off[-1]))
+        self._keep = bodies
+        self._chk(self.L.lib.exafmm_evaluate_forest(self.h, p, off.ctypes.data_as(C.c_void_p), len(off) - 1,
+                                                    int(on_device)))
+
+    def evaluate_batch(self, problems, reuse_geometry=False, forest=False):
         """Evaluate many problems [(bodies, P) or (device pointer, n, P)] with
         this context's configuration (exafmm_evaluate_batch).  Host numpy
         arrays or device pointers (then every problem must be on the device).
         With reuse_geometry, later calls with the same problems replay a HIP
-        graph (positions kept, charges read afresh)."""
+        graph (positions kept, charges read afresh); with forest, the
+        problems of each order are evaluated as one forest (trees and lists
+        built afresh by every call)."""
         arr = (Problem * len(problems))()
         keep = []
         for k, pr in enumerate(problems):
@@ -255,8 +272,8 @@ class FMM:
                 ptr, n, P = pr
                 arr[k] = Problem(ptr, n, P, 1)
         self._keep_batch = (arr, keep)
-        self._chk(self.L.lib.exafmm_evaluate_batch(self.h, C.cast(arr, C.c_void_p), len(problems),
-                                                   BATCH_REUSE_GEOMETRY if reuse_geometry else 0))
+        flags = (BATCH_REUSE_GEOMETRY if reuse_geometry else 0) | (BATCH_FOREST if forest else 0)
+        self._chk(self.L.lib.exafmm_evaluate_batch(self.h, C.cast(arr, C.c_void_p), len(problems), flags))
 
     def dipole_correction(self):
         self._chk(self.L.lib.exafmm_dipole_correction(self.h))
