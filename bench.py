@@ -649,13 +649,14 @@ def main():
     ap.add_argument("--P-fixed", dest="P_fixed", type=int, default=12, help="learned workload: the fixed order compared")
     ap.add_argument("--P-learned", dest="P_learned", type=int, default=14,
                     help="learned workload: highest learned order (the learned context's P)")
-    ap.add_argument("--tol", type=float, default=-10.0,
-                    help="learned workload: log10 of the allowed M2L error per pair, relative to sum|q| / R0")
+    ap.add_argument("--tol", type=float, default=-7.0,
+                    help="learned workload: log10 of the allowed M2L error per pair, relative to the potential "
+                         "scale sqrt((sum q)^2 + sum q^2) / R0")
     ap.add_argument("--ncrit", type=int, default=64)
     ap.add_argument("--theta", type=float, default=0.4)
     ap.add_argument("--m2l", choices=["bf16x3", "fp32", "fp64"], default="bf16x3")
     ap.add_argument("--no-learned", action="store_true", help="plain workload: skip the learned-order comparison")
-    ap.add_argument("--plain-learned-tol", type=float, default=-9.5,
+    ap.add_argument("--plain-learned-tol", type=float, default=-7.0,
                     help="plain workload: log10 tolerance of the learned-order comparison")
     ap.add_argument("--p2p", choices=["fp32", "fp32_newton"], default="fp32_newton",
                     help="plain workload: P2P precision (fp32_newton, the stated arithmetic of the N=2^20 "

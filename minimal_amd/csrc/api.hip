@@ -1189,7 +1189,11 @@ static void batch_forests(exafmm_ctx *ctx, const exafmm_problem *problems, int c
       ctx->forests[gi] = nullptr;
       if (exafmm_create(&cfg, &ctx->forests[gi]) != EXAFMM_OK) fail(EXAFMM_ERR_ALLOC, g_error);
     }
-    Ctx &fc = ctx->forests[gi]->c;
+  }
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    const std::vector<int> &g = groups[gi];
+    exafmm_ctx *fx = ctx->forests[gi];
+    Ctx &fc = fx->c;
     std::vector<int64_t> off(g.size() + 1, 0);
     for (size_t k = 0; k < g.size(); k++) off[k + 1] = off[k] + problems[g[k]].n;
     Buf<exafmm_body> &buf = *ctx->forest_bodies[gi];
@@ -1199,8 +1203,8 @@ static void batch_forests(exafmm_ctx *ctx, const exafmm_problem *problems, int c
       XF_HIP(hipMemcpyAsync(buf.p + off[k], q.bodies, sizeof(exafmm_body) * q.n,
                             q.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, fc.s0));
     }
-    if (exafmm_evaluate_forest(ctx->forests[gi], buf.p, off.data(), int(g.size()), 1) != EXAFMM_OK)
-      fail(EXAFMM_ERR_HIP, std::string("forest of order ") + std::to_string(P) + ": " + fc.last_error);
+    if (exafmm_evaluate_forest(fx, buf.p, off.data(), int(g.size()), 1) != EXAFMM_OK)
+      fail(EXAFMM_ERR_HIP, std::string("forest of order ") + std::to_string(fc.P) + ": " + fc.last_error);
     for (size_t k = 0; k < g.size(); k++) {
       const exafmm_problem &q = problems[g[k]];
       XF_HIP(hipMemcpyAsync(q.bodies, buf.p + off[k], sizeof(exafmm_body) * q.n,

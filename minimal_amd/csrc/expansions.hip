@@ -343,6 +343,11 @@ __global__ void __launch_bounds__(kM2MWaves *kWave) k_m2m(int P, int lstart, int
 // kShiftBatch2 k-steps at a time.
 constexpr int kShiftBatch2 = 8;
 constexpr int kRowTileMaxWaves = 16;  // P <= 16
+// Levels with fewer parents run the octant-parallel kernels (k_m2m: an
+// octant per wave, k_l2l: an octant per workgroup): their chains of
+// dependent loads are 8x shorter, and few parents leave the cache-sharing
+// of the row-tile form little to share.
+constexpr int kRowTileMinParents = 8192;
 
 // waves of a row-tile workgroup: one per row tile, at least two (filing the
 // children of 16 parents takes 128 threads)
@@ -791,8 +796,12 @@ void upward(Ctx &c) {
   XF_LAUNCH();
   for (int l = t.depth - 1; l >= t.lroot; l--) {  // (a forest stops at its problems' roots)
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_m2m_rows, dim3(ceil_div(cnt, 16)), dim3(row_tile_waves(PP) * kWave), 0, s, c.P,
-                       t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
+    if (cnt >= kRowTileMinParents)
+      hipLaunchKernelGGL(k_m2m_rows, dim3(ceil_div(cnt, 16)), dim3(row_tile_waves(PP) * kWave), 0, s, c.P,
+                         t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
+    else
+      hipLaunchKernelGGL(k_m2m, dim3(ceil_div(cnt, 16), m2m_group_rows(P)), dim3(kM2MWaves * kWave), 0, s, c.P,
+                         t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.M.p);
     XF_LAUNCH();
   }
   c.have_M = true;
@@ -805,8 +814,13 @@ void l2l_pass(Ctx &c) {
   shift_ops(c);
   for (int l = t.lroot; l < t.depth; l++) {  // parents at level l
     const int cnt = t.lstart[l + 1] - t.lstart[l];
-    hipLaunchKernelGGL(k_l2l_rows, dim3(ceil_div(cnt, 16)), dim3(row_tile_waves(PP) * kWave), 0, s, c.P,
-                       t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.L.p);
+    if (cnt >= kRowTileMinParents)
+      hipLaunchKernelGGL(k_l2l_rows, dim3(ceil_div(cnt, 16)), dim3(row_tile_waves(PP) * kWave), 0, s, c.P,
+                         t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p, c.shift_T.p, c.L.p);
+    else
+      hipLaunchKernelGGL(k_l2l, dim3(ceil_div(cnt, 16), ceil_div(PP, 16 * kShiftWaves), 8),
+                         dim3(kShiftWaves * kWave), 0, s, c.P, t.lstart[l], cnt, t.cchild.p, t.cnch.p, t.cx.p,
+                         c.shift_T.p, c.L.p);
     XF_LAUNCH();
   }
 }

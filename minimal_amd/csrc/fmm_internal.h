@@ -228,8 +228,9 @@ struct Ctx {
   // learned order model (order.hip): FP32 master weights, tolerance
   bool have_model = false;
   float order_log_tol = -5.0f;
-  Buf<float> mlp_w, mlp_part, mshape;
-  Buf<double> absq;  // sum |q| per cell (order selection)
+  Buf<float> mlp_w, mlp_part;
+  Buf<float2> mshape;  // per cell: multipole shape, log10 sum|q| (order selection)
+  Buf<double2> absq;  // (sum |q|, sum q^2) per cell (order selection)
   int train_n = 0;
   Buf<float> train_X, train_T;
   std::vector<int> train_pairs;  // (target cell, source cell, image) per sample, internal numbering  // scratch of m2l_prepare (may run on the side stream)

@@ -282,24 +282,32 @@ __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirs
   }
 }
 
-// Learned orders: each tile's slots grouped by order (ascending order,
-// key order within an order), so that one GEMM launch per order covers a
-// contiguous run of every tile's slots at a uniform order.  One wave per
-// tile; per 64 slots, one ballot per order gives every slot its rank.
-// perm[tbase[t] + j] = slot of the j-th slot of tile t in that grouping;
-// cls[o * ts + t] = tbase[t] + first grouped slot of order pmin + o in tile
-// t, cls[(ncls + o) * ts + t] = their count.
+// Learned orders are run in a few order classes (ranges of orders, one
+// GEMM launch each): of[p] = class of order p, hi[k] = highest order of
+// class k.
+struct OrderClasses {
+  int n;
+  unsigned char of[kMaxP + 1], hi[kMaxP + 1];
+};
+
+// Each tile's slots grouped by order class (ascending class, key order
+// within a class), so that one GEMM launch per class covers a contiguous
+// run of every tile's slots.  One wave per tile; per 64 slots, one ballot
+// per class gives every slot its rank.  perm[tbase[t] + j] = slot of the
+// j-th slot of tile t in that grouping; cls[o * ts + t] = tbase[t] + first
+// grouped slot of class o in tile t, cls[(ncls + o) * ts + t] = their count.
 __global__ void __launch_bounds__(256) k_order_partition(int ntiles, int ts, const int *__restrict__ tbase,
                                                          const int *__restrict__ tU,
-                                                         const unsigned char *__restrict__ tkp, int pmin, int ncls,
+                                                         const unsigned char *__restrict__ tkp, OrderClasses oc,
                                                          int *__restrict__ perm, int *__restrict__ cls) {
+  const int ncls = oc.n;
   const int lane = threadIdx.x & 63;
   const int t = blockIdx.x * 4 + int(threadIdx.x >> 6);
   if (t >= ntiles) return;  // whole wave
   const int b = tbase[t], U = tU[t];
   const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  auto cls_of = [&](int u) { return u < U ? min(max(int(tkp[b + u]) - pmin, 0), ncls - 1) : -1; };
-  int cnt = 0;  // lane o < ncls: slots of order pmin + o
+  auto cls_of = [&](int u) { return u < U ? int(oc.of[min(int(tkp[b + u]), kMaxP)]) : -1; };
+  int cnt = 0;  // lane o < ncls: slots of class o
   for (int u0 = 0; u0 < U; u0 += 64) {
     const int o = cls_of(u0 + lane);
     for (int k = 0; k < ncls; k++) {
@@ -1023,7 +1031,8 @@ static size_t gemm_lds(const Geom &g, bool bf16, int RTB, int kr) {
 // slots of that order (tile t: grouped slots cb[t] .. cb[t] + cnt[t] - 1,
 // slot permutation perm).
 struct ClassPass {
-  int p;
+  int p;          // highest order of the class
+  bool variable;  // the class holds several orders: each slot runs at its own (tile_p)
   const int *perm;
   const int *start, *cnt;  // per tile: first order-grouped slot (tbase included) and count
   const int *pre, *tot;    // per tile: exclusive prefix of the counts; their total (stream-K)
@@ -1051,7 +1060,7 @@ static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool 
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
     hipLaunchKernelGGL((k_m2l_bf16x3<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, op,
                        tfirst, tcount, cp ? cp->start : tbase, cp ? cp->cnt : tU, sorted, c.tile_keys.p,
-                       c.tile_mat.p, ophi, oplo, mhi, mlo, (c.learned && !cp) ? c.tile_p.p : nullptr, kr,
+                       c.tile_mat.p, ophi, oplo, mhi, mlo, (c.learned && (!cp || cp->variable)) ? c.tile_p.p : nullptr, kr,
                        int64_t(c.t.ncells), ksplit > 1 ? Lpart : nullptr, Lout);
   } else {
     const size_t lds = gemm_lds(g, false, RTB, kr);
@@ -1102,7 +1111,7 @@ static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
   hipLaunchKernelGGL((k_m2l_bf16x3_sk<RH>), dim3(G), dim3(kGemmThreads), lds, c.s0, g.PP, g.R, g.K32, KPb, op, ntiles,
                      S, Sdev, tfirst, tcount, pre, start, cnt, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi,
-                     mlo, (c.learned && !cp) ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), c.m2l_part.p);
+                     mlo, (c.learned && (!cp || cp->variable)) ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), c.m2l_part.p);
   XF_LAUNCH();
   const int rows = std::min(g.PP, 16 * go.R);
   hipLaunchKernelGGL(k_sk_combine, dim3(ceil_div(int64_t(kTN) * rows, 256), ntiles), dim3(256), 0, c.s0, g.PP, rows,
@@ -1403,24 +1412,37 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
 
   // 3. GEMM over tiles, shape from the autotuner
   if (c.learned && bf16) {
-    // one launch per order over that order's slots of every tile (grouped
-    // by k_order_partition), each with its own tuned shape: a slot's cost
-    // then follows its order, and the launches add into the locals in
-    // order sequence
-    const int pmin = kMlpPmin, ncls = std::max(1, c.P - kMlpPmin + 1);
+    // one launch per order class over that class's slots of every tile
+    // (grouped by k_order_partition), each with its own tuned shape: a
+    // slot's cost then follows its order, and the launches add into the
+    // locals in class sequence.  Classes: orders up to 8, then pairs (9-10,
+    // 11-12, ...): few launches (each adds a partial-locals pass), each
+    // class's LDS stage and row tiles sized to its highest order
+    OrderClasses oc{};
+    for (int p = 0; p <= kMaxP; p++) {
+      const int q = std::min(std::max(p, kMlpPmin), c.P);
+      const int k = q <= 8 ? 0 : (q - 7) / 2;
+      oc.of[p] = (unsigned char)k;
+      oc.hi[k] = (unsigned char)std::max<int>(oc.hi[k], q);
+      oc.n = std::max(oc.n, k + 1);
+    }
+    const int ncls = oc.n;
     const int ts = c.m2l_tstride;
     c.tile_perm.ensure(std::max(c.m2l_totalU, 1));
     // tile_cls: [start | count | prefix] x order x tile stride, then the totals
     c.tile_cls.ensure(size_t(3) * ncls * ts + ncls);
     hipLaunchKernelGGL(k_order_partition, dim3(ceil_div(ntiles, 4)), dim3(256), 0, s, ntiles, ts, c.tiles.p + 2 * ts,
-                       c.tiles.p + 3 * ts, c.tile_p.p, pmin, ncls, c.tile_perm.p, c.tile_cls.p);
+                       c.tiles.p + 3 * ts, c.tile_p.p, oc, c.tile_perm.p, c.tile_cls.p);
     XF_LAUNCH();
     hipLaunchKernelGGL(k_order_prefix, dim3(1), dim3(64 * ncls), 0, s, ntiles, ts, ncls, c.tile_cls.p);
     XF_LAUNCH();
     if (!c.capturing) XF_HIP(hipEventRecord(c.ev[13], s));
     for (int o = ncls - 1; o >= 0; o--) {
       const int *cls = c.tile_cls.p;
-      const ClassPass cp{pmin + o, c.tile_perm.p, cls + size_t(o) * ts, cls + size_t(ncls + o) * ts,
+      int lo = kMaxP;
+      for (int p = kMlpPmin; p <= c.P; p++)
+        if (oc.of[p] == o) lo = std::min(lo, p);
+      const ClassPass cp{oc.hi[o], lo < oc.hi[o], c.tile_perm.p, cls + size_t(o) * ts, cls + size_t(ncls + o) * ts,
                          cls + size_t(2 * ncls + o) * ts, cls + size_t(3 * ncls) * ts + o};
       const GemmShape sh = gemm_shape(c, sc, g, ntiles, bf16, KP, KPb, &cp);
       run_gemm(c, sc, g, ntiles, bf16, KP, KPb, sh, c.L.p, &cp);

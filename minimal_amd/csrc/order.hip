@@ -231,8 +231,11 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ grad, fl
 // Per cell: log10 of the share of the scaled multipole norm in degrees >= 2.
 // One wave per cell: the shares of the squared real-form coefficients,
 // read coalesced across the lanes and reduced over the wave.
+// feat[c] = (shape, log10 sum|q| of the cell, or -30 without absq): the two
+// per-source quantities the order selection reduces over a slot's columns,
+// in one 8-byte load per column.
 __global__ void __launch_bounds__(256) k_mshape(int PP, int ncells, const double *__restrict__ M,
-                                                float *__restrict__ shape) {
+                                                const double2 *__restrict__ absq, float2 *__restrict__ feat) {
   const int lane = threadIdx.x & 63;
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (c >= ncells) return;
@@ -250,7 +253,9 @@ __global__ void __launch_bounds__(256) k_mshape(int PP, int ncells, const double
   }
   if (lane == 0) {
     const double r = tot > 0.0 ? sqrt(high / tot) : 0.0;
-    shape[c] = float(fmin(0.0, fmax(-6.0, log10(fmax(r, 1e-30)))));
+    const double aq = absq ? absq[c].x : 0.0;
+    feat[c] = make_float2(float(fmin(0.0, fmax(-6.0, log10(fmax(r, 1e-30))))),
+                          aq > 0.0 ? float(log10(aq)) : -30.0f);
   }
 }
 
@@ -269,29 +274,76 @@ __device__ inline int order_of(const float (&y)[kMlpOut], float log_tol) {
   return kMlpPmin + kMlpOut - 1;
 }
 
+// Per cell (sum |q|, sum q^2): sum |q| is the scale the pair errors are
+// normalised by, the root's sum q^2 enters the tolerance's scale.  Leaves
+// from their bodies, then level by level from the children.
+__global__ void __launch_bounds__(256) k_absq_leaf(int nleaves, const int *__restrict__ leaves,
+                                                   const int *__restrict__ cbeg, const int *__restrict__ ccnt,
+                                                   const double4 *__restrict__ xs, double2 *__restrict__ absq) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= nleaves) return;
+  const int c = leaves[i];
+  double s = 0.0, s2 = 0.0;
+  for (int j = cbeg[c], e = cbeg[c] + ccnt[c]; j < e; j++) {
+    const double q = xs[j].w;
+    s += fabs(q);
+    s2 += q * q;
+  }
+  absq[c] = make_double2(s, s2);
+}
+
+__global__ void __launch_bounds__(256) k_absq_up(int lstart, int lcount, const int *__restrict__ cnch,
+                                                 const int *__restrict__ cchild, double2 *__restrict__ absq) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= lcount) return;
+  const int c = lstart + i, k = cnch[c];
+  if (k == 0) return;
+  double s = 0.0, s2 = 0.0;
+  for (int j = 0; j < k; j++) {
+    const double2 v = absq[cchild[c] + j];
+    s += v.x;
+    s2 += v.y;
+  }
+  absq[c] = make_double2(s, s2);
+}
+
 // Orders of the (tile, key) slots of the keyed GEMM.  All pairs of one
 // slot share the key geometry and the target level (tiles hold cells of one
-// level); they differ only in the source multipole shape, so the slot is
-// evaluated once with the largest shape among its sources (the pair that
-// needs the highest order).  Per chunk of 256 slots of a tile: each wave
-// takes slots in turn and reduces the shapes of the slot's 256 columns
-// (coalesced row reads); then one thread per slot runs the MLP, takes the
-// smallest order under the tolerance and caps it at the context's order.
+// level); they differ in the source cell, so the slot is evaluated once
+// with the largest multipole shape and the largest sum of |q| among its
+// sources (the pair that needs the highest order).  The model predicts a
+// pair's potential error relative to sum|q_source| / |d|; the tolerance is
+// on the absolute error, relative to the tree's potential scale
+// S = sqrt((sum q)^2 + sum q^2) / R0 (the size of the potential whether the
+// charges share a sign or not): the order is the smallest p with
+//   y_p + log10((sum|q_s| / |d|) / S) <= log10_tol,
+// so pairs of large, near cells (whose errors dominate a body's total) get
+// higher orders than small, distant ones.  Per chunk of 1024
+// slots of a tile: each wave takes slots in turn and reduces over the slot's
+// 256 columns (coalesced row reads); then one thread per slot runs the MLP
+// and caps the order at the context's order.
 __global__ void __launch_bounds__(1024) k_order_select(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                        const int *__restrict__ tbase, const int *__restrict__ tU,
                                                        const int *__restrict__ sorted, const int *__restrict__ tkeys,
                                                        const int *__restrict__ tmat, const unsigned long long *keys,
                                                        const int *__restrict__ clevel,
-                                                       const float *__restrict__ shape, const float *__restrict__ w,
+                                                       const float2 *__restrict__ feat,
+                                                       const double2 *__restrict__ absq,
+                                                       const double4 *__restrict__ cx, const double *__restrict__ M,
+                                                       const float *__restrict__ w,
                                                        float log_tol, double R0, double cycle, int tn, int pmax,
                                                        unsigned char *__restrict__ tile_p) {
   __shared__ float W[kMlpParams];
   __shared__ float smax_s[1024];
+  __shared__ float qmax_s[1024];
   load_weights(w, W);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;  // 16 waves
   const int tile = blockIdx.x;
   const int ncol = tcount[tile], U = tU[tile], base = tbase[tile];
   const int lt = clevel[sorted[tfirst[tile]]];
+  const double Rt = cx[sorted[tfirst[tile]]].w;
+  // log10 of the potential scale (M[0] = sum q of the root)
+  const float lscale = float(log10(fmax(sqrt(M[0] * M[0] + absq[0].y) / cx[0].w, 1e-300)));
   for (int u0 = 0; u0 < U; u0 += 1024) {
     const int nu = min(1024, U - u0);
     // largest source shape per key: a wave per key, two keys (j, j + 16)
@@ -301,20 +353,34 @@ __global__ void __launch_bounds__(1024) k_order_select(const int *__restrict__ t
       const bool two = j2 < nu;
       const int *ra = tmat + (int64_t(base) + u0 + j) * tn;
       const int *rb = tmat + (int64_t(base) + u0 + (two ? j2 : j)) * tn;
-      float ma = -1e30f, mb = -1e30f;
+      float ma = -1e30f, mb = -1e30f, qa = -1e30f, qb = -1e30f;
       for (int c = lane; c < ncol; c += 64) {
         const int sa = ra[c], sb = rb[c];
-        if (sa >= 0) ma = fmaxf(ma, shape[sa]);
-        if (sb >= 0) mb = fmaxf(mb, shape[sb]);
+        if (sa >= 0) {
+          const float2 f = feat[sa];
+          ma = fmaxf(ma, f.x);
+          qa = fmaxf(qa, f.y);
+        }
+        if (sb >= 0) {
+          const float2 f = feat[sb];
+          mb = fmaxf(mb, f.x);
+          qb = fmaxf(qb, f.y);
+        }
       }
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) {
         ma = fmaxf(ma, __shfl_xor(ma, o, 64));
         mb = fmaxf(mb, __shfl_xor(mb, o, 64));
+        qa = fmaxf(qa, __shfl_xor(qa, o, 64));
+        qb = fmaxf(qb, __shfl_xor(qb, o, 64));
       }
       if (lane == 0) {
         smax_s[j] = ma;
-        if (two) smax_s[j2] = mb;
+        qmax_s[j] = qa;
+        if (two) {
+          smax_s[j2] = mb;
+          qmax_s[j2] = qb;
+        }
       }
     }
     __syncthreads();
@@ -329,7 +395,10 @@ __global__ void __launch_bounds__(1024) k_order_select(const int *__restrict__ t
         float x[kMlpIn], h1[kMlpH], h2[kMlpH], y[kMlpOut];
         pair_features(dx, dy, dz, a, b, lt, smax, x);
         mlp_forward(W, x, h1, h2, y);
-        p = order_of(y, log_tol);
+        // |d| in units of the finer radius Rf = Rt / 2^a
+        const double d = sqrt(dx * dx + dy * dy + dz * dz) * ldexp(Rt, -a);
+        const float shift = qmax_s[threadIdx.x] - float(log10(d)) - lscale;
+        p = order_of(y, log_tol - shift);
       }
       tile_p[base + u] = (unsigned char)min(p, pmax);
     }
@@ -351,7 +420,7 @@ __global__ void __launch_bounds__(64) k_pair_errors(int npairs, const int *__res
                                                     double R0, double cycle, const double4 *__restrict__ cx,
                                                     const int *__restrict__ cbeg, const int *__restrict__ ccnt,
                                                     const int *__restrict__ clevel, const double4 *__restrict__ xs,
-                                                    const double *__restrict__ M, const float *__restrict__ shape,
+                                                    const double *__restrict__ M, const float2 *__restrict__ shape,
                                                     float *__restrict__ X, float *__restrict__ T) {
   __shared__ cpx<double> I[(2 * kPm - 1) * (2 * kPm) / 2];
   __shared__ double Lp[kMlpOut][kPmPP];
@@ -451,7 +520,7 @@ __global__ void __launch_bounds__(64) k_pair_errors(int npairs, const int *__res
       T[int64_t(pi) * kMlpOut + o] = (q > 0.0 && nt > 0) ? float(log10(fmax(e, 1e-30))) : __builtin_nanf("");
     }
     float x[kMlpIn];
-    pair_features(dx, dy, dz, a, b, clevel[ct], shape[cs], x);
+    pair_features(dx, dy, dz, a, b, clevel[ct], shape[cs].x, x);
     for (int k = 0; k < kMlpIn; k++) X[int64_t(pi) * kMlpIn + k] = x[k];
   }
 }
@@ -476,13 +545,25 @@ void order_select(Ctx &c, const int *tfirst, const int *tcount, const int *tbase
                   int ntiles, int totalU, int tn, hipStream_t s) {
   c.tile_p.ensure(std::max(totalU, 1));
   const int nc = int(c.t.ncells);
+  // sum |q| per cell, then the per-cell features
+  const Tree &t = c.t;
+  c.absq.ensure(nc);
+  hipLaunchKernelGGL(k_absq_leaf, dim3(ceil_div(std::max<int64_t>(t.nleaves, 1), 256)), dim3(256), 0, s,
+                     int(t.nleaves), t.leaves.p, t.cbeg.p, t.ccnt.p, t.xs.p, c.absq.p);
+  XF_LAUNCH();
+  for (int l = t.depth - 1; l >= 0; l--) {
+    const int cnt = t.lstart[l + 1] - t.lstart[l];
+    hipLaunchKernelGGL(k_absq_up, dim3(ceil_div(cnt, 256)), dim3(256), 0, s, t.lstart[l], cnt, t.cnch.p,
+                       t.cchild.p, c.absq.p);
+    XF_LAUNCH();
+  }
   c.mshape.ensure(nc);
-  hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 4)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, c.mshape.p);
+  hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 4)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, c.absq.p, c.mshape.p);
   XF_LAUNCH();
   const double cyc = c.cfg.images > 0 ? c.cfg.cycle : 0.0;
   hipLaunchKernelGGL(k_order_select, dim3(ntiles), dim3(1024), 0, s, tfirst, tcount, tbase, tU, sorted,
-                     c.tile_keys.p, c.tile_mat.p, c.lists.keys.p, c.t.clevel.p, c.mshape.p, c.mlp_w.p,
-                     c.order_log_tol, c.t.R0, cyc, tn, c.P, c.tile_p.p);
+                     c.tile_keys.p, c.tile_mat.p, c.lists.keys.p, c.t.clevel.p, c.mshape.p, c.absq.p, c.t.cx.p,
+                     c.M.p, c.mlp_w.p, c.order_log_tol, c.t.R0, cyc, tn, c.P, c.tile_p.p);
   XF_LAUNCH();
 }
 
@@ -527,7 +608,8 @@ float order_train(Ctx &c, int nsamples, int epochs, float lr, uint64_t seed) {
   }
   XF_HIP(hipMemcpyAsync(dpe.p, pe.data(), sizeof(int2) * n, hipMemcpyHostToDevice, s));
   c.mshape.ensure(nc);
-  hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 4)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, c.mshape.p);
+  hipLaunchKernelGGL(k_mshape, dim3(ceil_div(nc, 4)), dim3(256), 0, s, c.P * c.P, nc, c.M.p, (const double2 *)nullptr,
+                     c.mshape.p);
   XF_LAUNCH();
   Buf<float> X, T;
   X.ensure(size_t(n) * kMlpIn);

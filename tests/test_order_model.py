@@ -175,8 +175,8 @@ def test_order_grouped_gemm_matches_per_slot_orders(gpu):
         f.evaluate(x)
         return x, f.order_stats()
 
-    lb, hb = run(fmm.ORDER_LEARNED, fmm.PREC_BF16X3, -7.5)
-    lf, hf = run(fmm.ORDER_LEARNED, fmm.PREC_FP32, -7.5)
+    lb, hb = run(fmm.ORDER_LEARNED, fmm.PREC_BF16X3, -7.0)
+    lf, hf = run(fmm.ORDER_LEARNED, fmm.PREC_FP32, -7.0)
     fb, _ = run(fmm.ORDER_FIXED, fmm.PREC_BF16X3)
     ff, _ = run(fmm.ORDER_FIXED, fmm.PREC_FP32)
     assert np.array_equal(hb, hf)
@@ -193,13 +193,13 @@ def test_order_grouped_gemm_matches_per_slot_orders(gpu):
 
 @pytest.mark.gpu
 def test_learned_order_beats_fixed_order(gpu):
-    """Train on one Plummer+background set, evaluate another.  Against the
-    fixed-order fallback (P = 12, the learned context's maximum) the learned
-    orders must meet the same accuracy target (relative L2 of p and F
-    <= 1e-5 against the FP64 direct sum), stay within 5% (p) and 25% (F) of
-    the fixed-order error, and cut the keyed-GEMM work (sum over
-    (tile, key) slots of p^4) by at least 20%.  A learned-order context
-    without a model runs the fixed order."""
+    """Train on one Plummer+background set, evaluate another.  The learned
+    context holds its expansions to order 14 and picks each slot's order in
+    4..14 (tolerance 10^-6.5 of the potential scale per pair); against fixed
+    order 12 it must reach a relative L2 error of p and F no larger than the
+    fixed order's (both against the FP64 direct sum) with at least 20% less
+    keyed-GEMM work (sum over (tile, key) slots of p^4).  A learned-order
+    context without a model runs its own order P."""
     n = 1 << 16
     tr = _plummer_bodies(n, 5)
     f = fmm.FMM(P=14, ncrit=64, theta=0.4)
@@ -216,12 +216,13 @@ def test_learned_order_beats_fixed_order(gpu):
     bf = b.copy()
     fixed.evaluate(bf)
     hist_f = fixed.order_stats()
-    learned = fmm.FMM(P=12, ncrit=64, theta=0.4, order_policy=fmm.ORDER_LEARNED, autotune=0)
+    nomodel = fmm.FMM(P=12, ncrit=64, theta=0.4, order_policy=fmm.ORDER_LEARNED, autotune=0)
     bn = b.copy()
-    learned.evaluate(bn)  # no model yet: fixed order 12
-    assert np.array_equal(learned.order_stats(), hist_f)
+    nomodel.evaluate(bn)  # no model yet: its own order 12
+    assert np.array_equal(nomodel.order_stats(), hist_f)
     assert np.array_equal(bn["p"], bf["p"])
-    learned.set_order_model(w, -7.5)
+    learned = fmm.FMM(P=14, ncrit=64, theta=0.4, order_policy=fmm.ORDER_LEARNED, autotune=0)
+    learned.set_order_model(w, -6.5)
     bl = b.copy()
     learned.evaluate(bl)
     hist_l = learned.order_stats()
@@ -233,8 +234,8 @@ def test_learned_order_beats_fixed_order(gpu):
     ef, el = rel_l2(bf["p"][idx], ref["p"]), rel_l2(bl["p"][idx], ref["p"])
     efF, elF = rel_l2(bf["F"][idx], ref["F"]), rel_l2(bl["F"][idx], ref["F"])
     work = lambda h: sum(int(c) * p ** 4 for p, c in enumerate(h))
-    assert max(ef, efF, el, elF) <= 1e-5, (ef, efF, el, elF)
-    print("fixed p/F", ef, efF, "learned p/F", el, elF, "work ratio", work(hist_l) / work(hist_f))
-    assert el <= 1.05 * ef and elF <= 1.25 * efF, (ef, efF, el, elF)
+    print("fixed p/F", ef, efF, "learned p/F", el, elF, "work ratio", work(hist_l) / work(hist_f),
+          "orders", {p: int(c) for p, c in enumerate(hist_l) if c})
+    assert el <= ef and elF <= efF, (ef, efF, el, elF)
     assert work(hist_l) <= 0.8 * work(hist_f), (work(hist_l), work(hist_f))
-    assert hist_l[13:].sum() == 0 and hist_l[:4].sum() == 0  # orders within 4..12
+    assert hist_l[15:].sum() == 0 and hist_l[:4].sum() == 0  # orders within 4..14
