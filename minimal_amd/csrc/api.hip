@@ -20,6 +20,8 @@ struct exafmm_ctx {
   // EXAFMM_BATCH_FOREST: one forest context per order and its gathered bodies
   std::vector<exafmm_ctx *> forests;
   std::vector<std::unique_ptr<xf::Buf<exafmm_body>>> forest_bodies;
+  std::vector<std::unique_ptr<xf::Buf<xf::ForestPart>>> forest_parts;  // gather / scatter tables
+  std::vector<std::vector<xf::ForestPart>> forest_parts_host;
 };
 
 namespace xf {
@@ -258,6 +260,7 @@ int exafmm_destroy(exafmm_ctx *ctx) {
   for (exafmm_ctx *f : ctx->forests) exafmm_destroy(f);
   ctx->forests.clear();
   ctx->forest_bodies.clear();
+  ctx->forest_parts.clear();
   hipSetDevice(ctx->c.cfg.device);
   if (ctx->c.s0) hipStreamSynchronize(ctx->c.s0);
   if (ctx->c.s1) hipStreamSynchronize(ctx->c.s1);
@@ -1153,6 +1156,35 @@ int exafmm_graph_destroy(exafmm_graph *g) {
 }
 
 
+namespace xf {
+// Device-resident problems of a forest: gathered into the forest's array and
+// scattered back by one launch each (one thread per body and 8-byte word;
+// a body's problem by binary search over the offsets).
+__global__ void __launch_bounds__(256) k_forest_copy(const ForestPart *__restrict__ parts, int np, int64_t n,
+                                                     exafmm_body *__restrict__ fb, int to_forest) {
+  const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t i = g >> 3;
+  const int w = int(g & 7);
+  if (i >= n) return;
+  int lo = 0, hi = np - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (parts[mid].off <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const ForestPart q = parts[lo];
+  if (!q.src) return;  // a host problem: copied by the runtime
+  double *f = reinterpret_cast<double *>(fb + i) + w;
+  double *o = reinterpret_cast<double *>(q.src + (i - q.off)) + w;
+  if (to_forest)
+    *f = *o;
+  else
+    *o = *f;
+}
+}  // namespace xf
+
 // EXAFMM_BATCH_FOREST: the problems of each order, in their given order and
 // at most kForestMax per forest, are gathered into one device array and
 // evaluated as one forest (exafmm_evaluate_forest: one tree build, one
@@ -1177,8 +1209,11 @@ static void batch_forests(exafmm_ctx *ctx, const exafmm_problem *problems, int c
     if (exafmm_create(&cfg, &sub) != EXAFMM_OK) fail(EXAFMM_ERR_ALLOC, g_error);
     ctx->forests.push_back(sub);
   }
-  while (ctx->forest_bodies.size() < ctx->forests.size())
+  while (ctx->forest_bodies.size() < ctx->forests.size()) {
     ctx->forest_bodies.push_back(std::make_unique<Buf<exafmm_body>>());
+    ctx->forest_parts.push_back(std::make_unique<Buf<ForestPart>>());
+    ctx->forest_parts_host.emplace_back();
+  }
   for (size_t gi = 0; gi < groups.size(); gi++) {
     const std::vector<int> &g = groups[gi];
     const int P = problems[g[0]].P;
@@ -1198,17 +1233,29 @@ static void batch_forests(exafmm_ctx *ctx, const exafmm_problem *problems, int c
     for (size_t k = 0; k < g.size(); k++) off[k + 1] = off[k] + problems[g[k]].n;
     Buf<exafmm_body> &buf = *ctx->forest_bodies[gi];
     buf.ensure(off.back());
+    // device problems in one gather launch, host problems by copies
+    std::vector<ForestPart> &ph = ctx->forest_parts_host[gi];
+    ph.resize(g.size());
     for (size_t k = 0; k < g.size(); k++) {
       const exafmm_problem &q = problems[g[k]];
-      XF_HIP(hipMemcpyAsync(buf.p + off[k], q.bodies, sizeof(exafmm_body) * q.n,
-                            q.on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, fc.s0));
+      ph[k] = ForestPart{q.on_device ? q.bodies : nullptr, off[k]};
+      if (!q.on_device)
+        XF_HIP(hipMemcpyAsync(buf.p + off[k], q.bodies, sizeof(exafmm_body) * q.n, hipMemcpyHostToDevice, fc.s0));
     }
+    Buf<ForestPart> &pd = *ctx->forest_parts[gi];
+    pd.ensure(g.size());
+    XF_HIP(hipMemcpyAsync(pd.p, ph.data(), sizeof(ForestPart) * g.size(), hipMemcpyHostToDevice, fc.s0));
+    const int nblk = ceil_div(off.back() * 8, 256);
+    hipLaunchKernelGGL(k_forest_copy, dim3(nblk), dim3(256), 0, fc.s0, pd.p, int(g.size()), off.back(), buf.p, 1);
+    XF_LAUNCH();
     if (exafmm_evaluate_forest(fx, buf.p, off.data(), int(g.size()), 1) != EXAFMM_OK)
       fail(EXAFMM_ERR_HIP, std::string("forest of order ") + std::to_string(fc.P) + ": " + fc.last_error);
+    hipLaunchKernelGGL(k_forest_copy, dim3(nblk), dim3(256), 0, fc.s0, pd.p, int(g.size()), off.back(), buf.p, 0);
+    XF_LAUNCH();
     for (size_t k = 0; k < g.size(); k++) {
       const exafmm_problem &q = problems[g[k]];
-      XF_HIP(hipMemcpyAsync(q.bodies, buf.p + off[k], sizeof(exafmm_body) * q.n,
-                            q.on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, fc.s0));
+      if (!q.on_device)
+        XF_HIP(hipMemcpyAsync(q.bodies, buf.p + off[k], sizeof(exafmm_body) * q.n, hipMemcpyDeviceToHost, fc.s0));
     }
   }
   for (exafmm_ctx *f : ctx->forests) {

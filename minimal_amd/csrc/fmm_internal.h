@@ -18,6 +18,12 @@ constexpr int kMaxLevel = 21;     // 63-bit octant keys
 // roots are the cells of level kForestLevel
 constexpr int kForestLevel = 3;
 constexpr int kForestMax = 1 << (3 * kForestLevel);
+// one problem of a gathered forest: its own device array (null: host
+// memory, copied by the runtime) and its first body in the forest's array
+struct ForestPart {
+  exafmm_body *src;
+  int64_t off;
+};
 constexpr int kMaxP = 16;         // expansion order cap (NTERM <= 136)
 constexpr int kWave = 64;
 
