@@ -978,6 +978,7 @@ int exafmm_graph_create(exafmm_ctx *const *ctxs, int count, int nstreams, exafmm
       const Ctx &c = ctxs[i]->c;
       if (c.cfg.device != dev) fail(EXAFMM_ERR_ARG, "contexts on different devices");
       if (!c.have_L || c.src != &c) fail(EXAFMM_ERR_STATE, "each context must have been evaluated once");
+      if (c.forest_n > 0) fail(EXAFMM_ERR_UNSUPPORTED, "a forest context cannot be recorded in a graph");
       if (c.host_mode) fail(EXAFMM_ERR_UNSUPPORTED, "graphs need device-resident bodies (on_device = 1)");
       if (c.cfg.images > 0) fail(EXAFMM_ERR_UNSUPPORTED, "periodic contexts cannot be recorded in a graph");
       if (c.graph_locks) fail(EXAFMM_ERR_STATE, "a context is already held by a graph");
@@ -1052,6 +1053,7 @@ int exafmm_graph_create_grouped(exafmm_ctx *const *ctxs, int count, exafmm_graph
       const Ctx &c = ctxs[i]->c;
       if (c.cfg.device != dev) fail(EXAFMM_ERR_ARG, "contexts on different devices");
       if (!c.have_L || c.src != &c) fail(EXAFMM_ERR_STATE, "each context must have been evaluated once");
+      if (c.forest_n > 0) fail(EXAFMM_ERR_UNSUPPORTED, "a forest context cannot be recorded in a graph");
       if (c.host_mode) fail(EXAFMM_ERR_UNSUPPORTED, "graphs need device-resident bodies (on_device = 1)");
       if (c.cfg.images > 0) fail(EXAFMM_ERR_UNSUPPORTED, "periodic contexts cannot be recorded in a graph");
       if (c.cfg.m2l_precision != EXAFMM_PREC_BF16X3 || c.cfg.p2p_precision != EXAFMM_PREC_FP32)

@@ -1405,7 +1405,8 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   }
 
   // per-key orders of a learned-order evaluation (need the multipoles)
-  c.learned = c.cfg.order_policy == EXAFMM_ORDER_LEARNED && c.have_model;
+  // (a forest has no common root to scale the tolerance by: fixed order)
+  c.learned = c.cfg.order_policy == EXAFMM_ORDER_LEARNED && c.have_model && c.forest_n == 0;
   if (c.learned)
     order_select(c, c.tiles.p, c.tiles.p + c.m2l_tstride, c.tiles.p + 2 * c.m2l_tstride,
                  c.tiles.p + 3 * c.m2l_tstride, c.tiles.p + 4 * c.m2l_tstride, ntiles, c.m2l_totalU, kTN, s);

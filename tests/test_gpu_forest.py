@@ -124,6 +124,13 @@ def test_forest_argument_errors(gpu):
         f.evaluate_forest(many, np.arange(fmm.FOREST_MAX + 2))
     assert e.value.code == -1
     f.close()
+    # a forest context cannot be recorded in a graph
+    g = fmm.FMM(P=6)
+    g.evaluate_forest(b.copy(), [0, 10, 30])
+    with pytest.raises(fmm.ExaFMMError) as e:
+        fmm.Graph([g])
+    assert e.value.code == -8
+    g.close()
     p = fmm.FMM(P=6, images=1, cycle=1.0)
     with pytest.raises(fmm.ExaFMMError) as e:
         p.evaluate_forest(b, [0, 10, 30])
