@@ -38,7 +38,7 @@ constexpr int kP2PSeg = 128;      // list entries staged per segment
 constexpr int kP2PChunk = 256;    // sources staged per chunk (4 rounds of 64)
 constexpr int kP2PTC = 16;        // targets per block (plain)
 #ifndef P2P_FLUSH
-#define P2P_FLUSH 32
+#define P2P_FLUSH 64
 #endif
 constexpr int kP2PFlush = P2P_FLUSH;  // rounds per compensated block (Newton + compensated)
 #ifndef P2P_TC_STEP
