@@ -41,6 +41,18 @@ struct Failure {
       ::xf::fail(EXAFMM_ERR_HIP, std::string(#expr) + " -> " + hipGetErrorString(e_));     \
   } while (0)
 
+// Host wait for a stream by polling it.  A blocking wait sleeps and the
+// host wakes tens of microseconds after the stream drains; an evaluation
+// waits mid-pass a few times (tree level sizes, list totals, tile plan),
+// and each such wake-up is time the GPU sits idle.
+inline void stream_wait(hipStream_t s) {
+  for (;;) {
+    const hipError_t e = hipStreamQuery(s);
+    if (e == hipSuccess) return;
+    if (e != hipErrorNotReady) XF_HIP(e);
+  }
+}
+
 #define XF_LAUNCH()                                                                        \
   do {                                                                                     \
     hipError_t e_ = hipGetLastError();                                                     \

@@ -1327,7 +1327,7 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   if (nkeys)
     XF_HIP(hipMemcpyAsync(keys.data(), c.lists.keys.p, sizeof(unsigned long long) * nkeys, hipMemcpyDeviceToHost,
                           s));
-  XF_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   const int totalU = plan[0], ntiles = plan[1], off = plan[2];
   c.m2l_tstride = ntmax;
   if (ntiles == 0) {

@@ -384,7 +384,7 @@ void traverse(Ctx &tc, const Ctx &sc) {
       XF_LAUNCH();
     }
     XF_HIP(hipMemcpyAsync(tot, Lr.tot.p, sizeof(tot), hipMemcpyDeviceToHost, s));
-    XF_HIP(hipStreamSynchronize(s));
+    stream_wait(s);
     const int flags = tot[3];
     if (flags & 4) fail(EXAFMM_ERR_CAPACITY, "traversal work queue overflow (pairs per target cell > 2048)");
     if (flags & 16) fail(EXAFMM_ERR_CAPACITY, "M2L offset out of key range");

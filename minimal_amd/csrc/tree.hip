@@ -614,7 +614,7 @@ void tree_build(Ctx &c) {
   double rootv[4];
   XF_HIP(hipMemcpyAsync(hm, t.misc.p, sizeof(hm), hipMemcpyDeviceToHost, s));
   XF_HIP(hipMemcpyAsync(rootv, root, sizeof(rootv), hipMemcpyDeviceToHost, s));
-  XF_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   if (hm[0] & 1) fail(EXAFMM_ERR_ARG, "non-finite body coordinate or charge");
   for (int d = 0; d < 3; d++) t.X0[d] = rootv[d];
   t.R0 = rootv[3];
@@ -690,7 +690,7 @@ void tree_build(Ctx &c) {
   std::vector<int> lc(kMaxLevel + 2), fl(19);
   XF_HIP(hipMemcpyAsync(lc.data(), t.lcount.p, sizeof(int) * (kMaxLevel + 2), hipMemcpyDeviceToHost, s));
   XF_HIP(hipMemcpyAsync(fl.data(), t.misc.p, sizeof(int) * 19, hipMemcpyDeviceToHost, s));
-  XF_HIP(hipStreamSynchronize(s));
+  stream_wait(s);
   if (fl[0] & 2) fail(EXAFMM_ERR_DEPTH, "more than ncrit bodies share a 21-level octant path");
   t.lstart.assign(1, 0);
   int depth = 0;
