@@ -113,11 +113,11 @@ void do_build(Ctx &c) {
   c.lists.valid = false;
 }
 
-void do_upward(Ctx &c) {
+void do_upward(Ctx &c, hipStream_t s) {
   if (!c.t.built) fail(EXAFMM_ERR_STATE, "upward pass before build_tree");
-  rec(c, 2, c.s0);
-  upward(c);
-  rec(c, 3, c.s0);
+  rec(c, 2, s);
+  upward(c, s);
+  rec(c, 3, s);
 }
 
 // Interaction lists (geometry only: no multipoles needed).
@@ -171,6 +171,23 @@ void do_downward(Ctx &c) {
   l2l_pass(c);
   l2p_pass(c);
   rec(c, 8, c.s0);
+}
+
+// One evaluation over two streams.  P2P needs only the lists (event 14): it
+// runs on s1 beside the upward pass and the M2L tile and operator build on
+// s0, its long VALU-bound blocks filling the CUs those latency-bound chains
+// leave idle; the keyed GEMM follows on s0.  The upward pass is enqueued
+// first so the P2M workgroups are dispatched ahead of P2P's.  (Running the
+// upward pass beside the traversal instead is slower: its workgroups hold
+// the CUs each short traversal launch waits for.)
+void do_passes(Ctx &c) {
+  do_build(c);
+  do_lists(c, c);
+  rec(c, 14, c.s0);
+  do_upward(c, c.s0);
+  do_p2p(c, c, true);
+  do_far(c, c);
+  do_downward(c);
 }
 
 }  // namespace
@@ -312,7 +329,7 @@ int exafmm_get_cells(exafmm_ctx *ctx, exafmm_cell *out, int64_t capacity) {
 int exafmm_upward(exafmm_ctx *ctx) {
   return guard(ctx, [&] {
     if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
-    do_upward(ctx->c);
+    do_upward(ctx->c, ctx->c.s0);
   });
 }
 
@@ -338,18 +355,7 @@ int exafmm_evaluate(exafmm_ctx *ctx, exafmm_body *bodies, int64_t n, int on_devi
     if (!ctx) fail(EXAFMM_ERR_ARG, "null context");
     Ctx &c = ctx->c;
     set_bodies(c, bodies, n, on_device);
-    do_build(c);
-    do_lists(c, c);
-    // P2P needs only the lists (event 14): it runs on s1 beside the upward
-    // pass and the M2L tile and operator build on s0, its long VALU-bound
-    // blocks filling the CUs those latency-bound chains leave idle; the
-    // keyed GEMM follows on s0.  The upward pass is enqueued first so the
-    // P2M workgroups are dispatched ahead of P2P's.
-    rec(c, 14, c.s0);
-    do_upward(c);
-    do_p2p(c, c, true);
-    do_far(c, c);
-    do_downward(c);
+    do_passes(c);
     bodies_to_host(c);
   });
 }
@@ -370,13 +376,7 @@ int exafmm_evaluate_forest(exafmm_ctx *ctx, exafmm_body *bodies, const int64_t *
     c.forest_roots.ensure(nprob);
     // synchronous upload: the caller's offsets need not outlive the call
     XF_HIP(hipMemcpy(c.forest_off.p, offsets, sizeof(int64_t) * (nprob + 1), hipMemcpyHostToDevice));
-    do_build(c);
-    do_lists(c, c);
-    rec(c, 14, c.s0);
-    do_upward(c);
-    do_p2p(c, c, true);
-    do_far(c, c);
-    do_downward(c);
+    do_passes(c);
     bodies_to_host(c);
   });
 }
@@ -764,7 +764,7 @@ void replay(Ctx &c, hipStream_t s) {
     XF_LAUNCH();
     hipLaunchKernelGGL(k_zero, dim3(ceil_div(nl, 256)), dim3(256), 0, s, nl, c.L.p);
     XF_LAUNCH();
-    upward(c);
+    upward(c, s);
     p2p_apply(c, c, s);
     m2l_apply(c, c);
     l2l_pass(c);

@@ -210,11 +210,11 @@ __global__ void k_shift_ops(int P, double *__restrict__ T) {
   }
 }
 
-void shift_ops(Ctx &c) {
+void shift_ops(Ctx &c, hipStream_t s) {
   if (c.shift_P == c.P) return;
   const int PP = c.P * c.P;
   c.shift_T.ensure(size_t(2) * 8 * PP * PP);
-  hipLaunchKernelGGL(k_shift_ops, dim3(ceil_div(2 * 8 * PP, 64)), dim3(64), 0, c.s0, c.P, c.shift_T.p);
+  hipLaunchKernelGGL(k_shift_ops, dim3(ceil_div(2 * 8 * PP, 64)), dim3(64), 0, s, c.P, c.shift_T.p);
   XF_LAUNCH();
   c.shift_P = c.P;
 }
@@ -767,11 +767,10 @@ void l2p_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, h
   XF_LAUNCH();
 }
 
-void upward(Ctx &c) {
+void upward(Ctx &c, hipStream_t s) {
   Tree &t = c.t;
-  hipStream_t s = c.s0;
   const int P = c.P, PP = P * P;
-  shift_ops(c);
+  shift_ops(c, s);
   c.M.ensure(size_t(t.ncells) * PP);
   const int64_t nl = t.nleaves;
   // the per-lane degree loop is unrolled to PMAX (masked past P): the
@@ -811,7 +810,7 @@ void l2l_pass(Ctx &c) {
   Tree &t = c.t;
   hipStream_t s = c.s0;
   const int PP = c.P * c.P;
-  shift_ops(c);
+  shift_ops(c, s);
   for (int l = t.lroot; l < t.depth; l++) {  // parents at level l
     const int cnt = t.lstart[l + 1] - t.lstart[l];
     if (cnt >= kRowTileMinParents)

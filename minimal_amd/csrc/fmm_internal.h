@@ -143,6 +143,7 @@ struct Tree {
   std::vector<int> lstart;       // host copy: level start, size depth+2
   double X0[3] = {0, 0, 0}, R0 = 0;
   int lroot = 0;                 // level of the (problem) roots: kForestLevel in a forest
+  int ds_hint = -1;              // sort depth of the last build (-1: none yet)
 
   Buf<double4> cx;     // centre xyz, radius R
   Buf<int> cbeg, ccnt; // body range in tree order
@@ -272,7 +273,7 @@ void tree_build(Ctx &c);
 void tree_export_cells(Ctx &c, exafmm_cell *out);     // reference numbering
 void tree_ref_order(Ctx &c, std::vector<int64_t> &bfs2ref, std::vector<int64_t> &ref2bfs);
 void traverse(Ctx &tgt, const Ctx &src);
-void upward(Ctx &c);
+void upward(Ctx &c, hipStream_t s);
 void m2l_apply(Ctx &tgt, const Ctx &src);
 void m2l_prepare(Ctx &c, hipStream_t s);
 int m2l_fp64_levels(const Ctx &c);  // levels whose M2L runs per pair in FP64 (m2l.hip)
@@ -310,7 +311,7 @@ struct GemmGroupPlan {
 };
 void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, hipStream_t s, bool tune);
 void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s);
-void shift_ops(Ctx &c);
+void shift_ops(Ctx &c, hipStream_t s);
 void l2l_pass(Ctx &c);
 void l2p_pass(Ctx &c);
 void direct_sum(int device, hipStream_t s, exafmm_body *t, int64_t nt, const exafmm_body *src, int64_t ns);

@@ -560,6 +560,14 @@ __global__ void __launch_bounds__(256) k_writeback(int64_t n, const double4 *__r
   b[i].q = v.w;
 }
 
+// Sort depth from the largest prefix population of levels 1..7 (lmax[l]).
+static int sort_depth(const int *lmax, int64_t n, int ncrit, bool forest, int lforce) {
+  if (n <= ncrit && !forest) return 0;
+  for (int l = std::max(1, lforce); l <= kHistLevels; l++)
+    if (lmax[l] <= ncrit) return l;
+  return kMaxLevel;
+}
+
 void tree_build(Ctx &c) {
   Tree &t = c.t;
   const int64_t n = t.n;
@@ -609,24 +617,22 @@ void tree_build(Ctx &c) {
   hipLaunchKernelGGL(k_hist_top, dim3(1), dim3(1 << (3 * 3)), 0, s, t.hist.p, t.hist_bmax.p, t.misc.p);
   XF_LAUNCH();
 
-  // sync 1: bounds, input check, depth needed
+  // Sort depth Ds: the first level whose most populous prefix holds at
+  // most ncrit bodies (kMaxLevel if none of levels 1..7 does).  It needs the
+  // histogram on the host; instead of waiting for it here the build reuses
+  // the previous call's depth and checks it at the level-size read below.
+  // A depth at or above the true one builds the same tree (a cell splits
+  // only above ncrit bodies, and each leaf keeps its bodies in input order),
+  // so only a smaller one makes the build start over.
+  const bool speculate = t.ds_hint >= 0;
+  int Ds = t.ds_hint;
   int hm[9];
   double rootv[4];
-  XF_HIP(hipMemcpyAsync(hm, t.misc.p, sizeof(hm), hipMemcpyDeviceToHost, s));
-  XF_HIP(hipMemcpyAsync(rootv, root, sizeof(rootv), hipMemcpyDeviceToHost, s));
-  stream_wait(s);
-  if (hm[0] & 1) fail(EXAFMM_ERR_ARG, "non-finite body coordinate or charge");
-  for (int d = 0; d < 3; d++) t.X0[d] = rootv[d];
-  t.R0 = rootv[3];
-  int Ds = kMaxLevel;
-  if (n <= ncrit && !forest) {
-    Ds = 0;
-  } else {
-    for (int l = std::max(1, lforce); l <= kHistLevels; l++)
-      if (hm[l] <= ncrit) {
-        Ds = l;
-        break;
-      }
+  if (!speculate) {
+    XF_HIP(hipMemcpyAsync(hm, t.misc.p, sizeof(hm), hipMemcpyDeviceToHost, s));
+    stream_wait(s);
+    if (hm[0] & 1) fail(EXAFMM_ERR_ARG, "non-finite body coordinate or charge");
+    Ds = sort_depth(hm, n, ncrit, forest, lforce);
   }
 
   // stable sort by the top Ds octant digits
@@ -686,11 +692,26 @@ void tree_build(Ctx &c) {
                      t.lstart_d.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.misc.p);
   XF_LAUNCH();
 
-  // sync 2: level sizes, flags, number of cells that split
+  // level sizes, flags, level maxima, root
   std::vector<int> lc(kMaxLevel + 2), fl(19);
   XF_HIP(hipMemcpyAsync(lc.data(), t.lcount.p, sizeof(int) * (kMaxLevel + 2), hipMemcpyDeviceToHost, s));
   XF_HIP(hipMemcpyAsync(fl.data(), t.misc.p, sizeof(int) * 19, hipMemcpyDeviceToHost, s));
+  XF_HIP(hipMemcpyAsync(rootv, root, sizeof(rootv), hipMemcpyDeviceToHost, s));
   stream_wait(s);
+  if (fl[0] & 1) fail(EXAFMM_ERR_ARG, "non-finite body coordinate or charge");
+  if (speculate) {
+    std::copy(fl.begin(), fl.begin() + 9, hm);
+    const int need = sort_depth(hm, n, ncrit, forest, lforce);
+    t.ds_hint = need;
+    if (need > Ds) {  // sorted too shallow: build again at the right depth
+      tree_build(c);
+      return;
+    }
+  } else {
+    t.ds_hint = Ds;
+  }
+  for (int d = 0; d < 3; d++) t.X0[d] = rootv[d];
+  t.R0 = rootv[3];
   if (fl[0] & 2) fail(EXAFMM_ERR_DEPTH, "more than ncrit bodies share a 21-level octant path");
   t.lstart.assign(1, 0);
   int depth = 0;

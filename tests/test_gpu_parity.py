@@ -15,7 +15,7 @@ import pytest
 
 from conftest import golden
 from minimal_amd import fmm
-from minimal_amd.inputs import BODY_DTYPE, rel_l2
+from minimal_amd.inputs import BODY_DTYPE, plummer, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -175,6 +175,51 @@ def test_errors_are_status_codes(gpu):
     with pytest.raises(fmm.ExaFMMError) as e:
         g.build_tree(c)
     assert e.value.code == -3
+
+
+def test_depth_reuse_matches_fresh(gpu):
+    """A context sorts at its last call's depth (no host round trip before
+    the sort) and checks that depth once the levels are built.  Shallow then
+    clustered input (the depth grows: the build starts over) and clustered
+    then shallow (sorted deeper than needed): every call equals a fresh
+    context bit for bit.  Errors found on a reused context keep their codes."""
+    rng = np.random.default_rng(21)
+    n = 20000
+    shallow = np.zeros(n, BODY_DTYPE)
+    shallow["X"] = rng.random((n, 3))
+    shallow["q"] = rng.uniform(-1, 1, n)
+    deep = np.zeros(n, BODY_DTYPE)
+    deep["X"] = plummer(n, 0.002, 0.5, rng)
+    deep["q"] = rng.uniform(-1, 1, n)
+    f = fmm.FMM(P=6, ncrit=32, autotune=0)
+    depths = []
+    for b in (shallow, deep, shallow, deep):
+        got = b.copy()
+        f.evaluate(got)
+        ref = b.copy()
+        g = fmm.FMM(P=6, ncrit=32, autotune=0)
+        g.evaluate(ref)
+        for k in ("X", "q", "p", "F"):
+            assert np.array_equal(got[k], ref[k]), k
+        assert f.work_counts() == g.work_counts()
+        depths.append(f.work_counts()["depth"])
+        g.close()
+    assert depths[0] <= 5 and depths[1] >= 9, depths
+    bad = shallow.copy()
+    bad["X"][7, 2] = np.inf
+    with pytest.raises(fmm.ExaFMMError) as e:
+        f.evaluate(bad)
+    assert e.value.code == -1
+    same = np.zeros(200, BODY_DTYPE)
+    same["X"] = 0.5
+    same["X"][0] = 0.1
+    h = fmm.FMM(ncrit=8)
+    h.build_tree(shallow[:1000].copy())
+    with pytest.raises(fmm.ExaFMMError) as e:
+        h.build_tree(same)
+    assert e.value.code == -3
+    h.close()
+    f.close()
 
 
 @pytest.mark.parametrize("name", TREES)
