@@ -121,14 +121,14 @@ void do_upward(Ctx &c, hipStream_t s) {
 }
 
 // Interaction lists (geometry only: no multipoles needed).
-void do_lists(Ctx &tc, const Ctx &sc) {
+void do_lists(Ctx &tc, const Ctx &sc, const std::function<void()> &after_kernels = {}) {
   if (!tc.t.built || !sc.t.built) fail(EXAFMM_ERR_STATE, "horizontal pass before build_tree");
   if (tc.graph_locks) fail(EXAFMM_ERR_STATE, "the context is held by a graph (exafmm_graph_destroy it first)");
   if (tc.P != sc.P) fail(EXAFMM_ERR_ARG, "target and source contexts differ in P");
   if (tc.cfg.images != sc.cfg.images || (tc.cfg.images > 0 && &tc != &sc))
     fail(EXAFMM_ERR_UNSUPPORTED, "a periodic horizontal pass needs source == target");
   rec(tc, 11, tc.s0);
-  traverse(tc, sc);
+  traverse(tc, sc, after_kernels);
   rec(tc, 12, tc.s0);
   tc.L.ensure(size_t(tc.t.ncells) * tc.P * tc.P);
   XF_HIP(hipMemsetAsync(tc.L.p, 0, sizeof(double) * tc.t.ncells * tc.P * tc.P, tc.s0));
@@ -173,19 +173,24 @@ void do_downward(Ctx &c) {
   rec(c, 8, c.s0);
 }
 
-// One evaluation over two streams.  P2P needs only the lists (event 14): it
-// runs on s1 beside the upward pass and the M2L tile and operator build on
-// s0, its long VALU-bound blocks filling the CUs those latency-bound chains
-// leave idle; the keyed GEMM follows on s0.  The upward pass is enqueued
-// first so the P2M workgroups are dispatched ahead of P2P's.  (Running the
-// upward pass beside the traversal instead is slower: its workgroups hold
-// the CUs each short traversal launch waits for.)
+// One evaluation over two streams.  P2P needs only the P2P lists: it is
+// launched on s1 as soon as the traversal kernels are enqueued (before the
+// host reads the list totals; a pass that has to be repeated makes it exit
+// at once), and runs beside the rest of the lists, the upward pass and the
+// M2L tile and operator build on s0, its long VALU-bound blocks filling the
+// CUs those latency-bound chains leave idle; the keyed GEMM follows on s0.
+// (Running the upward pass beside the traversal instead is slower: its
+// workgroups hold the CUs each short traversal launch waits for.)
 void do_passes(Ctx &c) {
   do_build(c);
-  do_lists(c, c);
-  rec(c, 14, c.s0);
+  do_lists(c, c, [&c] {
+    rec(c, 14, c.s0);
+    XF_HIP(hipStreamWaitEvent(c.s1, c.ev[14], 0));
+    rec(c, 5, c.s1);
+    p2p_apply(c, c, c.s1, c.lists.tot.p + 3);
+    rec(c, 6, c.s1);
+  });
   do_upward(c, c.s0);
-  do_p2p(c, c, true);
   do_far(c, c);
   do_downward(c);
 }

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <map>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -90,6 +91,49 @@ struct Buf {
   Buf(const Buf &) = delete;
   Buf &operator=(const Buf &) = delete;
 };
+
+// Pinned host memory for the readbacks of a pass.  Device-to-host copies
+// into it are asynchronous, so one wait covers every copy of a read point
+// (a pageable destination turns each copy into a blocking round trip).
+// Resized only while no copy into it is pending.
+template <class T>
+struct HostBuf {
+  T *p = nullptr;
+  size_t cap = 0;
+  void ensure(size_t count) {
+    if (count <= cap && p) return;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t bytes = (count ? count : 1) * sizeof(T);
+    if (hipHostMalloc(reinterpret_cast<void **>(&p), bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      fail(EXAFMM_ERR_ALLOC, "hipHostMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    cap = count ? count : 1;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  ~HostBuf() { release(); }
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+};
+
+// Slots of Ctx::pin_i (ints read back by the tree build, traversal and M2L plan)
+constexpr int kPinLevelMax = 0;   // 9: flags, largest prefix population of levels 1..8
+constexpr int kPinLevels = 16;    // kMaxLevel + 2 level sizes
+constexpr int kPinMisc = 48;      // 19: tree flags and counters
+constexpr int kPinTot = 72;       // 8: list totals and flags
+constexpr int kPinPlan = 80;      // 3: M2L slots, tiles, cells in tiles
+constexpr int kPinInts = 96;
+// traversal flag bits that make the pass repeat: list capacity, key table full
+constexpr int kListRetry = 1 | 8;
+static_assert(kPinLevels + kMaxLevel + 2 <= kPinMisc && kPinMisc + 19 <= kPinTot, "pinned slots overlap");
 
 inline int ceil_div(int64_t a, int64_t b) { return int((a + b - 1) / b); }
 
@@ -183,6 +227,8 @@ struct Lists {
   Buf<int> slot2id;
   Buf<int> scan_tmp;       // device-wide scan partials
   Buf<int64_t> capd;       // list capacities on the device (scan overflow check)
+  HostBuf<int64_t> caps_pin;
+  int64_t caps_up[3] = {-1, -1, -1};  // capacities last uploaded
   Buf<unsigned long long> keys;    // distinct keys, ascending
   Buf<int> kscratch;
   int nkeys = 0;
@@ -194,6 +240,9 @@ struct Ctx {
   hipStream_t s0 = nullptr, s1 = nullptr;
   hipEvent_t ev[16] = {};
   std::string last_error;
+  HostBuf<int> pin_i;                   // kPin* slots
+  HostBuf<double> pin_d;                // root cube
+  HostBuf<unsigned long long> pin_keys; // M2L key set
 
   exafmm_body *bodies = nullptr;   // caller's array (device pointer or our mirror)
   exafmm_body *host_bodies = nullptr;
@@ -272,7 +321,11 @@ struct Ctx {
 void tree_build(Ctx &c);
 void tree_export_cells(Ctx &c, exafmm_cell *out);     // reference numbering
 void tree_ref_order(Ctx &c, std::vector<int64_t> &bfs2ref, std::vector<int64_t> &ref2bfs);
-void traverse(Ctx &tgt, const Ctx &src);
+// Dual-tree traversal into the interaction lists.  after_kernels (if set)
+// runs once per pass, right after the pass's traversal kernels are enqueued
+// on s0 and before the host reads the totals (flags in lists.tot[3]; a pass
+// with kListRetry bits set is repeated after s1 has drained).
+void traverse(Ctx &tgt, const Ctx &src, const std::function<void()> &after_kernels = {});
 void upward(Ctx &c, hipStream_t s);
 void m2l_apply(Ctx &tgt, const Ctx &src);
 void m2l_prepare(Ctx &c, hipStream_t s);
@@ -285,7 +338,7 @@ void order_select(Ctx &c, const int *tfirst, const int *tcount, const int *tbase
 float order_train(Ctx &c, int nsamples, int epochs, float lr, uint64_t seed);
 void mlp_eval_host(Ctx &c, const float *w, const float *X, int64_t n, float *Y);
 void mlp_grad_host(Ctx &c, const float *w, const float *X, const float *T, int64_t n, float *grad, float *loss);  // keyed-M2L tiles and operators (fast precisions only)
-void p2p_apply(Ctx &tgt, const Ctx &src, hipStream_t s);
+void p2p_apply(Ctx &tgt, const Ctx &src, hipStream_t s, const int *list_flags = nullptr);
 // grouped launches over the problems of a batch (exafmm_graph_create)
 int p2m_group_blocks(int64_t nleaves);
 int shift_group_rows(int P);

@@ -1321,13 +1321,15 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   XF_LAUNCH();
   scan_exclusive_dev(d_U, d_base, d_plan, ntmax, c.m2l_misc.p + 256, c.m2l_scan, s);
   // one host round trip: tile count, slots, cells in tiles, and the key set
-  int plan[3] = {0, 0, 0};
-  std::vector<unsigned long long> keys(nkeys);
-  XF_HIP(hipMemcpyAsync(plan, c.m2l_misc.p + 256, sizeof(plan), hipMemcpyDeviceToHost, s));
+  c.pin_i.ensure(kPinInts);
+  c.pin_keys.ensure(std::max(nkeys, 1));
+  XF_HIP(hipMemcpyAsync(c.pin_i.p + kPinPlan, c.m2l_misc.p + 256, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
   if (nkeys)
-    XF_HIP(hipMemcpyAsync(keys.data(), c.lists.keys.p, sizeof(unsigned long long) * nkeys, hipMemcpyDeviceToHost,
+    XF_HIP(hipMemcpyAsync(c.pin_keys.p, c.lists.keys.p, sizeof(unsigned long long) * nkeys, hipMemcpyDeviceToHost,
                           s));
   stream_wait(s);
+  const int *plan = c.pin_i.p + kPinPlan;
+  std::vector<unsigned long long> keys(c.pin_keys.p, c.pin_keys.p + nkeys);
   const int totalU = plan[0], ntiles = plan[1], off = plan[2];
   c.m2l_tstride = ntmax;
   if (ntiles == 0) {

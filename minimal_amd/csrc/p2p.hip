@@ -359,19 +359,27 @@ __device__ __attribute__((always_inline)) inline void p2p_leaf(
   }
 }
 
+// A P2P launched before the host has read the traversal's flags: it does
+// nothing if that pass is to be repeated (list capacity or key table full)
+// or failed (work queue overflow, offset out of key range), as its lists
+// are then incomplete.
+__device__ inline bool p2p_skip(const int *list_flags) {
+  return list_flags && (__atomic_load_n(list_flags, __ATOMIC_RELAXED) & (kListRetry | 4 | 16));
+}
+
 template <bool REFINE>
 __global__ void __launch_bounds__(kP2PWaves *kWave) __attribute__((amdgpu_waves_per_eu(REFINE ? 3 : 4))) k_p2p32(
     const int *__restrict__ leaves, const int *__restrict__ nleaves, const int *__restrict__ cbeg,
     const int *__restrict__ ccnt, const double4 *__restrict__ cx, const float4 *__restrict__ xq,
     const int *__restrict__ lbeg, const int *__restrict__ lcnt, const int *__restrict__ list,
     const int *__restrict__ scbeg, const int *__restrict__ sccnt, const double4 *__restrict__ sxs, double cycle,
-    int same_tree, exafmm_body *__restrict__ bodies) {
+    int same_tree, exafmm_body *__restrict__ bodies, const int *__restrict__ list_flags) {
   __shared__ P2PLds lds[kP2PWaves];
   // wave-uniform (scalar) leaf index: the per-leaf quantities, the block
   // targets and the loop bounds stay in scalar registers
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int li = blockIdx.x * kP2PWaves + w;
-  if (li >= *nleaves) return;  // whole wave; the kernel uses wave-level syncs only
+  if (li >= *nleaves || p2p_skip(list_flags)) return;  // whole wave; the kernel uses wave-level syncs only
   p2p_leaf<REFINE>(li, leaves, cbeg, ccnt, cx, xq, lbeg, lcnt, list, scbeg, sccnt, sxs, cycle, same_tree, bodies,
                    lds[w]);
 }
@@ -400,10 +408,11 @@ __global__ void __launch_bounds__(64) k_p2p64(const int *__restrict__ leaves, co
                                               const int *__restrict__ lcnt, const int *__restrict__ list,
                                               const int *__restrict__ scbeg, const int *__restrict__ sccnt,
                                               const double4 *__restrict__ sxs, double cycle,
-                                              exafmm_body *__restrict__ bodies) {
+                                              exafmm_body *__restrict__ bodies,
+                                              const int *__restrict__ list_flags) {
 #pragma clang fp contract(off)
   const int li = blockIdx.x;
-  if (li >= *nleaves) return;
+  if (li >= *nleaves || p2p_skip(list_flags)) return;
   const int c = leaves[li];
   const int tb = cbeg[c], tn = ccnt[c];
   const int b0 = lbeg[c], nl = lcnt[c];
@@ -446,18 +455,19 @@ void p2p_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, h
   XF_LAUNCH();
 }
 
-void p2p_apply(Ctx &tc, const Ctx &sc, hipStream_t s) {
+void p2p_apply(Ctx &tc, const Ctx &sc, hipStream_t s, const int *list_flags) {
   const Tree &T = tc.t, &S = sc.t;
   const Lists &L = tc.lists;
   const double cycle = tc.cfg.images > 0 ? tc.cfg.cycle : 0.0;
   if (tc.cfg.p2p_precision == EXAFMM_PREC_FP64) {
     hipLaunchKernelGGL(k_p2p64, dim3(T.nleaves), dim3(64), 0, s, T.leaves.p, T.misc.p + 17, T.cbeg.p, T.ccnt.p,
-                       T.xs.p, L.p2p_beg.p, L.p2p_cnt.p, L.p2p.p, S.cbeg.p, S.ccnt.p, S.xs.p, cycle, tc.bodies);
+                       T.xs.p, L.p2p_beg.p, L.p2p_cnt.p, L.p2p.p, S.cbeg.p, S.ccnt.p, S.xs.p, cycle, tc.bodies,
+                       list_flags);
   } else {
     auto *k = tc.cfg.p2p_precision == EXAFMM_PREC_FP32_NEWTON ? k_p2p32<true> : k_p2p32<false>;
     hipLaunchKernelGGL(k, dim3(ceil_div(T.nleaves, kP2PWaves)), dim3(kP2PWaves * kWave), 0, s, T.leaves.p,
                        T.misc.p + 17, T.cbeg.p, T.ccnt.p, T.cx.p, T.xq.p, L.p2p_beg.p, L.p2p_cnt.p, L.p2p.p,
-                       S.cbeg.p, S.ccnt.p, S.xs.p, cycle, int(&tc == &sc), tc.bodies);
+                       S.cbeg.p, S.ccnt.p, S.xs.p, cycle, int(&tc == &sc), tc.bodies, list_flags);
   }
   XF_LAUNCH();
 }

@@ -71,8 +71,12 @@ __device__ inline unsigned long long mix64(unsigned long long x) {
 // nothing else.  Layout: dx,dy,dz 16 bits each, image 5, level 5, diff 6.
 __device__ inline bool m2l_key(double4 Xc, int lc, double4 Xb, int lb, int img, int periodic,
                                unsigned long long &key) {
-  const double rf = fmin(Xc.w, Xb.w);
-  const double dx = rint((Xc.x - Xb.x) / rf), dy = rint((Xc.y - Xb.y) / rf), dz = rint((Xc.z - Xb.z) / rf);
+  // The offset is a whole number of finer radii (up to the rounding of the
+  // centres), so one reciprocal serves the three components: its relative
+  // error moves an offset of at most 32767 by far less than the 0.5 rint
+  // tolerates.
+  const double inv = 1.0 / fmin(Xc.w, Xb.w);
+  const double dx = rint((Xc.x - Xb.x) * inv), dy = rint((Xc.y - Xb.y) * inv), dz = rint((Xc.z - Xb.z) * inv);
   const double lim = 32767.0;
   const int ld = lc - lb;
   if (fabs(dx) > lim || fabs(dy) > lim || fabs(dz) > lim || ld < -31 || ld > 31) return false;
@@ -160,7 +164,7 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
   }
   while (head < tail) {
     const int k = min(kWave, tail - head);
-    int b = -1, kind = -1, nchb = 0;
+    int b = -1, kind = -1, nchb = 0, lb = 0;
     double4 Xb;
     int img = 13;
     if (lane < k) {
@@ -169,6 +173,7 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
       img = e & 31;
       Xb = a.scx[b];
       nchb = a.scnch[b];
+      if (FILL && a.want_keys) lb = a.sclevel[b];  // loaded beside the centre, ahead of the MAC
       const double sx = ((img % 3) - 1) * a.cycle, sy = (((img / 3) % 3) - 1) * a.cycle,
                    sz = ((img / 9) - 1) * a.cycle;
       bool mac;
@@ -195,7 +200,7 @@ __global__ void __launch_bounds__(kTravWaves *kWave) k_traverse(TravArgs a) {
         int slot = 0;
         if (a.want_keys) {
           unsigned long long key;
-          if (m2l_key(Xc, lc, Xb, a.sclevel[b], img, a.periodic, key))
+          if (m2l_key(Xc, lc, Xb, lb, img, a.periodic, key))
             slot = key_insert(a.hslots, a.hmask, key, a.flags);
           else
             atomicOr(a.flags, 16);
@@ -255,20 +260,31 @@ __global__ void k_hash_collect(const unsigned long long *__restrict__ slots, int
 
 // M2L entries carry their hash slot out of the traversal; after the keys are
 // sorted the slot becomes the key's rank (the id the operators are stored by).
-__global__ void k_entry_ids(int2 *__restrict__ m2l, int64_t n, const int *__restrict__ slot2id) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i < n) m2l[i].y = slot2id[m2l[i].y];
+// The count is the device total, capped at the list capacity; flags (if
+// given) skip a pass that is repeated or ranked by the radix sort instead.
+__global__ void k_entry_ids(int2 *__restrict__ m2l, const int *__restrict__ total, int64_t cap,
+                            const int *__restrict__ slot2id, const int *__restrict__ flags) {
+  if (flags && (*flags & (1 | 8 | 32))) return;
+  const int64_t n = min(int64_t(*total), cap);
+  for (int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x; i < n; i += int64_t(gridDim.x) * 256)
+    m2l[i].y = slot2id[m2l[i].y];
 }
 
 // Key sort for small key sets (the usual case: hundreds to a few thousand
 // distinct translations): one workgroup, keys in LDS, each key's rank is the
 // number of smaller keys (keys are distinct).  Writes the sorted keys in
-// place and the slot -> rank map.
+// place and the slot -> rank map.  A larger set raises bit 32 of *flags
+// and is left to the radix sort.
 constexpr int kRankMax = 6144;
 __global__ void __launch_bounds__(1024) k_key_rank(unsigned long long *__restrict__ keys, const int *__restrict__ sl,
-                                                   const int *__restrict__ count, int *__restrict__ slot2id) {
+                                                   const int *__restrict__ count, int *__restrict__ slot2id,
+                                                   int *__restrict__ flags) {
   __shared__ unsigned long long K[kRankMax];
   const int n = *count;
+  if (n > kRankMax) {
+    if (threadIdx.x == 0) atomicOr(flags, 32);
+    return;
+  }
   for (int i = threadIdx.x; i < n; i += 1024) K[i] = keys[i];
   __syncthreads();
   for (int i = threadIdx.x; i < n; i += 1024) {
@@ -285,7 +301,7 @@ __global__ void k_slot_ids(const int *__restrict__ sl, int n, int *__restrict__ 
   if (i < n) slot2id[sl[i]] = i;
 }
 
-void traverse(Ctx &tc, const Ctx &sc) {
+void traverse(Ctx &tc, const Ctx &sc, const std::function<void()> &after_kernels) {
   Tree &T = tc.t;
   const Tree &S = sc.t;
   Lists &Lr = tc.lists;
@@ -312,9 +328,16 @@ void traverse(Ctx &tc, const Ctx &sc) {
     Lr.p2p.ensure(Lr.cap_p2p);
     Lr.out.ensure(Lr.cap_out);
     XF_HIP(hipMemsetAsync(Lr.tot.p, 0, 8 * sizeof(int), s));
+    // capacities on the device, uploaded when they change (from pinned
+    // memory: an asynchronous copy)
     const int64_t caps[3] = {Lr.cap_m2l, Lr.cap_p2p, Lr.cap_out};
-    Lr.capd.ensure(3);
-    XF_HIP(hipMemcpyAsync(Lr.capd.p, caps, sizeof(caps), hipMemcpyHostToDevice, s));
+    if (!Lr.capd.p || !std::equal(caps, caps + 3, Lr.caps_up)) {
+      Lr.capd.ensure(3);
+      Lr.caps_pin.ensure(3);
+      std::copy(caps, caps + 3, Lr.caps_pin.p);
+      XF_HIP(hipMemcpyAsync(Lr.capd.p, Lr.caps_pin.p, sizeof(caps), hipMemcpyHostToDevice, s));
+      std::copy(caps, caps + 3, Lr.caps_up);
+    }
     if (keys) {
       Lr.hslots.ensure(Lr.hcap);
       hipLaunchKernelGGL(k_hash_clear, dim3(ceil_div(Lr.hcap, 256)), dim3(256), 0, s, Lr.hslots.p, Lr.hcap);
@@ -373,6 +396,7 @@ void traverse(Ctx &tc, const Ctx &sc) {
       hipLaunchKernelGGL(k_traverse<true>, dim3(g), dim3(kTravWaves * kWave), 0, s, a);
       XF_LAUNCH();
     }
+    if (after_kernels) after_kernels();
     if (keys) {
       // distinct translation keys out of the hash table (discarded if the
       // pass has to be repeated), counted into tot[6]
@@ -382,10 +406,24 @@ void traverse(Ctx &tc, const Ctx &sc) {
       hipLaunchKernelGGL(k_hash_collect, dim3(ceil_div(Lr.hcap, 256)), dim3(256), 0, s, Lr.hslots.p, Lr.hcap,
                          Lr.keys.p, Lr.kscratch.p, Lr.tot.p + 6);
       XF_LAUNCH();
+      // The usual small key set is ranked, and the entries renumbered,
+      // before the totals are read, so the read overlaps them (counts from
+      // the device; a repeated pass redoes it all, a set above kRankMax is
+      // flagged and sorted below).
+      hipLaunchKernelGGL(k_key_rank, dim3(1), dim3(1024), 0, s, Lr.keys.p, Lr.kscratch.p, Lr.tot.p + 6,
+                         Lr.slot2id.p, Lr.tot.p + 3);
+      XF_LAUNCH();
+      hipLaunchKernelGGL(k_entry_ids, dim3(int(std::min<int64_t>(ceil_div(Lr.cap_m2l, 256), 4096))), dim3(256), 0, s,
+                         Lr.m2l.p, Lr.tot.p, Lr.cap_m2l, Lr.slot2id.p, Lr.tot.p + 3);
+      XF_LAUNCH();
     }
-    XF_HIP(hipMemcpyAsync(tot, Lr.tot.p, sizeof(tot), hipMemcpyDeviceToHost, s));
+    tc.pin_i.ensure(kPinInts);
+    XF_HIP(hipMemcpyAsync(tc.pin_i.p + kPinTot, Lr.tot.p, sizeof(tot), hipMemcpyDeviceToHost, s));
     stream_wait(s);
+    std::copy(tc.pin_i.p + kPinTot, tc.pin_i.p + kPinTot + 8, tot);
     const int flags = tot[3];
+    // work started by after_kernels ends before a repeated pass resets the flags
+    if (after_kernels && (flags & kListRetry)) stream_wait(tc.s1);
     if (flags & 4) fail(EXAFMM_ERR_CAPACITY, "traversal work queue overflow (pairs per target cell > 2048)");
     if (flags & 16) fail(EXAFMM_ERR_CAPACITY, "M2L offset out of key range");
     if (flags & 1) {  // list capacity: retry with the exact totals
@@ -407,24 +445,20 @@ void traverse(Ctx &tc, const Ctx &sc) {
   Lr.valid = true;
   Lr.nkeys = 0;
   if (keys) {
-    const int *cnt = Lr.tot.p + 6;
     unsigned long long *k1 = Lr.keys.p, *k2 = Lr.keys.p + Lr.hcap;
     int *v1 = Lr.kscratch.p, *v2 = Lr.kscratch.p + Lr.hcap;
     const int nk = tot[6];
-    if (nk <= kRankMax) {
-      hipLaunchKernelGGL(k_key_rank, dim3(1), dim3(1024), 0, s, k1, v1, cnt, Lr.slot2id.p);
-      XF_LAUNCH();
-    } else {
+    if (tot[3] & 32) {  // more keys than k_key_rank holds
       radix_sort_pairs(k1, v1, k2, v2, nk, 0, 64, T.rs_hist, s);
       hipLaunchKernelGGL(k_slot_ids, dim3(ceil_div(std::max(nk, 1), 256)), dim3(256), 0, s, v1, nk, Lr.slot2id.p);
       XF_LAUNCH();
       if (k1 != Lr.keys.p)
         XF_HIP(hipMemcpyAsync(Lr.keys.p, k1, sizeof(unsigned long long) * nk, hipMemcpyDeviceToDevice, s));
-    }
-    if (Lr.nm2l > 0) {
-      hipLaunchKernelGGL(k_entry_ids, dim3(ceil_div(Lr.nm2l, 256)), dim3(256), 0, s, Lr.m2l.p, Lr.nm2l,
-                         Lr.slot2id.p);
-      XF_LAUNCH();
+      if (Lr.nm2l > 0) {
+        hipLaunchKernelGGL(k_entry_ids, dim3(std::min(ceil_div(Lr.nm2l, 256), 4096)), dim3(256), 0, s, Lr.m2l.p,
+                           Lr.tot.p, Lr.cap_m2l, Lr.slot2id.p, (const int *)nullptr);
+        XF_LAUNCH();
+      }
     }
     Lr.nkeys = nk;
   }

@@ -501,6 +501,7 @@ __global__ void __launch_bounds__(256) k_last_level(int l, int ncrit, const int 
                                                     int *__restrict__ cchild, int *__restrict__ cnch,
                                                     int *__restrict__ flags) {
   const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t == 0) flags[19] = lstart[l] + lcount[l];  // cells in the tree
   if (t >= lcount[l]) return;
   const int c = lstart[l] + t;
   cnch[c] = 0;
@@ -508,14 +509,18 @@ __global__ void __launch_bounds__(256) k_last_level(int l, int ncrit, const int 
   if (ccnt[c] > ncrit) atomicOr(flags, 2);
 }
 
-__global__ void __launch_bounds__(256) k_leaf_flag(int ncells, const int *__restrict__ cnch, int *__restrict__ f) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c < ncells) f[c] = cnch[c] == 0;
+// Leaf flags and the compacted leaf list over the *ncells cells (a device
+// count: grid-stride loops over a grid sized from a host bound).
+__global__ void __launch_bounds__(256) k_leaf_flag(const int *__restrict__ ncells, const int *__restrict__ cnch,
+                                                   int *__restrict__ f) {
+  const int nc = *ncells;
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < nc; c += gridDim.x * 256) f[c] = cnch[c] == 0;
 }
-__global__ void __launch_bounds__(256) k_leaf_compact(int ncells, const int *__restrict__ cnch,
+__global__ void __launch_bounds__(256) k_leaf_compact(const int *__restrict__ ncells, const int *__restrict__ cnch,
                                                       const int *__restrict__ off, int *__restrict__ leaves) {
-  const int c = blockIdx.x * 256 + threadIdx.x;
-  if (c < ncells && cnch[c] == 0) leaves[off[c]] = c;
+  const int nc = *ncells;
+  for (int c = blockIdx.x * 256 + threadIdx.x; c < nc; c += gridDim.x * 256)
+    if (cnch[c] == 0) leaves[off[c]] = c;
 }
 
 // One workgroup per leaf: restore input order inside the leaf (rank of
@@ -526,33 +531,40 @@ __global__ void __launch_bounds__(64) k_leaf_gather(const int *__restrict__ leav
                                                     const double4 *__restrict__ cx, const int *__restrict__ val,
                                                     const exafmm_body *__restrict__ b, int *__restrict__ perm,
                                                     double4 *__restrict__ xs, float4 *__restrict__ xq,
-                                                    int *__restrict__ body_leaf) {
-  extern __shared__ int sidx[];
-  const int li = blockIdx.x;
-  if (li >= *nleaves) return;
-  const int c = leaves[li];
-  const int beg = cbeg[c], cnt = ccnt[c];
-  for (int i = threadIdx.x; i < cnt; i += 64) sidx[i] = val[beg + i];
-  __syncthreads();
-  const double4 C = cx[c];
-  for (int i = threadIdx.x; i < cnt; i += 64) {
-    const int me = sidx[i];
-    int r = 0;
-    for (int j = 0; j < cnt; j++) r += sidx[j] < me;
-    const int pos = beg + r;
-    perm[pos] = me;
-    const double x = b[me].X[0], y = b[me].X[1], z = b[me].X[2], q = b[me].q;
-    xs[pos] = make_double4(x, y, z, q);
-    xq[pos] = make_float4(float(x - C.x), float(y - C.y), float(z - C.z), float(q));
-    body_leaf[pos] = c;
+                                                    int *__restrict__ body_leaf, int ncrit) {
+  extern __shared__ int sidx[];  // ncrit ints
+  const int nl = *nleaves;
+  for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+    const int c = leaves[li];
+    const int beg = cbeg[c], cnt = ccnt[c];
+    // a leaf above ncrit: a tree the host rejects or builds again (flag 2)
+    if (cnt > ncrit) continue;
+    for (int i = threadIdx.x; i < cnt; i += 64) sidx[i] = val[beg + i];
+    __syncthreads();
+    const double4 C = cx[c];
+    for (int i = threadIdx.x; i < cnt; i += 64) {
+      const int me = sidx[i];
+      int r = 0;
+      for (int j = 0; j < cnt; j++) r += sidx[j] < me;
+      const int pos = beg + r;
+      perm[pos] = me;
+      const double x = b[me].X[0], y = b[me].X[1], z = b[me].X[2], q = b[me].q;
+      xs[pos] = make_double4(x, y, z, q);
+      xq[pos] = make_float4(float(x - C.x), float(y - C.y), float(z - C.z), float(q));
+      body_leaf[pos] = c;
+    }
+    __syncthreads();
   }
 }
 
 // Reference permutes X and q of the caller's array in place (p, F stay).
+// Bodies back to the caller in tree order -- unless the build flagged a
+// non-finite input (bit 1) or a leaf above ncrit (bit 2): the caller's
+// array is then left as it was.
 __global__ void __launch_bounds__(256) k_writeback(int64_t n, const double4 *__restrict__ xs,
-                                                   exafmm_body *__restrict__ b) {
+                                                   exafmm_body *__restrict__ b, const int *__restrict__ flags) {
   const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (*flags & 3)) return;
   const double4 v = xs[i];
   b[i].X[0] = v.x;
   b[i].X[1] = v.y;
@@ -626,10 +638,11 @@ void tree_build(Ctx &c) {
   // so only a smaller one makes the build start over.
   const bool speculate = t.ds_hint >= 0;
   int Ds = t.ds_hint;
-  int hm[9];
-  double rootv[4];
+  c.pin_i.ensure(kPinInts);
+  c.pin_d.ensure(4);
+  int *hm = c.pin_i.p + kPinLevelMax;
   if (!speculate) {
-    XF_HIP(hipMemcpyAsync(hm, t.misc.p, sizeof(hm), hipMemcpyDeviceToHost, s));
+    XF_HIP(hipMemcpyAsync(hm, t.misc.p, sizeof(int) * 9, hipMemcpyDeviceToHost, s));
     stream_wait(s);
     if (hm[0] & 1) fail(EXAFMM_ERR_ARG, "non-finite body coordinate or charge");
     Ds = sort_depth(hm, n, ncrit, forest, lforce);
@@ -662,9 +675,15 @@ void tree_build(Ctx &c) {
   XF_HIP(hipMemsetAsync(t.lcount.p, 0, sizeof(int) * (kMaxLevel + 2), s));
   int64_t ubmax = 1;
   for (int l = 0; l <= Ds; l++) ubmax = std::max(ubmax, ub[l]);
-  t.tmp_nch.ensure(ubmax);
-  t.tmp_off.ensure(ubmax);
+  // (the leaf flags and offsets below span all cells: ubtot)
+  t.tmp_nch.ensure(ubtot);
+  t.tmp_off.ensure(ubtot);
   t.tmp_bnd.ensure(ubmax * 9);
+  t.leaves.ensure(ubtot);
+  t.perm.ensure(n);
+  t.xs.ensure(n);
+  t.xq.ensure(n);
+  t.body_leaf.ensure(n);
   hipLaunchKernelGGL(k_root, dim3(1), dim3(1), 0, s, n, root, t.cx.p, t.cbeg.p, t.ccnt.p, t.cparent.p, t.clevel.p,
                      t.lcount.p, t.lstart_d.p);
   XF_LAUNCH();
@@ -692,16 +711,37 @@ void tree_build(Ctx &c) {
                      t.lstart_d.p, t.ccnt.p, t.cchild.p, t.cnch.p, t.misc.p);
   XF_LAUNCH();
 
+  // Leaves, the gather into tree order and the write-back run before the
+  // level sizes are read, so that read overlaps them: the cell count comes
+  // from the device (misc[19]), the leaf count too (misc[17]), and grids are
+  // sized from the bound ubtot.  A tree the checks below reject or rebuild
+  // (flag bit 1 or 2) leaves the caller's bodies untouched.
+  const int gcells = int(std::min<int64_t>(ceil_div(ubtot, 256), 2048));
+  hipLaunchKernelGGL(k_leaf_flag, dim3(gcells), dim3(256), 0, s, t.misc.p + 19, t.cnch.p, t.tmp_nch.p);
+  XF_LAUNCH();
+  scan_exclusive_dev(t.tmp_nch.p, t.tmp_off.p, t.misc.p + 19, int(std::min<int64_t>(ubtot, INT32_MAX)), t.misc.p + 17,
+                     t.scan_tmp, s);
+  hipLaunchKernelGGL(k_leaf_compact, dim3(gcells), dim3(256), 0, s, t.misc.p + 19, t.cnch.p, t.tmp_off.p,
+                     t.leaves.p);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_leaf_gather, dim3(int(std::min<int64_t>(ubtot, 16384))), dim3(64), sizeof(int) * ncrit, s,
+                     t.leaves.p, t.misc.p + 17, t.cbeg.p, t.ccnt.p, t.cx.p, v1, c.bodies, t.perm.p, t.xs.p, t.xq.p,
+                     t.body_leaf.p, ncrit);
+  XF_LAUNCH();
+  hipLaunchKernelGGL(k_writeback, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, t.xs.p, c.bodies, t.misc.p);
+  XF_LAUNCH();
+
   // level sizes, flags, level maxima, root
-  std::vector<int> lc(kMaxLevel + 2), fl(19);
-  XF_HIP(hipMemcpyAsync(lc.data(), t.lcount.p, sizeof(int) * (kMaxLevel + 2), hipMemcpyDeviceToHost, s));
-  XF_HIP(hipMemcpyAsync(fl.data(), t.misc.p, sizeof(int) * 19, hipMemcpyDeviceToHost, s));
-  XF_HIP(hipMemcpyAsync(rootv, root, sizeof(rootv), hipMemcpyDeviceToHost, s));
+  const int *lc = c.pin_i.p + kPinLevels, *fl = c.pin_i.p + kPinMisc;
+  const double *rootv = c.pin_d.p;
+  XF_HIP(hipMemcpyAsync(c.pin_i.p + kPinLevels, t.lcount.p, sizeof(int) * (kMaxLevel + 2), hipMemcpyDeviceToHost,
+                        s));
+  XF_HIP(hipMemcpyAsync(c.pin_i.p + kPinMisc, t.misc.p, sizeof(int) * 19, hipMemcpyDeviceToHost, s));
+  XF_HIP(hipMemcpyAsync(c.pin_d.p, root, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
   stream_wait(s);
   if (fl[0] & 1) fail(EXAFMM_ERR_ARG, "non-finite body coordinate or charge");
   if (speculate) {
-    std::copy(fl.begin(), fl.begin() + 9, hm);
-    const int need = sort_depth(hm, n, ncrit, forest, lforce);
+    const int need = sort_depth(fl, n, ncrit, forest, lforce);
     t.ds_hint = need;
     if (need > Ds) {  // sorted too shallow: build again at the right depth
       tree_build(c);
@@ -722,29 +762,8 @@ void tree_build(Ctx &c) {
   }
   t.depth = depth;
   t.ncells = t.lstart.back();
-
-  // leaves
   const int nc = int(t.ncells);
-  t.tmp_nch.ensure(nc);
-  t.tmp_off.ensure(nc);
-  t.leaves.ensure(nc);
-  hipLaunchKernelGGL(k_leaf_flag, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, t.cnch.p, t.tmp_nch.p);
-  XF_LAUNCH();
-  scan_exclusive(t.tmp_nch.p, t.tmp_off.p, nc, t.misc.p + 17, t.scan_tmp, s);
-  hipLaunchKernelGGL(k_leaf_compact, dim3(ceil_div(nc, 256)), dim3(256), 0, s, nc, t.cnch.p, t.tmp_off.p,
-                     t.leaves.p);
-  XF_LAUNCH();
-
-  t.perm.ensure(n);
-  t.xs.ensure(n);
-  t.xq.ensure(n);
-  t.body_leaf.ensure(n);
-  hipLaunchKernelGGL(k_leaf_gather, dim3(nc), dim3(64), sizeof(int) * ncrit, s, t.leaves.p, t.misc.p + 17, t.cbeg.p,
-                     t.ccnt.p, t.cx.p, v1, c.bodies, t.perm.p, t.xs.p, t.xq.p, t.body_leaf.p);
-  XF_LAUNCH();
-  hipLaunchKernelGGL(k_writeback, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, t.xs.p, c.bodies);
-  XF_LAUNCH();
-  // every cell either split or is a leaf (the scan above writes the same
+  // every cell either split or is a leaf (the scan above wrote the same
   // count to misc[17] for the kernels)
   t.nleaves = nc - fl[18];
   t.built = true;

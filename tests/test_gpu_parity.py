@@ -177,6 +177,39 @@ def test_errors_are_status_codes(gpu):
     assert e.value.code == -3
 
 
+def test_list_capacity_retry(gpu):
+    """A small opening angle overflows the first pass's list capacities (200
+    M2L and 96 P2P entries per cell; theta = 0.3 gives 280 and 117 here):
+    the traversal runs again with larger lists.  P2P
+    is launched before the host reads the totals, so the abandoned pass's
+    P2P must do nothing: the retried call equals a second call on the same
+    context (capacities already large, one pass) bit for bit, and its error
+    against the FP64 direct sum is the method's: the FP64 oracle FMM gives
+    p 1.89e-4, F 2.17e-4 on this input (P = 4, theta = 0.3); bound 2.2e-4,
+    2.5e-4 (a lost or doubled P2P would be off by O(1))."""
+    rng = np.random.default_rng(8)
+    n = 40000
+    b0 = np.zeros(n, BODY_DTYPE)
+    b0["X"] = rng.random((n, 3))
+    b0["q"] = rng.uniform(-1, 1, n)
+    f = fmm.FMM(P=4, ncrit=32, theta=0.3, autotune=0)
+    first = b0.copy()
+    f.evaluate(first)
+    w = f.work_counts()
+    assert w["m2l_pairs"] > 200 * w["cells"] and w["p2p_pairs"] > 96 * w["cells"], w
+    second = b0.copy()
+    f.evaluate(second)
+    for k in ("X", "p", "F"):
+        assert np.array_equal(first[k], second[k]), k
+    idx = np.arange(0, n, 97)
+    ref = first[idx].copy()
+    ref["p"] = 0
+    ref["F"] = 0
+    f.direct(ref, first.copy())
+    assert rel_l2(first["p"][idx], ref["p"]) <= 2.2e-4 and rel_l2(first["F"][idx], ref["F"]) <= 2.5e-4
+    f.close()
+
+
 def test_depth_reuse_matches_fresh(gpu):
     """A context sorts at its last call's depth (no host round trip before
     the sort) and checks that depth once the levels are built.  Shallow then

@@ -15,7 +15,7 @@ import pytest
 
 from conftest import golden
 from minimal_amd import fmm
-from minimal_amd.inputs import rel_l2
+from minimal_amd.inputs import BODY_DTYPE, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -111,3 +111,30 @@ def test_periodic_errors(gpu):
         f.dipole_correction()
     with pytest.raises(fmm.ExaFMMError):
         fmm.FMM(images=2, cycle=0.0)
+
+
+def test_large_key_set(gpu):
+    """More distinct M2L translations than the one-workgroup key rank holds
+    (6144: periodic keys carry the image and the level), so the keys are
+    radix-sorted after the totals are read.  The keyed GEMM on that
+    numbering against the FP64 per-pair M2L on the same tree:
+    relative L2 <= 2e-5 (p, F)."""
+    rng = np.random.default_rng(5)
+    n = 1 << 14
+    b0 = np.zeros(n, BODY_DTYPE)
+    b0["X"] = rng.random((n, 3)) * CYCLE
+    b0["q"] = rng.uniform(-1, 1, n)
+    b0["q"] -= b0["q"].mean()
+    out = {}
+    for prec in (fmm.PREC_BF16X3, fmm.PREC_FP64):
+        f = fmm.FMM(P=6, ncrit=32, theta=0.4, images=2, cycle=CYCLE, m2l_precision=prec, autotune=0)
+        b = b0.copy()
+        f.evaluate(b)
+        out[prec] = (b, f.work_counts()["m2l_keys"])
+        f.close()
+    (fast, nk), (ref, _) = out[fmm.PREC_BF16X3], out[fmm.PREC_FP64]
+    assert nk > 6144, nk
+    assert np.array_equal(fast["X"], ref["X"])
+    ep, eF = rel_l2(fast["p"], ref["p"]), rel_l2(fast["F"], ref["F"])
+    print(f"large key set: {nk} keys, p {ep:.3e} F {eF:.3e}")
+    assert ep <= 2e-5 and eF <= 2e-5
