@@ -31,7 +31,17 @@
 namespace xf {
 using namespace xfd;
 
-constexpr int kTN = 256;  // targets per tile: 4 column groups x 4 column tiles of 16
+// GEMM workgroup: 8 waves = column groups of 64 targets x kKPar k-parities
+// (a wave multiplies the k-steps kk = kp, kp + kKPar, ... of its columns).
+constexpr int kGemmThreads = 512;
+constexpr int kGemmWaves = kGemmThreads / 64;
+#ifndef M2L_KPAR
+#define M2L_KPAR 1
+#endif
+constexpr int kKPar = M2L_KPAR;
+constexpr int kColGroups = kGemmWaves / kKPar;
+constexpr int kTN = 64 * kColGroups;  // targets per tile (512: 8 column groups x 4 column tiles of 16)
+static_assert(kKPar == 1 || kKPar == 2, "one or two k-parities");
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -124,13 +134,13 @@ __global__ void __launch_bounds__(256) k_group_hist(int ncells, const unsigned l
 // (entries (source << 5 | image, key id)), kTileUnroll entries in flight.
 
 // Distinct keys of one tile: bitmap over key ids in LDS.
-// Four threads per column (blockDim.x == 4 kTN), entries i = sub mod 4.
+// Four threads per column (blockDim.x / 4 columns at a time), entries i = sub mod 4.
 __device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, const int *cnt, const int2 *list,
                                    unsigned *bm, int nwords) {
   for (int i = threadIdx.x; i < nwords; i += blockDim.x) bm[i] = 0;
   __syncthreads();
-  const int t = threadIdx.x >> 2, sub = threadIdx.x & 3;
-  if (t < ncol) {
+  const int sub = threadIdx.x & 3;
+  for (int t = threadIdx.x >> 2; t < ncol; t += blockDim.x >> 2) {
     const int c = cells[t];
     const int b = beg[c], n = cnt[c];
     for (int i0 = sub; i0 < n; i0 += 4 * kTileUnroll) {
@@ -176,7 +186,8 @@ __global__ void __launch_bounds__(256) k_tile_plan(const int *__restrict__ gc, i
   }
 }
 
-__global__ void __launch_bounds__(4 * kTN) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
+constexpr int kTileCountThreads = 4 * kTN < 1024 ? 4 * kTN : 1024;
+__global__ void __launch_bounds__(kTileCountThreads) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                     const int *__restrict__ sorted, const int *__restrict__ beg,
                                                     const int *__restrict__ cnt, const int2 *__restrict__ list,
                                                     int nkeys, const int *__restrict__ plan, int *__restrict__ tU,
@@ -419,18 +430,19 @@ __global__ void __launch_bounds__(256) k_ops_gen(int P, int fillP, int R, int K4
 }
 
 // ---------------------------------------------------------------- GEMM
-// One workgroup = one tile of up to 256 targets and a block of RTB row
-// tiles (blockIdx.y).  8 waves = 4 column groups (64 targets = 4 column
-// tiles of 16) x 2 k-parities: wave (g, kp) multiplies the k-steps kk = kp,
-// kp+2, ... of every key for its 64 targets and all RTB row tiles, so each
-// source multipole is fetched once per tile; the two parities are added
-// through LDS at the end.  Per key the block's operator rows are copied
+// One workgroup = one tile of up to kTN targets and a block of RTB row
+// tiles (blockIdx.y).  8 waves = kColGroups column groups (64 targets = 4
+// column tiles of 16) x kKPar k-parities: wave (g, kp) multiplies the
+// k-steps kk = kp, kp + kKPar, ... of every key for its 64 targets and all
+// RTB row tiles, so each source multipole is fetched once per tile; two
+// parities are added through LDS at the end.  With one parity the tile is
+// 512 targets wide, so each staged operator serves twice the targets (the
+// operator staging is most of the GEMM's fetch).  Per key the block's operator rows are copied
 // global -> LDS (global_load_lds, no register round trip) into the second
 // of two buffers while the first is multiplied; one wait + barrier per key.
 // Each lane's source multipoles for its next k-step are loaded while the
 // current one is multiplied.  All keys of a tile accumulate into the same
 // MFMA accumulators, so each target's L is written once, without atomics.
-constexpr int kGemmThreads = 512;
 constexpr int kGemmMaxRT = 8;  // row tiles per wave (accumulator registers)
 
 template <int RH>
@@ -490,6 +502,7 @@ __global__ void k_combine_slices(int PP, int64_t ncells, int nslice, const int *
 // buffers are free by then); parity 0 adds them and writes the locals.
 template <int RH>
 __device__ inline void reduce_parities(v4f (&acc)[RH][4], int kp, int cg, int lane, void *lds) {
+  if (kKPar == 1) return;  // one wave per column group: nothing to add
   v4f *red = reinterpret_cast<v4f *>(lds);
   __syncthreads();
   if (kp == 1) {
@@ -536,7 +549,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
   // wave index read through a scalar register: the k-step and key loops
   // below then branch uniformly instead of under execution masks
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int cg = w & 3, kp = w >> 2;
+  const int cg = w % kColGroups, kp = w / kColGroups;
   const int tile = blockIdx.x;
   const int ncol = tcount[tile];
   // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
@@ -569,7 +582,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
         float *dst = As + size_t((r & 1) * kr + u - r * kr) * BUF;
         const int kc = kcols(u), ks = ksteps(kc), nr = rtiles(kc);
         const float *Ak = op32 + int64_t(tkeys[base + u]) * per;
-        for (int j = w; j < ks * nr; j += kGemmThreads / 64) {
+        for (int j = w; j < ks * nr; j += kGemmWaves) {
           const int kk = j / nr, rl = j - kk * nr, c = kk * nrt + rl;
           __builtin_amdgcn_global_load_lds(Ak + int64_t(op_run(c, nrt, RT, rt0)) * 64 + lane, dst + c * 64, 4, 0, 0);
         }
@@ -577,8 +590,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
     };
     auto stage_src = [&](int r) {
       const int nu = min(kr, U - r * kr);
-      for (int j = w; j < 4 * nu; j += kGemmThreads / 64) {
-        const int ui = j >> 2, q = j & 3;
+      for (int j = w; j < kFillParts * nu; j += kGemmWaves) {
+        const int ui = j / kFillParts, q = j % kFillParts;
         __builtin_amdgcn_global_load_lds(tmat + (int64_t(base) + r * kr + ui) * kTN + q * 64 + lane,
                                          Ts + ((r % 3) * kr + ui) * kTN + q * 64, 4, 0, 0);
       }
@@ -617,11 +630,11 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
         const int kc_next = u + 1 < U ? kcols(u + 1) : 0;
         const bool next = u + 1 < U && kp < ksteps(kc_next);
         if (u + 1 < U) col_src(u + 1, nsrc);
-        for (int kk = kp; kk < ks; kk += 2) {
+        for (int kk = kp; kk < ks; kk += kKPar) {
 #pragma unroll
           for (int ct = 0; ct < 4; ct++) bcur[ct] = bnext[ct];
-          if (kk + 2 < ks) {
-            load_b(src, kk + 2, kc, bnext);
+          if (kk + kKPar < ks) {
+            load_b(src, kk + kKPar, kc, bnext);
             b_after = true;
           } else if (next) {
             load_b(nsrc, kp, kc_next, bnext);
@@ -700,7 +713,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
   // wave index read through a scalar register: the k-step and key loops
   // below then branch uniformly instead of under execution masks
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int cg = w & 3, kp = w >> 2;
+  const int cg = w % kColGroups, kp = w / kColGroups;
   const int tile = bx;
   const int ncol = tcount[tile];
   // column subtiles of 16 that hold target cells (with SKIP, a partly
@@ -744,7 +757,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
         const int kc = kcols(u), ks = ksteps(kc), nr = rtiles(kc);
         const int64_t k = tkeys[slot(u)];
         const int n1 = ks * nr;
-        for (int j = w; j < 2 * n1; j += kGemmThreads / 64) {
+        for (int j = w; j < 2 * n1; j += kGemmWaves) {
           const int part = j >= n1, jj = part ? j - n1 : j;
           const int kk = jj / nr, rl = jj - kk * nr, cc = kk * nrt + rl;
           const v8bf *A = (part ? oplo : ophi) + k * per;
@@ -755,8 +768,8 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
     };
     auto stage_src = [&](int r) {
       const int nu = min(kr, U - r * kr);
-      for (int j = w; j < 4 * nu; j += kGemmThreads / 64) {
-        const int ui = j >> 2, q = j & 3;
+      for (int j = w; j < kFillParts * nu; j += kGemmWaves) {
+        const int ui = j / kFillParts, q = j % kFillParts;
         __builtin_amdgcn_global_load_lds(tmat + int64_t(slot(r * kr + ui)) * kTN + q * 64 + lane,
                                          Ts + ((r % 3) * kr + ui) * kTN + q * 64, 4, 0, 0);
       }
@@ -816,7 +829,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
         const int kc_next = u + 1 < U ? kcols(u + 1) : 0;
         const bool next = u + 1 < U && kp < ksteps(kc_next);
         if (u + 1 < U) col_src(u + 1, nsrc);  // in LDS: round r or r + 1
-        for (int kk = kp; kk < ks; kk += 2) {
+        for (int kk = kp; kk < ks; kk += kKPar) {
 #pragma unroll
           for (int ct = 0; ct < 4; ct++) {
             bh[ct] = nbh[ct];
@@ -824,8 +837,8 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
           }
           // only an order below the operator's has a bound inside the padded K range
           if (varK && kk * 32 + 32 > kc) mask_b(kk, kc, bh, bl);
-          if (kk + 2 < ks) {
-            load_b(src, kk + 2, kc, nbh, nbl);
+          if (kk + kKPar < ks) {
+            load_b(src, kk + kKPar, kc, nbh, nbl);
             b_after = true;
           } else if (next) {
             load_b(nsrc, kp, kc_next, nbh, nbl);
@@ -1315,7 +1328,7 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   XF_HIP(hipMemcpyAsync(d_sorted, v1, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
   const int nwords = (nkeys + 31) / 32;
   c.tile_bits.ensure(size_t(ntmax) * std::max(nwords, 1));
-  hipLaunchKernelGGL(k_tile_count, dim3(ntmax), dim3(4 * kTN), sizeof(unsigned) * nwords, s, d_first, d_count,
+  hipLaunchKernelGGL(k_tile_count, dim3(ntmax), dim3(kTileCountThreads), sizeof(unsigned) * nwords, s, d_first, d_count,
                      d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_plan, d_U,
                      c.tile_bits.p);
   XF_LAUNCH();
