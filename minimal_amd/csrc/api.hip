@@ -103,7 +103,7 @@ void bodies_to_host(Ctx &c) {
 }
 
 // event slots: 0/1 build, 2/3 upward, 11/12 traversal, 9/4 M2L (s0),
-// 5/6 P2P (s1), 10/8 downward
+// 5/6 P2P (s1), 15 end of s1's work, 10/8 downward
 void do_build(Ctx &c) {
   if (c.graph_locks) fail(EXAFMM_ERR_STATE, "the context is held by a graph (exafmm_graph_destroy it first)");
   rec(c, 0, c.s0);
@@ -143,6 +143,7 @@ void do_p2p(Ctx &tc, const Ctx &sc, bool after_lists = false) {
   rec(tc, 5, tc.s1);
   p2p_apply(tc, sc, tc.s1);
   rec(tc, 6, tc.s1);
+  rec(tc, 15, tc.s1);
 }
 
 // M2L (and the periodic far field) on the main stream; joins the P2P stream.
@@ -153,7 +154,7 @@ void do_far(Ctx &tc, const Ctx &sc) {
   m2l_apply(tc, sc);
   rec(tc, 4, tc.s0);
   if (tc.cfg.images > 0) periodic_far(tc);
-  XF_HIP(hipStreamWaitEvent(tc.s0, tc.ev[6], 0));
+  XF_HIP(hipStreamWaitEvent(tc.s0, tc.ev[15], 0));  // the end of the second stream's work
   tc.src = &sc;
   tc.have_L = true;
 }
@@ -182,13 +183,24 @@ void do_downward(Ctx &c) {
 // (Running the upward pass beside the traversal instead is slower: its
 // workgroups hold the CUs each short traversal launch waits for.)
 void do_passes(Ctx &c) {
-  do_build(c);
+  {
+    // the bodies' write-back in tree order runs on s1 after P2P, off the
+    // latency chain (nothing in the evaluation reads their X, q)
+    struct Defer {
+      Ctx &c;
+      ~Defer() { c.defer_writeback = false; }
+    } defer{c};
+    c.defer_writeback = true;
+    do_build(c);
+  }
   do_lists(c, c, [&c] {
     rec(c, 14, c.s0);
     XF_HIP(hipStreamWaitEvent(c.s1, c.ev[14], 0));
     rec(c, 5, c.s1);
     p2p_apply(c, c, c.s1, c.lists.tot.p + 3);
     rec(c, 6, c.s1);
+    if (c.writeback_pending) writeback_bodies(c, c.s1);
+    rec(c, 15, c.s1);
   });
   do_upward(c, c.s0);
   do_far(c, c);

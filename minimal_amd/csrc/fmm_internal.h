@@ -247,6 +247,9 @@ struct Ctx {
   exafmm_body *bodies = nullptr;   // caller's array (device pointer or our mirror)
   exafmm_body *host_bodies = nullptr;
   bool host_mode = false;
+  // tree_build leaves the write-back of X, q in tree order to the caller
+  // (writeback_bodies), who runs it off the latency chain
+  bool defer_writeback = false, writeback_pending = false;
   Buf<exafmm_body> mirror;         // device copy when the caller passed host memory
 
   Tree t;
@@ -280,6 +283,7 @@ struct Ctx {
   bool m2l_ready = false;
   int m2l_ntiles = 0;
   int m2l_tstride = 0;      // stride of the tile arrays in `tiles` (a bound on the tile count)
+  int m2l_tn = 512;         // targets per M2L tile (512 or 256)
   int m2l_ncell_tiled = 0;  // cells that receive M2L (the tiles' columns)
   Buf<int> m2l_misc, m2l_rs_hist, m2l_scan;
   Buf<float> m2l_part;      // FP32 partial locals of the key slices [slice][cell][P^2]
@@ -319,6 +323,8 @@ struct Ctx {
 
 // ------------------------------------------------------------------ stages
 void tree_build(Ctx &c);
+// X, q of the caller's bodies in tree order (a build with defer_writeback set)
+void writeback_bodies(Ctx &c, hipStream_t s);
 void tree_export_cells(Ctx &c, exafmm_cell *out);     // reference numbering
 void tree_ref_order(Ctx &c, std::vector<int64_t> &bfs2ref, std::vector<int64_t> &ref2bfs);
 // Dual-tree traversal into the interaction lists.  after_kernels (if set)
@@ -352,7 +358,7 @@ void p2p_grouped(const GroupTree *g, const int *start, int nprob, int nblocks, h
 // keyed M2L of every problem of one order (BF16x3, fixed order): cast +
 // GEMM, one launch each; the plan holds the device descriptor tables
 struct GemmGroupPlan {
-  int P = 0, rtb = 0, kr = 1, nprob = 0, nblocks = 0, cast_blocks = 0;
+  int P = 0, rtb = 0, kr = 1, nprob = 0, nblocks = 0, cast_blocks = 0, tn = 512;
   size_t lds = 0;
   Buf<char> desc, cast_desc;
   Buf<int> start, cast_start;

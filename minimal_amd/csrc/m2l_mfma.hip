@@ -31,17 +31,15 @@
 namespace xf {
 using namespace xfd;
 
-// GEMM workgroup: 8 waves = column groups of 64 targets x kKPar k-parities
-// (a wave multiplies the k-steps kk = kp, kp + kKPar, ... of its columns).
+// GEMM workgroup: 8 waves = column groups of 64 targets x KPAR k-parities
+// (a wave multiplies the k-steps kk = kp, kp + KPAR, ... of its columns):
+// tiles of 512 targets with one parity, 256 with two.
 constexpr int kGemmThreads = 512;
 constexpr int kGemmWaves = kGemmThreads / 64;
-#ifndef M2L_KPAR
-#define M2L_KPAR 1
-#endif
-constexpr int kKPar = M2L_KPAR;
-constexpr int kColGroups = kGemmWaves / kKPar;
-constexpr int kTN = 64 * kColGroups;  // targets per tile (512: 8 column groups x 4 column tiles of 16)
-static_assert(kKPar == 1 || kKPar == 2, "one or two k-parities");
+template <int KPAR>
+constexpr int tile_width() {
+  return 64 * (kGemmWaves / KPAR);
+}
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
@@ -157,14 +155,14 @@ __device__ inline void tile_bitmap(const int *cells, int ncol, const int *beg, c
 
 // Tiles from the group populations gc[0..254] (group 255 = cells without
 // M2L, not tiled): each group's cells, in sorted order, are cut into tiles
-// of at most kTN.  Writes each tile's first cell and count, and plan[0] =
+// of at most tn.  Writes each tile's first cell and count, and plan[0] =
 // tiles, plan[1] = cells in tiles.
-__global__ void __launch_bounds__(256) k_tile_plan(const int *__restrict__ gc, int *__restrict__ tfirst,
+__global__ void __launch_bounds__(256) k_tile_plan(const int *__restrict__ gc, int tn, int *__restrict__ tfirst,
                                                    int *__restrict__ tcount, int *__restrict__ plan) {
   __shared__ int tb[256], cb[256];
   const int gi = threadIdx.x;
   const int n = gi < 255 ? gc[gi] : 0;
-  tb[gi] = (n + kTN - 1) / kTN;
+  tb[gi] = (n + tn - 1) / tn;
   cb[gi] = n;
   __syncthreads();
   if (gi == 0) {  // exclusive prefixes of tiles and cells over the groups
@@ -180,13 +178,13 @@ __global__ void __launch_bounds__(256) k_tile_plan(const int *__restrict__ gc, i
     plan[1] = cc;
   }
   __syncthreads();
-  for (int j = 0; j * kTN < n; j++) {
-    tfirst[tb[gi] + j] = cb[gi] + j * kTN;
-    tcount[tb[gi] + j] = min(kTN, n - j * kTN);
+  for (int j = 0; j * tn < n; j++) {
+    tfirst[tb[gi] + j] = cb[gi] + j * tn;
+    tcount[tb[gi] + j] = min(tn, n - j * tn);
   }
 }
 
-constexpr int kTileCountThreads = 4 * kTN < 1024 ? 4 * kTN : 1024;
+constexpr int kTileCountThreads = 1024;
 __global__ void __launch_bounds__(kTileCountThreads) k_tile_count(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                     const int *__restrict__ sorted, const int *__restrict__ beg,
                                                     const int *__restrict__ cnt, const int2 *__restrict__ list,
@@ -211,18 +209,17 @@ __global__ void __launch_bounds__(kTileCountThreads) k_tile_count(const int *__r
   if (threadIdx.x == 0) tU[blockIdx.x] = tot;
 }
 
-// Key list (ascending) and source matrix [U][kTN] (-1 = no pair) of a tile.
+// Key list (ascending) and source matrix [U][tn] (-1 = no pair) of a tile.
 // Workgroup (tile, q) owns columns 64q .. 64q + 63: it marks them empty in
 // every key row, then four threads per column walk that column's M2L list
 // (entries i = sub mod 4) and write the source cell into the key's row.
 // The key set comes from k_tile_count's bitmap.
-constexpr int kFillParts = kTN / 64;
 __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirst, const int *__restrict__ tcount,
                                                    const int *__restrict__ sorted, const int *__restrict__ beg,
                                                    const int *__restrict__ cnt, const int2 *__restrict__ list,
                                                    int nkeys, const int *__restrict__ tbase,
                                                    const unsigned *__restrict__ tbits, int *__restrict__ tkeys,
-                                                   int *__restrict__ tmat) {
+                                                   int tn, int *__restrict__ tmat) {
   extern __shared__ unsigned bm[];
   const int nwords = (nkeys + 31) / 32;
   unsigned *rank = bm + nwords;  // exclusive popcount prefix per word
@@ -268,7 +265,7 @@ __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirs
     }
   }
   for (int i = threadIdx.x; i < U * 64; i += blockDim.x)
-    tmat[(int64_t(base) + i / 64) * kTN + q * 64 + (i & 63)] = -1;
+    tmat[(int64_t(base) + i / 64) * tn + q * 64 + (i & 63)] = -1;
   __syncthreads();
   const int col = q * 64 + (threadIdx.x >> 2), sub = threadIdx.x & 3;
   if (col < ncol) {
@@ -287,7 +284,7 @@ __global__ void __launch_bounds__(256) k_tile_fill(const int *__restrict__ tfirs
         if (id < 0) continue;
         const unsigned w = bm[id >> 5];
         const int r = rank[id >> 5] + __popc(w & ((1u << (id & 31)) - 1u));
-        tmat[(int64_t(base) + r) * kTN + col] = en[k].x >> 5;  // source cell (the image is part of the key)
+        tmat[(int64_t(base) + r) * tn + col] = en[k].x >> 5;  // source cell (the image is part of the key)
       }
     }
   }
@@ -430,10 +427,10 @@ __global__ void __launch_bounds__(256) k_ops_gen(int P, int fillP, int R, int K4
 }
 
 // ---------------------------------------------------------------- GEMM
-// One workgroup = one tile of up to kTN targets and a block of RTB row
-// tiles (blockIdx.y).  8 waves = kColGroups column groups (64 targets = 4
-// column tiles of 16) x kKPar k-parities: wave (g, kp) multiplies the
-// k-steps kk = kp, kp + kKPar, ... of every key for its 64 targets and all
+// One workgroup = one tile of up to tile_width<KPAR>() targets and a block
+// of RTB row tiles (blockIdx.y).  8 waves = 8 / KPAR column groups (64
+// targets = 4 column tiles of 16) x KPAR k-parities: wave (g, kp) multiplies
+// the k-steps kk = kp, kp + KPAR, ... of every key for its 64 targets and all
 // RTB row tiles, so each source multipole is fetched once per tile; two
 // parities are added through LDS at the end.  With one parity the tile is
 // 512 targets wide, so each staged operator serves twice the targets (the
@@ -500,9 +497,9 @@ __global__ void k_combine_slices(int PP, int64_t ncells, int nslice, const int *
 
 // Parity 1 hands its accumulators to parity 0 through LDS (the operator
 // buffers are free by then); parity 0 adds them and writes the locals.
-template <int RH>
+template <int RH, int KPAR>
 __device__ inline void reduce_parities(v4f (&acc)[RH][4], int kp, int cg, int lane, void *lds) {
-  if (kKPar == 1) return;  // one wave per column group: nothing to add
+  if (KPAR == 1) return;  // one wave per column group: nothing to add
   v4f *red = reinterpret_cast<v4f *>(lds);
   __syncthreads();
   if (kp == 1) {
@@ -532,7 +529,7 @@ __device__ inline int op_run(int c, int nrt, int RT, int rt0) {
   return kk * RT + rt0 + rl;
 }
 
-template <int RH>
+template <int RH, int KPAR>
 __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RTB, int K4, int KP,
                                                           const int *__restrict__ tfirst,
                                                           const int *__restrict__ tcount,
@@ -546,10 +543,11 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
                                                           int64_t ncells, float *__restrict__ Lpart,
                                                           double *__restrict__ L) {
   extern __shared__ float As[];
+  constexpr int TN = tile_width<KPAR>(), NG = kGemmWaves / KPAR, PARTS = TN / 64;
   // wave index read through a scalar register: the k-step and key loops
   // below then branch uniformly instead of under execution masks
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int cg = w % kColGroups, kp = w / kColGroups;
+  const int cg = w % NG, kp = w / NG;
   const int tile = blockIdx.x;
   const int ncol = tcount[tile];
   // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
@@ -574,7 +572,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
   auto rtiles = [&](int kc) { return max(0, min(nrt, (kc + 15) / 16 - rt0)); };
   if (U > 0) {
     // LDS: operator slots [2 buffers][kr keys][BUF], then the keys' column
-    // sources [3 buffers][kr keys][kTN]; staging as in the BF16 kernel below
+    // sources [3 buffers][kr keys][TN]; staging as in the BF16 kernel below
     int *Ts = reinterpret_cast<int *>(As + size_t(2) * kr * BUF);
     const int R = (U + kr - 1) / kr;
     auto stage_ops = [&](int r) {
@@ -590,14 +588,14 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
     };
     auto stage_src = [&](int r) {
       const int nu = min(kr, U - r * kr);
-      for (int j = w; j < kFillParts * nu; j += kGemmWaves) {
-        const int ui = j / kFillParts, q = j % kFillParts;
-        __builtin_amdgcn_global_load_lds(tmat + (int64_t(base) + r * kr + ui) * kTN + q * 64 + lane,
-                                         Ts + ((r % 3) * kr + ui) * kTN + q * 64, 4, 0, 0);
+      for (int j = w; j < PARTS * nu; j += kGemmWaves) {
+        const int ui = j / PARTS, q = j % PARTS;
+        __builtin_amdgcn_global_load_lds(tmat + (int64_t(base) + r * kr + ui) * TN + q * 64 + lane,
+                                         Ts + ((r % 3) * kr + ui) * TN + q * 64, 4, 0, 0);
       }
     };
     auto col_src = [&](int u, int (&sc)[4]) {
-      const int *T = Ts + (((u / kr) % 3) * kr + u % kr) * kTN + cg * 64 + (lane & 15);
+      const int *T = Ts + (((u / kr) % 3) * kr + u % kr) * TN + cg * 64 + (lane & 15);
 #pragma unroll
       for (int ct = 0; ct < 4; ct++) sc[ct] = T[ct * 16];
     };
@@ -630,11 +628,11 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
         const int kc_next = u + 1 < U ? kcols(u + 1) : 0;
         const bool next = u + 1 < U && kp < ksteps(kc_next);
         if (u + 1 < U) col_src(u + 1, nsrc);
-        for (int kk = kp; kk < ks; kk += kKPar) {
+        for (int kk = kp; kk < ks; kk += KPAR) {
 #pragma unroll
           for (int ct = 0; ct < 4; ct++) bcur[ct] = bnext[ct];
-          if (kk + kKPar < ks) {
-            load_b(src, kk + kKPar, kc, bnext);
+          if (kk + KPAR < ks) {
+            load_b(src, kk + KPAR, kc, bnext);
             b_after = true;
           } else if (next) {
             load_b(nsrc, kp, kc_next, bnext);
@@ -669,7 +667,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_f32(int PP, int RT, int RT
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  reduce_parities<RH>(acc, kp, cg, lane, As);
+  reduce_parities<RH, KPAR>(acc, kp, cg, lane, As);
   if (kp == 0) {
     if (Lpart)
       write_partial<RH>(PP, rt0, nrt, sorted + tfirst[tile], ncol, cg, acc, lane,
@@ -692,7 +690,7 @@ struct OrderPass {
   const int *perm;
 };
 
-template <int RH, bool SKIP>
+template <int RH, bool SKIP, int KPAR>
 __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, int by, int ub, int ue, bool part_local,
                                        int PP, int RT, int RTB, int K32, int KP, OrderPass op,
                                        const int *__restrict__ tfirst,
@@ -710,10 +708,11 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
                                        int64_t ncells, float *__restrict__ Lpart,
                                        double *__restrict__ L) {
   extern __shared__ v8bf Ab[];
+  constexpr int TN = tile_width<KPAR>(), NG = kGemmWaves / KPAR, PARTS = TN / 64;
   // wave index read through a scalar register: the k-step and key loops
   // below then branch uniformly instead of under execution masks
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int cg = w % kColGroups, kp = w / kColGroups;
+  const int cg = w % NG, kp = w / NG;
   const int tile = bx;
   const int ncol = tcount[tile];
   // column subtiles of 16 that hold target cells (with SKIP, a partly
@@ -744,7 +743,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
   auto rtiles = [&](int kc) { return max(0, min(nrt, (kc + 15) / 16 - rt0)); };
   if (U > 0) {
     // LDS: operator slots [2 buffers][kr keys][hi, lo][E], then the keys'
-    // column sources (tile matrix rows) [3 buffers][kr keys][kTN].  Round r
+    // column sources (tile matrix rows) [3 buffers][kr keys][TN].  Round r
     // (keys r kr .. r kr + kr - 1) reads operator buffer r & 1 and source
     // buffer r % 3; at its start it stages round r + 1's operators and round
     // r + 2's sources, so the sources of the next round's first key are
@@ -768,14 +767,14 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
     };
     auto stage_src = [&](int r) {
       const int nu = min(kr, U - r * kr);
-      for (int j = w; j < kFillParts * nu; j += kGemmWaves) {
-        const int ui = j / kFillParts, q = j % kFillParts;
-        __builtin_amdgcn_global_load_lds(tmat + int64_t(slot(r * kr + ui)) * kTN + q * 64 + lane,
-                                         Ts + ((r % 3) * kr + ui) * kTN + q * 64, 4, 0, 0);
+      for (int j = w; j < PARTS * nu; j += kGemmWaves) {
+        const int ui = j / PARTS, q = j % PARTS;
+        __builtin_amdgcn_global_load_lds(tmat + int64_t(slot(r * kr + ui)) * TN + q * 64 + lane,
+                                         Ts + ((r % 3) * kr + ui) * TN + q * 64, 4, 0, 0);
       }
     };
     auto col_src = [&](int u, int (&sc)[4]) {
-      const int *T = Ts + (((u / kr) % 3) * kr + u % kr) * kTN + cg * 64 + (lane & 15);
+      const int *T = Ts + (((u / kr) % 3) * kr + u % kr) * TN + cg * 64 + (lane & 15);
 #pragma unroll
       for (int ct = 0; ct < 4; ct++) sc[ct] = T[ct * 16];
     };
@@ -829,7 +828,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
         const int kc_next = u + 1 < U ? kcols(u + 1) : 0;
         const bool next = u + 1 < U && kp < ksteps(kc_next);
         if (u + 1 < U) col_src(u + 1, nsrc);  // in LDS: round r or r + 1
-        for (int kk = kp; kk < ks; kk += kKPar) {
+        for (int kk = kp; kk < ks; kk += KPAR) {
 #pragma unroll
           for (int ct = 0; ct < 4; ct++) {
             bh[ct] = nbh[ct];
@@ -837,8 +836,8 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
           }
           // only an order below the operator's has a bound inside the padded K range
           if (varK && kk * 32 + 32 > kc) mask_b(kk, kc, bh, bl);
-          if (kk + kKPar < ks) {
-            load_b(src, kk + kKPar, kc, nbh, nbl);
+          if (kk + KPAR < ks) {
+            load_b(src, kk + KPAR, kc, nbh, nbl);
             b_after = true;
           } else if (next) {
             load_b(nsrc, kp, kc_next, nbh, nbl);
@@ -879,7 +878,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
       __builtin_amdgcn_sched_barrier(0);
     }
   }
-  reduce_parities<RH>(acc, kp, cg, lane, Ab);
+  reduce_parities<RH, KPAR>(acc, kp, cg, lane, Ab);
   if (kp == 0) {
     if (Lpart)
       write_partial<RH>(PP, rt0, nrt, part_local ? nullptr : sorted + tfirst[tile], ncol, cg, acc, lane, Lpart);
@@ -888,7 +887,7 @@ __device__ __attribute__((always_inline)) inline void m2l_bf16x3_body(int bx, in
   }
 }
 
-template <int RH>
+template <int RH, int KPAR>
 __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int RTB, int K32, int KP, OrderPass op,
                                                              const int *__restrict__ tfirst,
                                                              const int *__restrict__ tcount,
@@ -907,7 +906,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
   // keys of this workgroup: slice blockIdx.z of gridDim.z of the tile's keys
   const int Ut = tU[blockIdx.x];
   const int ub = int(int64_t(Ut) * blockIdx.z / gridDim.z), ue = int(int64_t(Ut) * (blockIdx.z + 1) / gridDim.z);
-  m2l_bf16x3_body<RH, false>(blockIdx.x, blockIdx.y, ub, ue, false, PP, RT, RTB, K32, KP, op, tfirst, tcount, tbase,
+  m2l_bf16x3_body<RH, false, KPAR>(blockIdx.x, blockIdx.y, ub, ue, false, PP, RT, RTB, K32, KP, op, tfirst, tcount, tbase,
                              tU, sorted, tkeys, tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
                              Lpart ? Lpart + int64_t(blockIdx.z) * ncells * PP : nullptr, L);
 }
@@ -923,7 +922,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3(int PP, int RT, int
 // tbase is then the exclusive prefix of their per-tile counts tU, tstart[t]
 // where tile t's run of them starts in the order-grouped slots, and Sdev
 // their total.
-template <int RH>
+template <int RH, int KPAR>
 __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, int K32, int KP, OrderPass op,
                                                                 int ntiles, int64_t S, const int *__restrict__ Sdev,
                                                                 const int *__restrict__ tfirst,
@@ -957,9 +956,9 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, 
     const int64_t b = tbase[t];
     const int ub = int(max(s0, b) - b), ue = int(min(s1, b + tU[t]) - b);
     if (ub >= ue) continue;
-    m2l_bf16x3_body<RH, false>(t, 0, ub, ue, true, PP, RT, op.RTo, K32, KP, op, tfirst, tcount, tstart, tU, sorted,
+    m2l_bf16x3_body<RH, false, KPAR>(t, 0, ub, ue, true, PP, RT, op.RTo, K32, KP, op, tfirst, tcount, tstart, tU, sorted,
                                tkeys, tmat, ophi, oplo, mhi, mlo, tkp, kr, ncells,
-                               Lseg + (w + t) * int64_t(kTN) * PP, nullptr);
+                               Lseg + (w + t) * int64_t(tile_width<KPAR>()) * PP, nullptr);
     __syncthreads();  // the LDS of this segment is free before the next one stages
   }
 }
@@ -968,7 +967,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_sk(int PP, int RT, 
 // FP64 locals, rows 0 .. rows - 1 (those the launch computed).  One thread
 // per (tile column, row).
 __global__ void __launch_bounds__(256) k_sk_combine(int PP, int rows, int ntiles, int64_t S, const int *__restrict__ Sdev,
-                                                    int G, const int *__restrict__ tfirst,
+                                                    int G, int tn, const int *__restrict__ tfirst,
                                                     const int *__restrict__ tcount, const int *__restrict__ tbase,
                                                     const int *__restrict__ tU, const int *__restrict__ sorted,
                                                     const float *__restrict__ Lseg, double *__restrict__ L) {
@@ -982,7 +981,7 @@ __global__ void __launch_bounds__(256) k_sk_combine(int PP, int rows, int ntiles
   const int64_t b = tbase[t];
   const int wf = wg_of(b), wl = wg_of(b + tU[t] - 1);
   double acc = 0.0;
-  for (int w = wf; w <= wl; w++) acc += double(Lseg[(int64_t(w + t) * kTN + col) * PP + row]);
+  for (int w = wf; w <= wl; w++) acc += double(Lseg[(int64_t(w + t) * tn + col) * PP + row]);
   L[int64_t(sorted[tfirst[t] + col]) * PP + row] += acc;
 }
 
@@ -999,14 +998,14 @@ struct GroupGemm {
 
 // All problems of one order in one launch: each problem has ntiles x
 // (row blocks) workgroups, one key slice, kr = 1.
-template <int RH>
+template <int RH, int KPAR>
 __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_grouped(const GroupGemm *__restrict__ g,
                                                                      const int *__restrict__ start, int nprob,
                                                                      int PP, int RT, int RTB, int K32, int KP,
                                                                      int pl, int kr) {
   const int b = blockIdx.x, i = group_of(start, nprob, b), lb = b - start[i];
   const GroupGemm &q = g[i];
-  m2l_bf16x3_body<RH, true>(lb % q.ntiles, lb / q.ntiles, 0, q.tU[lb % q.ntiles], false, PP, RT, RTB, K32, KP,
+  m2l_bf16x3_body<RH, true, KPAR>(lb % q.ntiles, lb / q.ntiles, 0, q.tU[lb % q.ntiles], false, PP, RT, RTB, K32, KP,
                             OrderPass{pl, RT, K32, nullptr},
                             q.tfirst, q.tcount, q.tbase,
                             q.tU, q.sorted, q.tkeys, q.tmat, q.ophi, q.oplo, q.mhi, q.mlo, nullptr, kr, q.ncells,
@@ -1017,12 +1016,12 @@ __global__ void __launch_bounds__(kGemmThreads) k_m2l_bf16x3_grouped(const Group
 // Row tiles per workgroup: two operator buffers within the LDS budget, at
 // most kGemmMaxRT accumulator row tiles per wave, and room for the parity
 // reduction (4 x RTB x 4 KB).
-static int rows_per_block(const Geom &g, bool bf16, int device, int kr) {
+static int rows_per_block(const Geom &g, bool bf16, int device, int kr, int tn) {
   int max_lds = 0;
   XF_HIP(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device));
   const int per_rt = 2 * kr * (bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4);  // both buffers of kr keys
   const int need = std::max(per_rt, 4 * 4 * 64 * 16);                   // or the reduction
-  const int avail = max_lds - 3 * kr * kTN * 4;                          // less the column sources
+  const int avail = max_lds - 3 * kr * tn * 4;                           // less the column sources
   if (need > avail) {
     if (kr == 1) fail(EXAFMM_ERR_UNSUPPORTED, "expansion order too large for the LDS operator stage");
     return 0;
@@ -1033,9 +1032,9 @@ static int rows_per_block(const Geom &g, bool bf16, int device, int kr) {
   return ceil_div(g.R, nblk);
 }
 
-static size_t gemm_lds(const Geom &g, bool bf16, int RTB, int kr) {
+static size_t gemm_lds(const Geom &g, bool bf16, int RTB, int kr, int tn) {
   const size_t ops = size_t(2) * kr * RTB * (bf16 ? g.K32 * 64 * 16 * 2 : g.K4 * 64 * 4) +
-                     size_t(3) * kr * kTN * sizeof(int);  // operator slots + column sources
+                     size_t(3) * kr * tn * sizeof(int);  // operator slots + column sources
   const size_t red = size_t(4) * RTB * 4 * 64 * 16;
   return std::max(ops, red);
 }
@@ -1051,9 +1050,10 @@ struct ClassPass {
   const int *pre, *tot;    // per tile: exclusive prefix of the counts; their total (stream-K)
 };
 
-template <int RH>
+template <int RH, int KPAR>
 static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool bf16, int KP, int KPb, int RTB,
                         int kr, int ksplit, float *Lpart, double *Lout, const ClassPass *cp) {
+  constexpr int TN = tile_width<KPAR>();
   Ctx &c = tc;
   const int ts = c.m2l_tstride;
   const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ts, *tbase = c.tiles.p + 2 * ts, *tU = c.tiles.p + 3 * ts;
@@ -1067,19 +1067,19 @@ static void launch_gemm(Ctx &tc, const Ctx &sc, const Geom &g, int ntiles, bool 
                                                       per * c.lists.nkeys);
     const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
     const __bf16 *mlo = mhi + bf16_cast_elems(g, sc.t.ncells);
-    const size_t lds = gemm_lds(go, true, RTB, kr);
+    const size_t lds = gemm_lds(go, true, RTB, kr, TN);
     const OrderPass op{go.P, go.R, go.K32, cp ? cp->perm : nullptr};
-    XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3<RH>),
+    XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3<RH, KPAR>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL((k_m2l_bf16x3<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, op,
+    hipLaunchKernelGGL((k_m2l_bf16x3<RH, KPAR>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K32, KPb, op,
                        tfirst, tcount, cp ? cp->start : tbase, cp ? cp->cnt : tU, sorted, c.tile_keys.p,
                        c.tile_mat.p, ophi, oplo, mhi, mlo, (c.learned && (!cp || cp->variable)) ? c.tile_p.p : nullptr, kr,
                        int64_t(c.t.ncells), ksplit > 1 ? Lpart : nullptr, Lout);
   } else {
-    const size_t lds = gemm_lds(g, false, RTB, kr);
-    XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_f32<RH>),
+    const size_t lds = gemm_lds(g, false, RTB, kr, TN);
+    XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_f32<RH, KPAR>),
                                hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-    hipLaunchKernelGGL((k_m2l_f32<RH>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K4, KP, tfirst,
+    hipLaunchKernelGGL((k_m2l_f32<RH, KPAR>), grid, dim3(kGemmThreads), lds, c.s0, g.PP, g.R, RTB, g.K4, KP, tfirst,
                        tcount, tbase, tU, sorted, c.tile_keys.p, c.tile_mat.p, c.ops32.p, sc.mt32.p,
                        c.learned ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), ksplit > 1 ? Lpart : nullptr,
                        Lout);
@@ -1100,9 +1100,10 @@ static int cu_count(int device) {
 
 // Stream-K launch: all rows per workgroup (R <= kGemmMaxRT), then the
 // fixed-order combine of each tile's segments.
-template <int RH>
+template <int RH, int KPAR>
 static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int KPb, int kr, double *Lout,
                            const ClassPass *cp) {
+  constexpr int TN = tile_width<KPAR>();
   const int ts = c.m2l_tstride;
   const int *tfirst = c.tiles.p, *tcount = c.tiles.p + ts, *tbase = c.tiles.p + 2 * ts, *tU = c.tiles.p + 3 * ts;
   const int *sorted = c.tiles.p + 4 * ts;
@@ -1110,25 +1111,25 @@ static void launch_gemm_sk(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, int
   const int64_t S = c.m2l_totalU;
   const Geom go = cp ? geom(cp->p) : g;
   const OrderPass op{go.P, go.R, go.K32, cp ? cp->perm : nullptr};
-  c.m2l_part.ensure(size_t(G + ntiles) * kTN * g.PP);
+  c.m2l_part.ensure(size_t(G + ntiles) * TN * g.PP);
   const v8bf *ophi = reinterpret_cast<const v8bf *>(c.ops32.p);
   const size_t per = size_t(g.K32) * g.R * 64 * 8;
   const v8bf *oplo =
       reinterpret_cast<const v8bf *>(reinterpret_cast<const __bf16 *>(c.ops32.p) + per * c.lists.nkeys);
   const __bf16 *mhi = reinterpret_cast<const __bf16 *>(sc.mt32.p);
   const __bf16 *mlo = mhi + bf16_cast_elems(g, sc.t.ncells);
-  const size_t lds = gemm_lds(go, true, go.R, kr);
+  const size_t lds = gemm_lds(go, true, go.R, kr, TN);
   const int *pre = cp ? cp->pre : tbase, *start = cp ? cp->start : tbase, *cnt = cp ? cp->cnt : tU;
   const int *Sdev = cp ? cp->tot : nullptr;
-  XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3_sk<RH>),
+  XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3_sk<RH, KPAR>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-  hipLaunchKernelGGL((k_m2l_bf16x3_sk<RH>), dim3(G), dim3(kGemmThreads), lds, c.s0, g.PP, g.R, g.K32, KPb, op, ntiles,
+  hipLaunchKernelGGL((k_m2l_bf16x3_sk<RH, KPAR>), dim3(G), dim3(kGemmThreads), lds, c.s0, g.PP, g.R, g.K32, KPb, op, ntiles,
                      S, Sdev, tfirst, tcount, pre, start, cnt, sorted, c.tile_keys.p, c.tile_mat.p, ophi, oplo, mhi,
                      mlo, (c.learned && (!cp || cp->variable)) ? c.tile_p.p : nullptr, kr, int64_t(c.t.ncells), c.m2l_part.p);
   XF_LAUNCH();
   const int rows = std::min(g.PP, 16 * go.R);
-  hipLaunchKernelGGL(k_sk_combine, dim3(ceil_div(int64_t(kTN) * rows, 256), ntiles), dim3(256), 0, c.s0, g.PP, rows,
-                     ntiles, S, Sdev, G, tfirst, tcount, pre, cnt, sorted, c.m2l_part.p, Lout);
+  hipLaunchKernelGGL(k_sk_combine, dim3(ceil_div(int64_t(TN) * rows, 256), ntiles), dim3(256), 0, c.s0, g.PP, rows,
+                     ntiles, S, Sdev, G, TN, tfirst, tcount, pre, cnt, sorted, c.m2l_part.p, Lout);
   XF_LAUNCH();
 }
 
@@ -1136,8 +1137,13 @@ static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16
                      double *Lout, const ClassPass *cp = nullptr) {
   if (sh.sk) {
     switch (cp ? geom(cp->p).R : g.R) {
-#define XF_GEMM_SK(n) \
-  case n: launch_gemm_sk<n>(c, sc, g, ntiles, KPb, sh.kr, Lout, cp); return;
+#define XF_GEMM_SK(n)                                                  \
+  case n:                                                              \
+    if (c.m2l_tn == tile_width<1>())                                   \
+      launch_gemm_sk<n, 1>(c, sc, g, ntiles, KPb, sh.kr, Lout, cp);   \
+    else                                                               \
+      launch_gemm_sk<n, 2>(c, sc, g, ntiles, KPb, sh.kr, Lout, cp);   \
+    return;
       XF_GEMM_SK(1)
       XF_GEMM_SK(2)
       XF_GEMM_SK(3)
@@ -1156,8 +1162,13 @@ static void run_gemm(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bool bf16
     part = c.m2l_part.p;
   }
   switch (sh.rtb) {
-#define XF_GEMM_CASE(n) \
-  case n: launch_gemm<n>(c, sc, g, ntiles, bf16, KP, KPb, sh.rtb, sh.kr, sh.ksplit, part, Lout, cp); break;
+#define XF_GEMM_CASE(n)                                                                          \
+  case n:                                                                                        \
+    if (c.m2l_tn == tile_width<1>())                                                             \
+      launch_gemm<n, 1>(c, sc, g, ntiles, bf16, KP, KPb, sh.rtb, sh.kr, sh.ksplit, part, Lout, cp); \
+    else                                                                                         \
+      launch_gemm<n, 2>(c, sc, g, ntiles, bf16, KP, KPb, sh.rtb, sh.kr, sh.ksplit, part, Lout, cp); \
+    break;
     XF_GEMM_CASE(1)
     XF_GEMM_CASE(2)
     XF_GEMM_CASE(3)
@@ -1195,7 +1206,8 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
   // a learned-order launch of one order is tuned per order (its rows, k-steps and LDS)
   const Geom go = cp ? geom(cp->p) : g;
   const long long key = ((long long)g.P << 24) | ((long long)bf16 << 20) | ((long long)c.learned << 16) |
-                        ((long long)(cp ? cp->p : 0) << 32) | tclass;
+                        ((long long)(cp ? cp->p : 0) << 32) | ((long long)(c.m2l_tn == tile_width<1>()) << 40) |
+                        tclass;
   auto it = c.gemm_tuned.find(key);
   if (it != c.gemm_tuned.end())
     return GemmShape{int(it->second & 255), int((it->second >> 8) & 255), int((it->second >> 16) & 255),
@@ -1204,7 +1216,7 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
   // low orders: short keys, so more of them per LDS round
   const int krmax = go.P <= 10 ? 5 : 3;
   for (int kr = 1; kr <= krmax; kr++) {
-    const int rmax = rows_per_block(go, bf16, c.cfg.device, kr);
+    const int rmax = rows_per_block(go, bf16, c.cfg.device, kr, c.m2l_tn);
     if (rmax < 1) continue;
     int last = -1;
     for (int nb = ceil_div(go.R, rmax); nb <= ceil_div(go.R, rmax) + 2; nb++) {
@@ -1222,7 +1234,7 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
   // stream-K (BF16x3, all rows per workgroup)
   if (bf16 && go.R <= kGemmMaxRT && std::getenv("EXAFMM_GEMM_NO_SK") == nullptr)
     for (int kr = 1; kr <= 3; kr++)
-      if (rows_per_block(go, bf16, c.cfg.device, kr) >= go.R) cand.push_back(GemmShape{go.R, kr, 1, 1});
+      if (rows_per_block(go, bf16, c.cfg.device, kr, c.m2l_tn) >= go.R) cand.push_back(GemmShape{go.R, kr, 1, 1});
   GemmShape best = cand[0];
   if (std::getenv("EXAFMM_GEMM_SK_ONLY"))  // the stream-K candidates alone (tests, A/B timing)
     for (const GemmShape &sh : cand)
@@ -1239,7 +1251,7 @@ static GemmShape gemm_shape(Ctx &c, const Ctx &sc, const Geom &g, int ntiles, bo
     c.L_tune.ensure(size_t(c.t.ncells) * c.P * c.P);
     size_t need = 1;
     for (const GemmShape &sh : cand)
-      need = std::max(need, sh.sk ? size_t(cu_count(c.cfg.device) + ntiles) * kTN * g.PP
+      need = std::max(need, sh.sk ? size_t(cu_count(c.cfg.device) + ntiles) * c.m2l_tn * g.PP
                                   : size_t(sh.ksplit > 1 ? sh.ksplit : 0) * c.t.ncells * g.PP);
     c.m2l_part.ensure(need);  // no allocation inside the timings
     hipEvent_t e0, e1;
@@ -1315,15 +1327,21 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
   XF_HIP(hipMemsetAsync(c.m2l_misc.p, 0, 256 * sizeof(int), s));
   hipLaunchKernelGGL(k_group_hist, dim3(std::min(ceil_div(nc, 256), 64)), dim3(256), 0, s, nc, k1, c.m2l_misc.p);
   XF_LAUNCH();
-  // tiles of at most kTN cells per group, laid out on the device (no host
+  // tiles of at most tn cells per group, laid out on the device (no host
   // round trip before the per-tile counts); arrays strided by the bound
   // ntmax: [first | count | base | U] x ntmax, then the sorted cells
-  const int ntmax = ceil_div(nc, kTN) + 255;
+  // Tile width: 512 targets halves the operator staging per target where
+  // the tiles fill (a free-space tree's octant groups); periodic trees (image
+  // keys) and forests (many small problems) have more varied key sets, whose
+  // larger per-tile key unions favour 256.
+  c.m2l_tn = (c.cfg.images > 0 || c.forest_n > 0) ? tile_width<2>() : tile_width<1>();
+  const int tn = c.m2l_tn;
+  const int ntmax = ceil_div(nc, tn) + 255;
   c.tiles.ensure(size_t(4) * ntmax + nc);
   int *d_first = c.tiles.p, *d_count = c.tiles.p + ntmax, *d_base = c.tiles.p + 2 * ntmax,
       *d_U = c.tiles.p + 3 * ntmax, *d_sorted = c.tiles.p + 4 * ntmax;
   int *d_plan = c.m2l_misc.p + 257;  // [ntiles, cells in tiles]
-  hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(256), 0, s, c.m2l_misc.p, d_first, d_count, d_plan);
+  hipLaunchKernelGGL(k_tile_plan, dim3(1), dim3(256), 0, s, c.m2l_misc.p, tn, d_first, d_count, d_plan);
   XF_LAUNCH();
   XF_HIP(hipMemcpyAsync(d_sorted, v1, sizeof(int) * nc, hipMemcpyDeviceToDevice, s));
   const int nwords = (nkeys + 31) / 32;
@@ -1350,10 +1368,10 @@ void m2l_prepare(Ctx &c, hipStream_t s) {
     return;
   }
   c.tile_keys.ensure(std::max(totalU, 1));
-  c.tile_mat.ensure(size_t(std::max(totalU, 1)) * kTN);
-  hipLaunchKernelGGL(k_tile_fill, dim3(ntiles, kFillParts), dim3(256), sizeof(unsigned) * 2 * nwords, s, d_first,
+  c.tile_mat.ensure(size_t(std::max(totalU, 1)) * tn);
+  hipLaunchKernelGGL(k_tile_fill, dim3(ntiles, tn / 64), dim3(256), sizeof(unsigned) * 2 * nwords, s, d_first,
                      d_count, d_sorted, c.lists.m2l_beg.p, c.lists.m2l_cnt.p, c.lists.m2l.p, nkeys, d_base,
-                     c.tile_bits.p, c.tile_keys.p, c.tile_mat.p);
+                     c.tile_bits.p, c.tile_keys.p, tn, c.tile_mat.p);
   XF_LAUNCH();
 
   // 2. operators (cached by key set)
@@ -1424,7 +1442,7 @@ void m2l_fast(Ctx &tc, const Ctx &sc) {
   c.learned = c.cfg.order_policy == EXAFMM_ORDER_LEARNED && c.have_model && c.forest_n == 0;
   if (c.learned)
     order_select(c, c.tiles.p, c.tiles.p + c.m2l_tstride, c.tiles.p + 2 * c.m2l_tstride,
-                 c.tiles.p + 3 * c.m2l_tstride, c.tiles.p + 4 * c.m2l_tstride, ntiles, c.m2l_totalU, kTN, s);
+                 c.tiles.p + 3 * c.m2l_tstride, c.tiles.p + 4 * c.m2l_tstride, ntiles, c.m2l_totalU, c.m2l_tn, s);
 
   // 3. GEMM over tiles, shape from the autotuner
   if (c.learned && bf16) {
@@ -1496,9 +1514,9 @@ __global__ void __launch_bounds__(256) k_mt_grouped(const GroupCast *__restrict_
   q.mlo[e] = __bf16(v - float(h));
 }
 
-template <int RH>
+template <int RH, int KPAR>
 static void group_attr(size_t lds) {
-  XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3_grouped<RH>),
+  XF_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_m2l_bf16x3_grouped<RH, KPAR>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
 }
 
@@ -1517,7 +1535,8 @@ static void group_tables(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, i
   plan.rtb = rtb;
   plan.kr = kr;
   const int nrb = ceil_div(g.R, plan.rtb);
-  plan.lds = gemm_lds(g, true, plan.rtb, kr);
+  plan.tn = probs[0]->m2l_tn;
+  plan.lds = gemm_lds(g, true, plan.rtb, kr, plan.tn);
   std::vector<GroupGemm> d;
   std::vector<GroupCast> cd;
   std::vector<int> st, cst;
@@ -1525,6 +1544,7 @@ static void group_tables(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, i
   for (Ctx *cp : probs) {
     Ctx &c = *cp;
     if (c.P != P || c.cfg.m2l_precision != EXAFMM_PREC_BF16X3) fail(EXAFMM_ERR_ARG, "mixed orders in a GEMM group");
+    if (c.m2l_tn != plan.tn) fail(EXAFMM_ERR_ARG, "mixed tile widths in a GEMM group");
     if (!c.m2l_ready) fail(EXAFMM_ERR_STATE, "keyed M2L tiles missing");
     const int nt = c.m2l_ntiles;
     if (nt == 0) continue;
@@ -1574,14 +1594,14 @@ static void group_tables(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, i
   XF_HIP(hipMemcpy(plan.start.p, st.data(), sizeof(int) * st.size(), hipMemcpyHostToDevice));
   XF_HIP(hipMemcpy(plan.cast_start.p, cst.data(), sizeof(int) * cst.size(), hipMemcpyHostToDevice));
   switch (plan.rtb) {
-    case 1: group_attr<1>(plan.lds); break;
-    case 2: group_attr<2>(plan.lds); break;
-    case 3: group_attr<3>(plan.lds); break;
-    case 4: group_attr<4>(plan.lds); break;
-    case 5: group_attr<5>(plan.lds); break;
-    case 6: group_attr<6>(plan.lds); break;
-    case 7: group_attr<7>(plan.lds); break;
-    case 8: group_attr<8>(plan.lds); break;
+    case 1: plan.tn == tile_width<1>() ? group_attr<1, 1>(plan.lds) : group_attr<1, 2>(plan.lds); break;
+    case 2: plan.tn == tile_width<1>() ? group_attr<2, 1>(plan.lds) : group_attr<2, 2>(plan.lds); break;
+    case 3: plan.tn == tile_width<1>() ? group_attr<3, 1>(plan.lds) : group_attr<3, 2>(plan.lds); break;
+    case 4: plan.tn == tile_width<1>() ? group_attr<4, 1>(plan.lds) : group_attr<4, 2>(plan.lds); break;
+    case 5: plan.tn == tile_width<1>() ? group_attr<5, 1>(plan.lds) : group_attr<5, 2>(plan.lds); break;
+    case 6: plan.tn == tile_width<1>() ? group_attr<6, 1>(plan.lds) : group_attr<6, 2>(plan.lds); break;
+    case 7: plan.tn == tile_width<1>() ? group_attr<7, 1>(plan.lds) : group_attr<7, 2>(plan.lds); break;
+    case 8: plan.tn == tile_width<1>() ? group_attr<8, 1>(plan.lds) : group_attr<8, 2>(plan.lds); break;
     default: fail(EXAFMM_ERR_UNSUPPORTED, "expansion order");
   }
 }
@@ -1596,7 +1616,7 @@ void m2l_group_setup(const std::vector<Ctx *> &probs, GemmGroupPlan &plan, hipSt
   const int dev = probs[0]->cfg.device;
   std::vector<std::pair<int, int>> cand;  // (rtb, kr)
   for (int kr = 1; kr <= 3; kr++) {
-    const int rmax = rows_per_block(g, true, dev, kr);
+    const int rmax = rows_per_block(g, true, dev, kr, probs[0]->m2l_tn);
     if (rmax < 1) continue;
     int last = -1;
     for (int nb = ceil_div(g.R, rmax); nb <= ceil_div(g.R, rmax) + 1; nb++) {
@@ -1644,8 +1664,12 @@ void m2l_group_launch(const GemmGroupPlan &plan, hipStream_t s) {
   switch (plan.rtb) {
 #define XF_GROUP_CASE(n)                                                                                         \
   case n:                                                                                                         \
-    hipLaunchKernelGGL((k_m2l_bf16x3_grouped<n>), dim3(plan.nblocks), dim3(kGemmThreads), plan.lds, s, d,         \
-                       plan.start.p, plan.nprob, g.PP, g.R, plan.rtb, g.K32, KPb, g.P, plan.kr);                  \
+    if (plan.tn == tile_width<1>())                                                                              \
+      hipLaunchKernelGGL((k_m2l_bf16x3_grouped<n, 1>), dim3(plan.nblocks), dim3(kGemmThreads), plan.lds, s, d,    \
+                         plan.start.p, plan.nprob, g.PP, g.R, plan.rtb, g.K32, KPb, g.P, plan.kr);                \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_m2l_bf16x3_grouped<n, 2>), dim3(plan.nblocks), dim3(kGemmThreads), plan.lds, s, d,    \
+                         plan.start.p, plan.nprob, g.PP, g.R, plan.rtb, g.K32, KPb, g.P, plan.kr);                \
     break;
     XF_GROUP_CASE(1)
     XF_GROUP_CASE(2)

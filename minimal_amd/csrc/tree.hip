@@ -580,6 +580,13 @@ static int sort_depth(const int *lmax, int64_t n, int ncrit, bool forest, int lf
   return kMaxLevel;
 }
 
+void writeback_bodies(Ctx &c, hipStream_t s) {
+  const int64_t n = c.t.n;
+  hipLaunchKernelGGL(k_writeback, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, c.t.xs.p, c.bodies, c.t.misc.p);
+  XF_LAUNCH();
+  c.writeback_pending = false;
+}
+
 void tree_build(Ctx &c) {
   Tree &t = c.t;
   const int64_t n = t.n;
@@ -728,8 +735,10 @@ void tree_build(Ctx &c) {
                      t.leaves.p, t.misc.p + 17, t.cbeg.p, t.ccnt.p, t.cx.p, v1, c.bodies, t.perm.p, t.xs.p, t.xq.p,
                      t.body_leaf.p, ncrit);
   XF_LAUNCH();
-  hipLaunchKernelGGL(k_writeback, dim3(ceil_div(n, 256)), dim3(256), 0, s, n, t.xs.p, c.bodies, t.misc.p);
-  XF_LAUNCH();
+  if (c.defer_writeback)
+    c.writeback_pending = true;
+  else
+    writeback_bodies(c, s);
 
   // level sizes, flags, level maxima, root
   const int *lc = c.pin_i.p + kPinLevels, *fl = c.pin_i.p + kPinMisc;
