@@ -180,8 +180,12 @@ void do_downward(Ctx &c) {
 // at once), and runs beside the rest of the lists, the upward pass and the
 // M2L tile and operator build on s0, its long VALU-bound blocks filling the
 // CUs those latency-bound chains leave idle; the keyed GEMM follows on s0.
-// (Running the upward pass beside the traversal instead is slower: its
-// workgroups hold the CUs each short traversal launch waits for.)
+// A periodic tree launches P2P after the lists and the upward pass instead:
+// its far field adds a long latency-bound chain on s0 (the top levels' FP64
+// M2L per pair, the image ladder) that P2P started earlier would starve
+// (N = 2^20, images = 3: 6.5 ms against 6.9).  (Running the upward pass
+// beside the traversal is slower too: its workgroups hold the CUs each
+// short traversal launch waits for.)
 void do_passes(Ctx &c) {
   {
     // the bodies' write-back in tree order runs on s1 after P2P, off the
@@ -193,16 +197,27 @@ void do_passes(Ctx &c) {
     c.defer_writeback = true;
     do_build(c);
   }
-  do_lists(c, c, [&c] {
-    rec(c, 14, c.s0);
+  // P2P on s1 from event 14 (the lists' end on s0)
+  auto launch_p2p = [&c](const int *list_flags) {
     XF_HIP(hipStreamWaitEvent(c.s1, c.ev[14], 0));
     rec(c, 5, c.s1);
-    p2p_apply(c, c, c.s1, c.lists.tot.p + 3);
+    p2p_apply(c, c, c.s1, list_flags);
     rec(c, 6, c.s1);
     if (c.writeback_pending) writeback_bodies(c, c.s1);
     rec(c, 15, c.s1);
-  });
-  do_upward(c, c.s0);
+  };
+  if (c.cfg.images > 0) {
+    do_lists(c, c);
+    rec(c, 14, c.s0);
+    do_upward(c, c.s0);
+    launch_p2p(nullptr);
+  } else {
+    do_lists(c, c, [&] {
+      rec(c, 14, c.s0);
+      launch_p2p(c.lists.tot.p + 3);
+    });
+    do_upward(c, c.s0);
+  }
   do_far(c, c);
   do_downward(c);
 }
