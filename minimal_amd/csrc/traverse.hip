@@ -89,10 +89,15 @@ __device__ inline bool m2l_key(double4 Xc, int lc, double4 Xb, int lb, int img, 
   return true;
 }
 
+// Slots only go from empty to a key during a pass, so the probe reads at
+// workgroup scope (served from the CU's L1 once the few hundred live keys
+// are there): a stale read shows either the empty value, and the CAS then
+// returns the slot's real content, or another key, and the probe moves on;
+// it never shows a key the slot does not hold.
 __device__ inline int key_insert(unsigned long long *slots, int mask, unsigned long long key, int *flags) {
   int h = int(mix64(key) & (unsigned long long)mask);
   for (int probe = 0; probe <= mask; probe++) {
-    unsigned long long v = __atomic_load_n(&slots[h], __ATOMIC_RELAXED);
+    unsigned long long v = __hip_atomic_load(&slots[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (v == key) return h;
     if (v == kEmpty) {
       const unsigned long long prev = atomicCAS(&slots[h], kEmpty, key);
