@@ -188,6 +188,7 @@ struct Tree {
   double X0[3] = {0, 0, 0}, R0 = 0;
   int lroot = 0;                 // level of the (problem) roots: kForestLevel in a forest
   int ds_hint = -1;              // sort depth of the last build (-1: none yet)
+  int hist7_max = 0;             // largest level-7 prefix population of the last build
 
   Buf<double4> cx;     // centre xyz, radius R
   Buf<int> cbeg, ccnt; // body range in tree order

@@ -149,49 +149,84 @@ __global__ void __launch_bounds__(256) k_forest_bounds(const exafmm_body *__rest
 
 // ---------------------------------------------------------------- keys
 // Octant path of each body through the reference's centre sequence, and
-// the population of its level-7 prefix (dense, 8^7 bins: spread-out atomics).
-__global__ void __launch_bounds__(256) k_keys(const exafmm_body *__restrict__ b, int64_t n,
-                                              const double *__restrict__ root, unsigned long long *__restrict__ key,
-                                              int *__restrict__ val, int *__restrict__ hist7,
-                                              const double4 *__restrict__ froots, const int64_t *__restrict__ off,
-                                              int nprob) {
-  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
-  if (i >= n) return;
-  const double x = b[i].X[0], y = b[i].X[1], z = b[i].X[2];
-  double R0 = root[3], cx = root[0], cy = root[1], cz = root[2];
-  unsigned long long k = 0;
-  int l0 = 0;
-  if (froots) {
-    // forest: the problem's index fills the digits of levels 0 .. kForestLevel - 1,
-    // its own root's octant path the rest
-    int lo = 0, hi = nprob - 1;  // last problem with off[p] <= i
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (off[mid] <= i)
-        lo = mid;
-      else
-        hi = mid - 1;
+// the population of its level-7 prefix (dense, 8^7 bins).  With AGG the
+// workgroup first counts its 256 bodies' prefixes in an LDS table and adds
+// one atomic per distinct prefix: clustered inputs (a Plummer core puts
+// most bodies in a few hundred prefixes) would otherwise serialise on the
+// same global counters; spread-out inputs take the direct atomics.
+constexpr int kKeysThreads = 256;
+constexpr int kKeysSlots = 2 * kKeysThreads;  // load factor <= 1/2: every probe ends
+template <bool AGG>
+__global__ void __launch_bounds__(kKeysThreads) k_keys(const exafmm_body *__restrict__ b, int64_t n,
+                                                       const double *__restrict__ root,
+                                                       unsigned long long *__restrict__ key, int *__restrict__ val,
+                                                       int *__restrict__ hist7, const double4 *__restrict__ froots,
+                                                       const int64_t *__restrict__ off, int nprob) {
+  __shared__ unsigned tbin[AGG ? kKeysSlots : 1];  // prefix + 1 (0: empty)
+  __shared__ int tcnt[AGG ? kKeysSlots : 1];
+  if (AGG) {
+    for (int i = threadIdx.x; i < kKeysSlots; i += kKeysThreads) {
+      tbin[i] = 0u;
+      tcnt[i] = 0;
     }
-    const double4 Rt = froots[lo];
-    cx = Rt.x;
-    cy = Rt.y;
-    cz = Rt.z;
-    R0 = Rt.w;
-    k = (unsigned long long)lo;
-    l0 = kForestLevel;
+    __syncthreads();
   }
-  for (int l = l0; l < kMaxLevel; l++) {
-    const int bx = x > cx, by = y > cy, bz = z > cz;
-    const int oct = bx + (by << 1) + (bz << 2);
-    k = (k << 3) | unsigned(oct);
-    const double r = R0 / (1 << (l - l0 + 1));
-    cx += bx ? r : -r;
-    cy += by ? r : -r;
-    cz += bz ? r : -r;
+  const int64_t i = int64_t(blockIdx.x) * kKeysThreads + threadIdx.x;
+  if (i < n) {
+    const double x = b[i].X[0], y = b[i].X[1], z = b[i].X[2];
+    double R0 = root[3], cx = root[0], cy = root[1], cz = root[2];
+    unsigned long long k = 0;
+    int l0 = 0;
+    if (froots) {
+      // forest: the problem's index fills the digits of levels 0 .. kForestLevel - 1,
+      // its own root's octant path the rest
+      int lo = 0, hi = nprob - 1;  // last problem with off[p] <= i
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= i)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const double4 Rt = froots[lo];
+      cx = Rt.x;
+      cy = Rt.y;
+      cz = Rt.z;
+      R0 = Rt.w;
+      k = (unsigned long long)lo;
+      l0 = kForestLevel;
+    }
+    for (int l = l0; l < kMaxLevel; l++) {
+      const int bx = x > cx, by = y > cy, bz = z > cz;
+      const int oct = bx + (by << 1) + (bz << 2);
+      k = (k << 3) | unsigned(oct);
+      const double rr = R0 / (1 << (l - l0 + 1));
+      cx += bx ? rr : -rr;
+      cy += by ? rr : -rr;
+      cz += bz ? rr : -rr;
+    }
+    key[i] = k;
+    val[i] = int(i);
+    const unsigned bin = unsigned(k >> (3 * (kMaxLevel - kHistLevels)));
+    if (!AGG) {
+      atomicAdd(&hist7[bin], 1);
+    } else {
+      unsigned h = ((bin + 1u) * 2654435761u) & unsigned(kKeysSlots - 1);
+      for (;;) {
+        const unsigned prev = atomicCAS(&tbin[h], 0u, bin + 1u);
+        if (prev == 0u || prev == bin + 1u) {
+          atomicAdd(&tcnt[h], 1);
+          break;
+        }
+        h = (h + 1u) & unsigned(kKeysSlots - 1);
+      }
+    }
   }
-  key[i] = k;
-  val[i] = int(i);
-  atomicAdd(&hist7[int(k >> (3 * (kMaxLevel - kHistLevels)))], 1);
+  if (AGG) {
+    __syncthreads();
+    for (int j = threadIdx.x; j < kKeysSlots; j += kKeysThreads)
+      if (tcnt[j]) atomicAdd(&hist7[tbin[j] - 1u], tcnt[j]);
+  }
 }
 
 // Levels 6..3 of the prefix histogram from level 7 (prefixes are in Morton
@@ -626,7 +661,9 @@ void tree_build(Ctx &c) {
   t.val2.ensure(n);
   t.hist.ensure(kHistTotal);
   XF_HIP(hipMemsetAsync(t.hist.p + hist_off(kHistLevels), 0, sizeof(int) * (1 << (3 * kHistLevels)), s));
-  hipLaunchKernelGGL(k_keys, dim3(ceil_div(n, 256)), dim3(256), 0, s, c.bodies, n, root, t.key.p, t.val.p,
+  // crowded level-7 prefixes in the last tree: count them per workgroup first
+  hipLaunchKernelGGL(t.hist7_max > 64 ? k_keys<true> : k_keys<false>, dim3(ceil_div(n, kKeysThreads)),
+                     dim3(kKeysThreads), 0, s, c.bodies, n, root, t.key.p, t.val.p,
                      t.hist.p + hist_off(kHistLevels), froots, forest ? c.forest_off.p : nullptr, c.forest_n);
   XF_LAUNCH();
   // one block per level-3 prefix (4096 level-7 bins each)
@@ -749,15 +786,15 @@ void tree_build(Ctx &c) {
   XF_HIP(hipMemcpyAsync(c.pin_d.p, root, sizeof(double) * 4, hipMemcpyDeviceToHost, s));
   stream_wait(s);
   if (fl[0] & 1) fail(EXAFMM_ERR_ARG, "non-finite body coordinate or charge");
-  if (speculate) {
-    const int need = sort_depth(fl, n, ncrit, forest, lforce);
+  // The histogram fixes depths up to kHistLevels; a deeper tree was sorted
+  // at the last tree's depth + 1, which is enough unless a cell of that
+  // level still holds more than ncrit bodies (flag 2, below kMaxLevel).
+  const int need = sort_depth(fl, n, ncrit, forest, lforce);
+  const bool too_shallow = need <= kHistLevels ? need > Ds : (Ds < kMaxLevel && (fl[0] & 2));
+  if (too_shallow) {  // build again at the depth the histogram gives
     t.ds_hint = need;
-    if (need > Ds) {  // sorted too shallow: build again at the right depth
-      tree_build(c);
-      return;
-    }
-  } else {
-    t.ds_hint = Ds;
+    tree_build(c);
+    return;
   }
   for (int d = 0; d < 3; d++) t.X0[d] = rootv[d];
   t.R0 = rootv[3];
@@ -770,6 +807,8 @@ void tree_build(Ctx &c) {
     t.lstart.push_back(t.lstart.back() + lc[l]);
   }
   t.depth = depth;
+  t.ds_hint = need <= kHistLevels ? need : std::min(kMaxLevel, depth + 1);
+  t.hist7_max = fl[kHistLevels];
   t.ncells = t.lstart.back();
   const int nc = int(t.ncells);
   // every cell either split or is a leaf (the scan above wrote the same
