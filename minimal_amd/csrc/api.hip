@@ -1196,18 +1196,28 @@ namespace xf {
 // a body's problem by binary search over the offsets).
 __global__ void __launch_bounds__(256) k_forest_copy(const ForestPart *__restrict__ parts, int np, int64_t n,
                                                      exafmm_body *__restrict__ fb, int to_forest) {
+  // the workgroup's 32 bodies lie in one problem or a few consecutive ones:
+  // one search for the first body, then each thread steps forward from it
+  __shared__ int first;
   const int64_t g = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (threadIdx.x == 0) {
+    const int64_t i0 = (int64_t(blockIdx.x) * 256) >> 3;
+    int lo = 0, hi = np - 1;  // last problem with off <= i0
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (parts[mid].off <= i0)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    first = lo;
+  }
+  __syncthreads();
   const int64_t i = g >> 3;
   const int w = int(g & 7);
   if (i >= n) return;
-  int lo = 0, hi = np - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (parts[mid].off <= i)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
+  int lo = first;
+  while (lo + 1 < np && parts[lo + 1].off <= i) lo++;
   const ForestPart q = parts[lo];
   if (!q.src) return;  // a host problem: copied by the runtime
   double *f = reinterpret_cast<double *>(fb + i) + w;

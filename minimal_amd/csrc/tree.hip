@@ -100,10 +100,11 @@ __global__ void __launch_bounds__(256) k_bounds_final(const double *__restrict__
 // Forest: one workgroup per problem (bodies off[p] .. off[p+1] - 1), the
 // root of k_bounds_final per problem (min / max are exact, so any reduction
 // order gives the same root).  Empty problems get a unit root.
-__global__ void __launch_bounds__(256) k_forest_bounds(const exafmm_body *__restrict__ b,
+constexpr int kFBThreads = 1024;  // one workgroup per problem: up to ~10^5 bodies each
+__global__ void __launch_bounds__(kFBThreads) k_forest_bounds(const exafmm_body *__restrict__ b,
                                                        const int64_t *__restrict__ off,
                                                        double4 *__restrict__ roots, int *__restrict__ flags) {
-  __shared__ double s[6][256];
+  __shared__ double s[6][kFBThreads];
   const int p = blockIdx.x, t = threadIdx.x;
   const int64_t i0 = off[p], i1 = off[p + 1];
   if (i1 <= i0) {
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(256) k_forest_bounds(const exafmm_body *__rest
   double lo[3] = {b[i0].X[0], b[i0].X[1], b[i0].X[2]};
   double hi[3] = {lo[0], lo[1], lo[2]};
   int bad = 0;
-  for (int64_t i = i0 + t; i < i1; i += 256) {
+  for (int64_t i = i0 + t; i < i1; i += kFBThreads) {
     for (int d = 0; d < 3; d++) {
       const double x = b[i].X[d];
       if (!isfinite(x)) bad = 1;
@@ -128,7 +129,7 @@ __global__ void __launch_bounds__(256) k_forest_bounds(const exafmm_body *__rest
     s[3 + d][t] = hi[d];
   }
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
+  for (int o = kFBThreads / 2; o > 0; o >>= 1) {
     if (t < o)
       for (int d = 0; d < 3; d++) {
         s[d][t] = fmin(s[d][t], s[d][t + o]);
@@ -643,7 +644,7 @@ void tree_build(Ctx &c) {
   const int lforce = forest ? kForestLevel : 0;
   t.lroot = lforce;
   if (forest) {
-    hipLaunchKernelGGL(k_forest_bounds, dim3(c.forest_n), dim3(256), 0, s, c.bodies, c.forest_off.p,
+    hipLaunchKernelGGL(k_forest_bounds, dim3(c.forest_n), dim3(kFBThreads), 0, s, c.bodies, c.forest_off.p,
                        c.forest_roots.p, t.misc.p);
     XF_LAUNCH();
     const double unit[4] = {0.0, 0.0, 0.0, 1.0};
